@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the MI355X three-phase JSON parse.
+
+Default workload: the plain path on one device-resident standard-JSON
+document of 2.0e9 bytes (post-like objects, a depth-64 branch, every 8th
+1 MiB boundary inside a string).  A step is one full parse (UTF-8
+validation, tokenization, structure recognition) of that document, input
+resident in HBM and outputs left in HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--bytes N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "input GB/s and % HBM roofline, batched vs plain at 1 GPU; p50/p99 parse latency"
+HBM_PEAK_GBS = 8000.0
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_tokenize_config2_traffic.json")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sample_bytes: int):
+    """simdjson 3.2.1 ondemand iterate (stage 1) from the reference's vendored
+    sources, one core, on a bounded sample of the same generator; plus the
+    single-threaded oracle for information."""
+    import oracle
+    from cujson_amd import gen
+    sample = gen.config2_parallel(total_bytes=sample_bytes, seed=gen.SEED_BASE + 102, segments=16)
+    out = {"value": None, "unit": "GB/s", "cores": 1, "kind": "reference",
+           "sample": f"{sample_bytes / 1e6:.0f} MB document from the same generator, "
+                     f"simdjson 3.2.1 ondemand::parser::iterate (stage 1), best of 5"}
+    ref = oracle.ref_lib()
+    if ref is not None:
+        a, p = oracle._u8(sample)
+        import ctypes
+        err = ctypes.c_int()
+        ms = ref.sjr_iterate_ms(p, a.size, 5, ctypes.byref(err))
+        out["value"] = round(a.size / (ms * 1e-3) / 1e9, 3)
+        out["error_code"] = int(err.value)
+    else:
+        out["kind"] = "port"
+        out["sample"] = "simdjson build absent; oracle only"
+    # the oracle (a scalar transcription of the reference) on 32 MB
+    small = sample[: 32 << 20].copy()
+    small[-1] = ord("]")
+    t0 = time.perf_counter()
+    oracle.parse_unit(small, 0)
+    t1 = time.perf_counter()
+    out["oracle_port_gbs"] = round(small.size / (t1 - t0) / 1e9, 4)
+    if out["value"] is None:
+        out["value"] = out["oracle_port_gbs"]
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--bytes", type=int, default=2_000_000_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-bytes", type=int, default=512 << 20)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only: --gpus must be 1 "
+                                   "(the path shards by byte range with a tiny carry exchange; "
+                                   "multi-GPU is not built)"}))
+        sys.exit(2)
+
+    import cujson_amd as cj
+    from cujson_amd import gen
+    from cujson_amd.hip import DeviceBuffer, Event, synchronize
+
+    t0 = time.time()
+    doc = gen.config2_parallel(total_bytes=args.bytes)
+    n = doc.size
+    log(f"generated {n} bytes in {time.time() - t0:.1f}s")
+
+    ctx = cj.Context(0)
+    d_in = DeviceBuffer(((n + 63) // 64) * 64)
+    d_in.upload(doc)
+    cap = n // 4 + 4096
+    d_st = DeviceBuffer(cap * 4)
+    d_pr = DeviceBuffer(cap * 4)
+    stream = ctx.stream
+
+    def step():
+        rc = ctx.parse_device(d_in.ptr, n, d_st.ptr, d_pr.ptr, cap)
+        if rc != 0:
+            raise RuntimeError(f"parse_device: {cj.strerror(rc)}")
+
+    step()
+    rc, N, M, over = ctx.wait()
+    if over or rc != 0:
+        raise RuntimeError(f"first parse failed: rc={rc} overflow={over}")
+    log(f"N={N} M={M} rho={N / n:.4f}")
+    for _ in range(args.warmup):
+        step()
+    synchronize()
+
+    ctx.set_timing(True)
+    evs = [Event() for _ in range(args.steps + 1)]
+    synchronize()
+    wall0 = time.perf_counter()
+    evs[0].record(stream)
+    for k in range(args.steps):
+        step()
+        evs[k + 1].record(stream)
+    synchronize()
+    wall1 = time.perf_counter()
+    rc, N2, M2, over = ctx.wait()
+    assert rc == 0 and N2 == N and M2 == M and not over
+    tim = ctx.timings(args.steps)
+    ctx.set_timing(False)
+
+    step_ms = np.array([evs[k].elapsed_ms(evs[k + 1]) for k in range(args.steps)])
+    total_ms = evs[0].elapsed_ms(evs[-1])
+    ms_per_step = total_ms / args.steps
+    value = n / (ms_per_step * 1e-3) / 1e9
+    tok_ms = float(np.mean([t[0] for t in tim]))
+    p3_ms = float(np.mean([t[1] for t in tim]))
+
+    tok_bytes = n + 4 * N + 5 * M          # k_tokenize: read input, write rows it owns
+    path_bytes = n + 8 * (N + 2)            # the whole path's algorithmic bytes
+    achieved = tok_bytes / (tok_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        try:
+            traffic = json.load(open(PROFILE_TRAFFIC)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain standard JSON, one 2.0e9-byte document of post-like objects "
+                        "(depth-64 branch, every 8th 1 MiB boundary inside a string), "
+                        "device-resident, one parse per step",
+            "bytes": n, "structurals": int(N), "brackets": int(M),
+            "rho": round(N / n, 5), "path": "plain",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_tokenize",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes": int(tok_bytes),
+            "avg_ms": round(tok_ms, 4),
+        },
+        "path": {
+            "algorithmic_bytes": int(path_bytes),
+            "hbm_frac": round(path_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "input_frac_of_peak": round(value / HBM_PEAK_GBS, 4),
+            "tokenize_ms": round(tok_ms, 4),
+            "phase3_ms": round(p3_ms, 4),
+            "p50_ms": round(float(np.percentile(step_ms, 50)), 4),
+            "p99_ms": round(float(np.percentile(step_ms, 99)), 4),
+            "wall_ms_per_step": round((wall1 - wall0) * 1e3 / args.steps, 4),
+        },
+    }
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample_bytes)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

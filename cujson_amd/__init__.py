@@ -1,0 +1,367 @@
+"""cujson_amd - MI355X-native three-phase JSON parse (UTF-8 validation,
+tokenization, nesting-structure recognition) behind cuJSON's interface.
+
+Python view of the C ABI in ``include/cujson_amd.h`` (``libcujson_amd.so``,
+built in-tree by ``cujson_amd/build.py``).  The names mirror the reference's
+C++ API so code written against cuJSON reads the same:
+
+=============================================  ==========================================
+reference (cujson/)                             here
+=============================================  ==========================================
+``loadJSON`` load_file.h:9                      :func:`loadJSON`
+``loadJSONLines_chunkCount`` load_file.h:10     :func:`loadJSONLines_chunkCount`
+``loadJSONLines_chunkSizeBytes`` :11            :func:`loadJSONLines_chunkSizeBytes`
+``loadJSONLines_chunkSizeMegaBytes`` :12        :func:`loadJSONLines_chunkSizeMegaBytes`
+``parse_standard_json`` parse_standard_json.h:9 :func:`parse_standard_json`
+``parse_json_lines`` parse_json_lines.h:9       :func:`parse_json_lines`
+``cuJSONResult`` cujson_types.h:22-33           :class:`cuJSONResult`
+=============================================  ==========================================
+
+Error behaviour: where the reference returns an empty ``cuJSONResult{}``
+(bad arguments) these functions return a result with ``structural is None``;
+where the reference calls ``exit(0)`` (invalid UTF-8, invalid structure) they
+return a result with ``structural is None`` and ``error`` set, instead of
+ending the process.  There is no CPU fallback: every parse runs the HIP
+kernels, and importing the package fails if the library is not built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcujson_amd.so")
+
+CJX_OK = 0
+CJX_ERR_INPUT = 1
+CJX_ERR_UTF8 = 2
+CJX_ERR_STRUCTURE = 3
+CJX_ERR_ALLOC = 4
+CJX_ERR_DEVICE = 5
+CJX_ERR_CAPACITY = 6
+
+CJX_FLAG_UTF8_STRICT = 0x1
+CJX_FLAG_DEPTH_U16 = 0x2
+CJX_FLAG_JSONL = 0x4
+
+# Every symbol include/cujson_amd.h declares.
+ABI_SYMBOLS = (
+    "cjx_context_create", "cjx_context_destroy", "cjx_load", "cjx_free_input",
+    "cjx_chunk_lines", "cjx_parse_standard", "cjx_parse_lines", "cjx_free_result",
+    "cjx_strerror", "cjx_parse_device", "cjx_device_wait", "cjx_context_stream",
+    "cjx_version", "cjx_context_set_timing", "cjx_context_timings",
+)
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [
+        ("buf", ctypes.POINTER(ctypes.c_int32)),
+        ("structural", ctypes.POINTER(ctypes.c_int32)),
+        ("pair_pos", ctypes.POINTER(ctypes.c_int32)),
+        ("n_structural", ctypes.c_int64),
+        ("n_brackets", ctypes.c_int64),
+        ("chunk_count", ctypes.c_int32),
+        ("result_sizes", ctypes.POINTER(ctypes.c_int32)),
+        ("result_sizes_prefix", ctypes.POINTER(ctypes.c_int32)),
+        ("total_result_size", ctypes.c_int32),
+        ("file_size", ctypes.c_int32),
+        ("error", ctypes.c_int32),
+        ("error_chunk", ctypes.c_int32),
+        ("device_ms", ctypes.c_double),
+    ]
+
+
+class _Input(ctypes.Structure):
+    _fields_ = [("data", ctypes.POINTER(ctypes.c_uint8)), ("size", ctypes.c_size_t)]
+
+
+class _Timings(ctypes.Structure):
+    _fields_ = [("tokenize_ms", ctypes.c_float), ("phase3_ms", ctypes.c_float),
+                ("total_ms", ctypes.c_float)]
+
+
+class _DeviceStatus(ctypes.Structure):
+    _fields_ = [("n_structural", ctypes.c_int64), ("n_brackets", ctypes.c_int64),
+                ("error", ctypes.c_int32), ("capacity_exceeded", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded C ABI library.  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m cujson_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    szp = ctypes.POINTER(ctypes.c_size_t)
+    L.cjx_context_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    L.cjx_context_create.restype = ctypes.c_int
+    L.cjx_context_destroy.argtypes = [vp]
+    L.cjx_context_destroy.restype = None
+    L.cjx_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(_Input)]
+    L.cjx_load.restype = ctypes.c_int
+    L.cjx_free_input.argtypes = [ctypes.POINTER(_Input)]
+    L.cjx_free_input.restype = None
+    L.cjx_chunk_lines.argtypes = [u8p, sz, ctypes.c_int, sz, szp, szp, sz, szp]
+    L.cjx_chunk_lines.restype = ctypes.c_int
+    L.cjx_parse_standard.argtypes = [vp, u8p, sz, ctypes.c_uint, ctypes.POINTER(_Result)]
+    L.cjx_parse_standard.restype = ctypes.c_int
+    L.cjx_parse_lines.argtypes = [vp, u8p, sz, szp, szp, sz, ctypes.c_uint,
+                                  ctypes.POINTER(_Result)]
+    L.cjx_parse_lines.restype = ctypes.c_int
+    L.cjx_free_result.argtypes = [ctypes.POINTER(_Result)]
+    L.cjx_free_result.restype = None
+    L.cjx_strerror.argtypes = [ctypes.c_int]
+    L.cjx_strerror.restype = ctypes.c_char_p
+    L.cjx_parse_device.argtypes = [vp, vp, sz, ctypes.c_uint, vp, vp, sz]
+    L.cjx_parse_device.restype = ctypes.c_int
+    L.cjx_device_wait.argtypes = [vp, ctypes.POINTER(_DeviceStatus)]
+    L.cjx_device_wait.restype = ctypes.c_int
+    L.cjx_context_stream.argtypes = [vp]
+    L.cjx_context_stream.restype = vp
+    L.cjx_context_set_timing.argtypes = [vp, ctypes.c_int]
+    L.cjx_context_set_timing.restype = ctypes.c_int
+    L.cjx_context_timings.argtypes = [vp, ctypes.POINTER(_Timings), ctypes.c_int]
+    L.cjx_context_timings.restype = ctypes.c_int
+    L.cjx_version.argtypes = []
+    L.cjx_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def strerror(code: int) -> str:
+    return lib().cjx_strerror(code).decode()
+
+
+class Context:
+    """One HIP stream plus the device buffers of the parse (cjx_context)."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        rc = lib().cjx_context_create(device, ctypes.byref(self._h))
+        if rc != CJX_OK:
+            raise RuntimeError(f"cjx_context_create failed: {strerror(rc)}")
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self) -> int:
+        return lib().cjx_context_stream(self._h) or 0
+
+    def close(self):
+        if self._h:
+            lib().cjx_context_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- device-resident form ------------------------------------------------
+    def parse_device(self, d_input: int, n: int, d_structural: int, d_pair: int, cap: int,
+                     flags: int = 0) -> int:
+        """Enqueue a parse of a device-resident document (pointers are ints)."""
+        return lib().cjx_parse_device(self._h, ctypes.c_void_p(d_input), n, flags,
+                                      ctypes.c_void_p(d_structural), ctypes.c_void_p(d_pair), cap)
+
+    def set_timing(self, on: bool = True):
+        lib().cjx_context_set_timing(self._h, 1 if on else 0)
+
+    def timings(self, max_entries: int = 1):
+        """[(tokenize_ms, phase3_ms, total_ms)] of the most recent parses, oldest
+        first (timing enabled); the default returns the last parse only."""
+        arr = (_Timings * max_entries)()
+        k = lib().cjx_context_timings(self._h, arr, max_entries)
+        if k < 0:
+            raise RuntimeError("timings unavailable (timing not enabled?)")
+        out = [(arr[i].tokenize_ms, arr[i].phase3_ms, arr[i].total_ms) for i in range(k)]
+        return out if max_entries > 1 else out[0]
+
+    def wait(self):
+        st = _DeviceStatus()
+        rc = lib().cjx_device_wait(self._h, ctypes.byref(st))
+        return rc, st.n_structural, st.n_brackets, bool(st.capacity_exceeded)
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+@dataclass
+class cuJSONInput:
+    """cuJSONInput (cujson/cujson_types.h:9-12); ``data`` is a numpy uint8 view."""
+    data: Optional[np.ndarray]
+    size: int
+    _pinned: object = None
+
+
+@dataclass
+class cuJSONLinesInput:
+    """cuJSONLinesInput (cujson/cujson_types.h:14-20); chunks as (offset, size)."""
+    data: Optional[np.ndarray]
+    chunkCount: int
+    size: int
+    chunks: List[int] = field(default_factory=list)       # chunk start offsets
+    chunksSize: List[int] = field(default_factory=list)
+    _pinned: object = None
+
+
+class cuJSONResult:
+    """cuJSONResult (cujson/cujson_types.h:22-33) over the pinned result buffer.
+
+    ``structural`` and ``pair_pos`` are int32 numpy views into one buffer of
+    2(N+2) cells with ``pair_pos = buf + N + 1`` exactly as in the reference."""
+
+    def __init__(self, r: _Result):
+        self._r = r
+        self.error = int(r.error)
+        self.error_chunk = int(r.error_chunk)
+        self.device_ms = float(r.device_ms)
+        self.chunkCount = int(r.chunk_count)
+        self.bufferSize = 0
+        self.depth = 0
+        self.inputJSON = None
+        N = int(r.n_structural)
+        self.n_brackets = int(r.n_brackets)
+        if r.buf:
+            cells = 2 * (N + 2)
+            self.buf = np.ctypeslib.as_array(r.buf, shape=(cells,))
+            self.structural = self.buf[: N + 2]
+            self.pair_pos = self.buf[N + 1:]
+            self.totalResultSize = int(r.total_result_size)
+            self.fileSize = int(r.file_size)
+            k = self.chunkCount
+            self.resultSizes = list(np.ctypeslib.as_array(r.result_sizes, shape=(k,))) if k else []
+            self.resultSizesPrefix = (list(np.ctypeslib.as_array(r.result_sizes_prefix, shape=(k,)))
+                                      if k else [])
+        else:
+            self.buf = None
+            self.structural = None
+            self.pair_pos = None
+            self.totalResultSize = 0
+            self.fileSize = 0
+            self.resultSizes = []
+            self.resultSizesPrefix = []
+        self.N = N
+
+    def free(self):
+        if self._r is not None:
+            lib().cjx_free_result(ctypes.byref(self._r))
+            self._r = None
+            self.buf = self.structural = self.pair_pos = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _as_u8(data):
+    if isinstance(data, cuJSONInput) or isinstance(data, cuJSONLinesInput):
+        data = data.data
+    if data is None:
+        return None, None
+    a = np.frombuffer(data, dtype=np.uint8) if not isinstance(data, np.ndarray) else data
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+def loadJSON(filePath: str) -> cuJSONInput:
+    """load_file.cu:7-36: file -> pinned host buffer ({None, 0} on failure)."""
+    inp = _Input()
+    rc = lib().cjx_load(filePath.encode(), ctypes.byref(inp))
+    if rc != CJX_OK or not inp.data:
+        return cuJSONInput(None, 0)
+    arr = np.ctypeslib.as_array(inp.data, shape=(max(inp.size, 1),))[: inp.size]
+    return cuJSONInput(arr, int(inp.size), _PinnedInput(inp))
+
+
+class _PinnedInput:
+    def __init__(self, inp: _Input):
+        self.inp = inp
+
+    def __del__(self):
+        try:
+            lib().cjx_free_input(ctypes.byref(self.inp))
+        except Exception:
+            pass
+
+
+def chunk_lines(data, mode: int, param: int):
+    """Line-aligned chunking (load_file.cu:71-103 count mode 0, :137-176 bytes mode 1)."""
+    a, p = _as_u8(data)
+    n = ctypes.c_size_t()
+    lib().cjx_chunk_lines(p, a.size, mode, param, None, None, 0, ctypes.byref(n))
+    k = n.value
+    offs = (ctypes.c_size_t * max(k, 1))()
+    sizes = (ctypes.c_size_t * max(k, 1))()
+    rc = lib().cjx_chunk_lines(p, a.size, mode, param, offs, sizes, k, ctypes.byref(n))
+    if rc != CJX_OK:
+        raise ValueError(strerror(rc))
+    return [int(offs[i]) for i in range(k)], [int(sizes[i]) for i in range(k)]
+
+
+def _lines_input(inp: cuJSONInput, mode: int, param: int) -> cuJSONLinesInput:
+    if inp.data is None:
+        return cuJSONLinesInput(None, 0, 0)
+    offs, sizes = chunk_lines(inp.data, mode, param)
+    return cuJSONLinesInput(inp.data, len(offs), inp.size, offs, sizes, inp._pinned)
+
+
+def loadJSONLines_chunkCount(filePath: str, chunkCount: int = 1) -> cuJSONLinesInput:
+    return _lines_input(loadJSON(filePath), 0, chunkCount)
+
+
+def loadJSONLines_chunkSizeBytes(filePath: str, chunkSizeBytes: int) -> cuJSONLinesInput:
+    return _lines_input(loadJSON(filePath), 1, chunkSizeBytes)
+
+
+def loadJSONLines_chunkSizeMegaBytes(filePath: str, chunkSizeMegaBytes: int) -> cuJSONLinesInput:
+    return _lines_input(loadJSON(filePath), 1, chunkSizeMegaBytes * 1024 * 1024)
+
+
+def parse_standard_json(input, flags: int = 0, ctx: Optional[Context] = None) -> cuJSONResult:
+    """parse_standard_json (parse_standard_json.cu:1584-1687) on the GPU."""
+    a, p = _as_u8(input)
+    ctx = ctx or default_context()
+    r = _Result()
+    n = 0 if a is None else a.size
+    lib().cjx_parse_standard(ctx.handle, p, n, flags, ctypes.byref(r))
+    return cuJSONResult(r)
+
+
+def parse_json_lines(input: cuJSONLinesInput, flags: int = 0,
+                     ctx: Optional[Context] = None) -> cuJSONResult:
+    """parse_json_lines (parse_json_lines.cu:1092-1264) on the GPU."""
+    a, p = _as_u8(input.data)
+    ctx = ctx or default_context()
+    k = len(input.chunks)
+    offs = (ctypes.c_size_t * max(k, 1))(*input.chunks)
+    sizes = (ctypes.c_size_t * max(k, 1))(*input.chunksSize)
+    r = _Result()
+    n = 0 if a is None else a.size
+    nch = min(input.chunkCount, k)
+    if input.chunkCount > k:  # metadata smaller than chunkCount, parse_json_lines.cu:1118-1121
+        nch = 0
+    lib().cjx_parse_lines(ctx.handle, p, n, offs, sizes, nch, flags, ctypes.byref(r))
+    return cuJSONResult(r)

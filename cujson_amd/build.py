@@ -1,0 +1,75 @@
+"""Builds libcujson_amd.so (HIP kernels + C ABI) in-tree for gfx950.
+
+Usage: python -m cujson_amd.build   (or __graft_entry__.build())
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libcujson_amd.so")
+ARCH = os.environ.get("CJX_OFFLOAD_ARCH", "gfx950")
+
+SOURCES = ["cjx_kernels.hip", "cjx_api.cpp"]
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    # packed fp32 VALU ops are disabled for every HIP object of this library
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and "unknown target feature" not in l)
+    if r.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError(f"build step failed: {' '.join(cmd)}")
+    return out
+
+
+def build_library(verbose: bool = False) -> str:
+    objs = []
+    build_dir = os.path.join(HERE, "_build")
+    os.makedirs(build_dir, exist_ok=True)
+    for src in SOURCES:
+        obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
+        cmd = ["hipcc", *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-c",
+               os.path.join(CSRC, src), "-o", obj]
+        if src.endswith(".cpp"):
+            cmd.insert(1, "-x")
+            cmd.insert(2, "hip")
+        out = _run(cmd)
+        if verbose and out:
+            print(out)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+GEN_LIB = os.path.join(HERE, "libcjx_gen.so")
+
+
+def build_generator() -> str:
+    tmp = GEN_LIB + ".tmp"
+    _run(["gcc", "-O2", "-std=c11", "-Wall", "-fPIC", "-shared", "-o", tmp,
+          os.path.join(CSRC, "cjx_gen.c"), "-lm"])
+    os.replace(tmp, GEN_LIB)
+    return GEN_LIB
+
+
+def build_oracle() -> None:
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"], check=True)
+
+
+if __name__ == "__main__":
+    print(build_library(verbose=True))
+    print(build_generator())
+    build_oracle()

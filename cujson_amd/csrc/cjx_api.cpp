@@ -1,0 +1,585 @@
+// cjx_api.cpp - host driver and C ABI of the MI355X JSON parse.
+//
+// Orchestration replaces parse_standard_json (cujson/parse_standard_json.cu:1584-1687)
+// and parse_json_lines (cujson/parse_json_lines.cu:1092-1264): one H2D copy,
+// one tokenizer launch over every unit, the phase-3 kernels per unit, one
+// device synchronization, and D2H copies straight into the reference's host
+// layout.  No host round trip sits between the phases: every count the later
+// kernels need is read on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/cujson_amd.h"
+#include "cjx_internal.h"
+
+using namespace cjx;
+
+namespace {
+
+constexpr uint64_t kMaxLaunchBytes = 0x7FFFFFFFull;  // positions and indices stay in int32
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    // Grows (never shrinks); contents are not preserved.
+    bool ensure(size_t bytes) {
+        if (bytes <= cap) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = bytes + bytes / 8 + 4096;
+        if (hipMalloc(&p, want) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        cap = want;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    bool ensure(size_t bytes) {
+        if (bytes <= cap) return true;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        cap = bytes;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+}  // namespace
+
+struct cjx_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Timing ring: event triples (tokenizer start, tokenizer end, phase-3 end)
+    // for the last kTimingRing parses while timing is enabled.
+    static constexpr int kTimingRing = 256;
+    hipEvent_t tev[kTimingRing][3] = {};
+    uint64_t timing_count = 0;
+    bool timing = false;
+    DevBuf in, structural, pair;
+    DevBuf units, uout, ls, carry_st, count_st;
+    DevBuf oc_char, oc_idx, keys, sorted, tmp, spos, block_delta, hist;
+    PinnedBuf h_units, h_uout, h_ls;
+    // Shape of the launch in flight (device form).
+    uint32_t n_units = 0;
+    bool events_recorded = false;
+};
+
+namespace {
+
+int set_device(cjx_context *ctx) {
+    return hipSetDevice(ctx->device) == hipSuccess ? CJX_OK : CJX_ERR_DEVICE;
+}
+
+uint32_t phase3_blocks(uint64_t max_m) {
+    uint64_t g = (max_m + 65535) / 65536;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 512));
+}
+
+uint32_t pair_blocks(uint64_t max_n) {
+    uint64_t g = (max_n + kPairChunk - 1) / kPairChunk;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 2048));
+}
+
+// Units are given as (host offset, size); they are placed in the device
+// input at 16-byte aligned offsets.  Fills descriptors and returns the tile
+// count, or 0 on invalid shape.
+uint32_t plan_units(const std::vector<uint64_t> &sizes, const std::vector<uint64_t> &pos_base,
+                    std::vector<UnitDesc> &units, uint64_t &dev_bytes) {
+    units.resize(sizes.size());
+    uint64_t off = 0, tiles = 0;
+    for (size_t i = 0; i < sizes.size(); i++) {
+        UnitDesc &u = units[i];
+        u.in_off = off;
+        u.n = sizes[i];
+        u.pos_base = pos_base[i];
+        u.tile0 = (uint32_t)tiles;
+        u.ntiles = (uint32_t)((sizes[i] + kTileBytes - 1) / kTileBytes);
+        tiles += u.ntiles;
+        off += (sizes[i] + 15) & ~15ull;
+    }
+    dev_bytes = off + 64;
+    if (tiles > 0xFFFFFFF0ull) return 0;
+    return (uint32_t)tiles;
+}
+
+struct Shape {
+    uint32_t ntiles, n_units;
+    uint64_t n_cap, m_cap, max_unit;
+};
+
+int ensure_work(cjx_context *ctx, const Shape &sh) {
+    bool ok = ctx->units.ensure(sh.n_units * sizeof(UnitDesc)) &&
+              ctx->uout.ensure(sh.n_units * sizeof(UnitOut)) &&
+              ctx->ls.ensure(sizeof(LaunchState)) &&
+              ctx->carry_st.ensure((size_t)sh.ntiles * 4) &&
+              ctx->count_st.ensure((size_t)sh.ntiles * 8) && ctx->oc_char.ensure(sh.m_cap + 16) &&
+              ctx->oc_idx.ensure(sh.m_cap * 4 + 16) && ctx->keys.ensure(sh.m_cap * 2 + 16) &&
+              ctx->sorted.ensure(sh.m_cap * 4 + 16) && ctx->spos.ensure(sh.m_cap * 4 + 16) &&
+              ctx->block_delta.ensure(512 * 4) && ctx->hist.ensure(512 * 256 * 4) &&
+              ctx->h_units.ensure(sh.n_units * sizeof(UnitDesc)) &&
+              ctx->h_uout.ensure(sh.n_units * sizeof(UnitOut)) &&
+              ctx->h_ls.ensure(sizeof(LaunchState));
+    return ok ? CJX_OK : CJX_ERR_ALLOC;
+}
+
+LaunchBufs bufs_of(cjx_context *ctx) {
+    LaunchBufs b;
+    b.units = ctx->units.as<UnitDesc>();
+    b.uout = ctx->uout.as<UnitOut>();
+    b.ls = ctx->ls.as<LaunchState>();
+    b.carry_st = ctx->carry_st.as<uint32_t>();
+    b.count_st = ctx->count_st.as<unsigned long long>();
+    b.oc_char = ctx->oc_char.as<uint8_t>();
+    b.oc_idx = ctx->oc_idx.as<uint32_t>();
+    b.keys = ctx->keys.as<uint16_t>();
+    b.sorted = ctx->sorted.as<uint32_t>();
+    b.tmp = ctx->tmp.p ? ctx->tmp.as<uint32_t>() : nullptr;
+    b.spos = ctx->spos.as<uint32_t>();
+    b.block_delta = ctx->block_delta.as<int32_t>();
+    b.hist = ctx->hist.as<uint32_t>();
+    return b;
+}
+
+// Enqueues the whole parse of `units` (already in ctx->units on the device)
+// on the context stream.
+int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsigned flags,
+                  uint32_t *d_struct, int32_t *d_pair, bool local_pairs) {
+    const bool u16 = (flags & CJX_FLAG_DEPTH_U16) != 0;
+    if (u16 && !ctx->tmp.ensure(sh.m_cap * 4 + 16)) return CJX_ERR_ALLOC;
+    LaunchBufs b = bufs_of(ctx);
+    hipStream_t s = ctx->stream;
+    if (launch_reset(s, b, sh.ntiles, sh.n_units) != hipSuccess) return CJX_ERR_DEVICE;
+    hipEvent_t *tev = ctx->tev[ctx->timing_count % cjx_context::kTimingRing];
+    if (ctx->timing) (void)hipEventRecord(tev[0], s);
+    if (launch_tokenize(s, b, d_in, sh.ntiles, sh.n_units, (flags & CJX_FLAG_JSONL) != 0,
+                        d_struct, (uint32_t)std::min<uint64_t>(sh.n_cap, 0xFFFFFFFFull),
+                        (uint32_t)std::min<uint64_t>(sh.m_cap, 0xFFFFFFFFull)) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    if (ctx->timing) (void)hipEventRecord(tev[1], s);
+    const uint32_t G = phase3_blocks(std::min(sh.max_unit, sh.m_cap));
+    const uint32_t PG = pair_blocks(std::min(sh.max_unit, sh.n_cap));
+    for (uint32_t u = 0; u < sh.n_units; u++)
+        if (launch_phase3(s, b, u, G, PG, u16, d_pair, local_pairs) != hipSuccess)
+            return CJX_ERR_DEVICE;
+    if (ctx->timing) {
+        (void)hipEventRecord(tev[2], s);
+        ctx->timing_count++;
+    }
+    return CJX_OK;
+}
+
+int read_status(cjx_context *ctx, uint32_t n_units) {
+    hipStream_t s = ctx->stream;
+    if (hipMemcpyAsync(ctx->h_uout.p, ctx->uout.p, n_units * sizeof(UnitOut),
+                       hipMemcpyDeviceToHost, s) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    if (hipMemcpyAsync(ctx->h_ls.p, ctx->ls.p, sizeof(LaunchState), hipMemcpyDeviceToHost, s) !=
+        hipSuccess)
+        return CJX_ERR_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return CJX_ERR_DEVICE;
+    return CJX_OK;
+}
+
+int unit_code(const UnitOut &o) {
+    if (o.utf8_err) return CJX_ERR_UTF8;
+    if (o.struct_err) return CJX_ERR_STRUCTURE;
+    return CJX_OK;
+}
+
+void zero_result(cjx_result *out) {
+    std::memset(out, 0, sizeof(*out));
+    out->error_chunk = -1;
+}
+
+// Parses units whose bytes are on the host: copies them in, parses, and
+// assembles the reference's host layout.  Retries once with full capacity
+// when the first capacity guess is exceeded.
+int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint64_t> &offs,
+                     const std::vector<uint64_t> &sizes, const std::vector<uint64_t> &pos_base,
+                     unsigned flags, cjx_result *out) {
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    std::vector<UnitDesc> units;
+    uint64_t dev_bytes = 0;
+    const uint32_t ntiles = plan_units(sizes, pos_base, units, dev_bytes);
+    uint64_t total = 0, max_unit = 0;
+    for (uint64_t z : sizes) {
+        total += z;
+        max_unit = std::max(max_unit, z);
+    }
+    if (ntiles == 0 || total > kMaxLaunchBytes) return CJX_ERR_INPUT;
+    if (!ctx->in.ensure(dev_bytes)) return CJX_ERR_ALLOC;
+    hipStream_t s = ctx->stream;
+    for (size_t i = 0; i < units.size(); i++)
+        if (hipMemcpyAsync(ctx->in.as<uint8_t>() + units[i].in_off, data + offs[i], sizes[i],
+                           hipMemcpyHostToDevice, s) != hipSuccess)
+            return CJX_ERR_DEVICE;
+
+    Shape sh{ntiles, (uint32_t)units.size(), 0, 0, max_unit};
+    uint64_t guess = total / 3 + 4096;  // typical JSON has rho < 0.15
+    for (int attempt = 0; attempt < 2; attempt++) {
+        sh.n_cap = attempt == 0 ? std::min<uint64_t>(guess, total + 16) : total + 16;
+        sh.m_cap = sh.n_cap;
+        if ((rc = ensure_work(ctx, sh))) return rc;
+        if (!ctx->structural.ensure(sh.n_cap * 4 + 16) || !ctx->pair.ensure(sh.n_cap * 4 + 16))
+            return CJX_ERR_ALLOC;
+        std::memcpy(ctx->h_units.p, units.data(), units.size() * sizeof(UnitDesc));
+        if (hipMemcpyAsync(ctx->units.p, ctx->h_units.p, units.size() * sizeof(UnitDesc),
+                           hipMemcpyHostToDevice, s) != hipSuccess)
+            return CJX_ERR_DEVICE;
+        (void)hipEventRecord(ctx->ev0, s);
+        rc = enqueue_parse(ctx, ctx->in.as<uint8_t>(), sh, flags, ctx->structural.as<uint32_t>(),
+                           ctx->pair.as<int32_t>(), false);
+        if (rc) return rc;
+        (void)hipEventRecord(ctx->ev1, s);
+        if ((rc = read_status(ctx, sh.n_units))) return rc;
+        const LaunchState *ls = static_cast<const LaunchState *>(ctx->h_ls.p);
+        if (!ls->overflow) break;
+        if (attempt == 1) return CJX_ERR_CAPACITY;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+
+    const UnitOut *uo = static_cast<const UnitOut *>(ctx->h_uout.p);
+    const LaunchState *ls = static_cast<const LaunchState *>(ctx->h_ls.p);
+    const uint64_t N = ls->n_total;
+    out->n_structural = (int64_t)N;
+    out->n_brackets = (int64_t)ls->m_total;
+    out->chunk_count = (int32_t)units.size();
+    out->device_ms = ms;
+    out->error = CJX_OK;
+    out->error_chunk = -1;
+    for (size_t i = 0; i < units.size(); i++) {
+        int c = unit_code(uo[i]);
+        if (c != CJX_OK) {
+            out->error = c;
+            out->error_chunk = (int32_t)i;
+            break;
+        }
+    }
+    // Host layout, parse_standard_json.cu:1659-1680 / parse_json_lines.cu:1070-1089.
+    int32_t *buf = nullptr;
+    const size_t cells = 2 * (N + 2);
+    if (hipHostMalloc((void **)&buf, cells * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+        return CJX_ERR_ALLOC;
+    if (N) {
+        if (hipMemcpyAsync(buf + 1, ctx->structural.p, N * 4, hipMemcpyDeviceToHost, s) !=
+                hipSuccess ||
+            hipMemcpyAsync(buf + N + 2, ctx->pair.p, N * 4, hipMemcpyDeviceToHost, s) !=
+                hipSuccess) {
+            (void)hipHostFree(buf);
+            return CJX_ERR_DEVICE;
+        }
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipHostFree(buf);
+        return CJX_ERR_DEVICE;
+    }
+    buf[0] = 0;
+    buf[N + 1] = (int32_t)(N + 1);
+    buf[2 * N + 2] = -1;
+    buf[2 * N + 3] = -1;
+    out->buf = buf;
+    out->structural = buf;
+    out->pair_pos = buf + N + 1;
+    out->total_result_size = (int32_t)(N + 2);
+    out->file_size = (int32_t)(N + 2);
+    out->result_sizes = (int32_t *)std::malloc(units.size() * sizeof(int32_t));
+    out->result_sizes_prefix = (int32_t *)std::malloc(units.size() * sizeof(int32_t));
+    for (size_t i = 0; i < units.size(); i++) {
+        out->result_sizes[i] = (int32_t)(uo[i].n_end - uo[i].n_base);
+        out->result_sizes_prefix[i] = (int32_t)uo[i].n_end;
+    }
+    return out->error;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cjx_version(void) { return "cujson_amd 0.1 (gfx950)"; }
+
+const char *cjx_strerror(int code) {
+    switch (code) {
+        case CJX_OK: return "ok";
+        case CJX_ERR_INPUT: return "invalid input";
+        case CJX_ERR_UTF8: return "invalid UTF-8";
+        case CJX_ERR_STRUCTURE: return "invalid JSON structure";
+        case CJX_ERR_ALLOC: return "allocation failed";
+        case CJX_ERR_DEVICE: return "device error";
+        case CJX_ERR_CAPACITY: return "output capacity exceeded";
+        default: return "unknown error";
+    }
+}
+
+int cjx_context_create(int device, cjx_context **out) {
+    if (!out) return CJX_ERR_INPUT;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CJX_ERR_DEVICE;
+    if (device < 0 || device >= ndev) return CJX_ERR_INPUT;
+    cjx_context *ctx = new cjx_context();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        delete ctx;
+        return CJX_ERR_DEVICE;
+    }
+    *out = ctx;
+    return CJX_OK;
+}
+
+void cjx_context_destroy(cjx_context *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    DevBuf *bufs[] = {&ctx->in,       &ctx->structural, &ctx->pair,     &ctx->units,
+                      &ctx->uout,     &ctx->ls,         &ctx->carry_st, &ctx->count_st,
+                      &ctx->oc_char,  &ctx->oc_idx,     &ctx->keys,     &ctx->sorted,
+                      &ctx->tmp,      &ctx->spos,       &ctx->block_delta, &ctx->hist};
+    for (DevBuf *b : bufs) b->release();
+    ctx->h_units.release();
+    ctx->h_uout.release();
+    ctx->h_ls.release();
+    for (auto &tr : ctx->tev)
+        for (hipEvent_t e : tr)
+            if (e) (void)hipEventDestroy(e);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+void *cjx_context_stream(cjx_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int cjx_context_set_timing(cjx_context *ctx, int enable) {
+    if (!ctx) return CJX_ERR_INPUT;
+    if (enable && !ctx->tev[0][0]) {
+        if (set_device(ctx)) return CJX_ERR_DEVICE;
+        for (auto &tr : ctx->tev)
+            for (hipEvent_t &e : tr)
+                if (hipEventCreate(&e) != hipSuccess) return CJX_ERR_DEVICE;
+    }
+    ctx->timing = enable != 0;
+    ctx->timing_count = 0;
+    return CJX_OK;
+}
+
+int cjx_context_timings(cjx_context *ctx, cjx_timings *out, int max) {
+    if (!ctx || !out || max <= 0 || !ctx->timing) return -1;
+    const uint64_t have = std::min<uint64_t>(ctx->timing_count, cjx_context::kTimingRing);
+    const uint64_t k = std::min<uint64_t>(have, (uint64_t)max);
+    for (uint64_t i = 0; i < k; i++) {  // oldest first, ending with the latest parse
+        hipEvent_t *tr = ctx->tev[(ctx->timing_count - k + i) % cjx_context::kTimingRing];
+        if (hipEventSynchronize(tr[2]) != hipSuccess) return -1;
+        cjx_timings &t = out[i];
+        (void)hipEventElapsedTime(&t.tokenize_ms, tr[0], tr[1]);
+        (void)hipEventElapsedTime(&t.phase3_ms, tr[1], tr[2]);
+        (void)hipEventElapsedTime(&t.total_ms, tr[0], tr[2]);
+    }
+    return (int)k;
+}
+
+int cjx_load(const char *path, cjx_input *out) {
+    if (!out) return CJX_ERR_INPUT;
+    out->data = nullptr;
+    out->size = 0;
+    FILE *f = path ? std::fopen(path, "rb") : nullptr;
+    if (!f) return CJX_ERR_INPUT;  // load_file.cu:10-13
+    std::fseek(f, 0, SEEK_END);
+    long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    if (sz < 0) {
+        std::fclose(f);
+        return CJX_ERR_INPUT;
+    }
+    uint8_t *buf = nullptr;
+    if (hipHostMalloc((void **)&buf, (size_t)sz + 1, hipHostMallocDefault) != hipSuccess) {
+        std::fclose(f);
+        return CJX_ERR_ALLOC;  // load_file.cu:23-27
+    }
+    size_t got = sz ? std::fread(buf, 1, (size_t)sz, f) : 0;
+    std::fclose(f);
+    if (got != (size_t)sz) {
+        (void)hipHostFree(buf);
+        return CJX_ERR_INPUT;
+    }
+    out->data = buf;
+    out->size = (size_t)sz;
+    return CJX_OK;
+}
+
+void cjx_free_input(cjx_input *in) {
+    if (!in) return;
+    if (in->data) (void)hipHostFree(in->data);
+    in->data = nullptr;
+    in->size = 0;
+}
+
+int cjx_chunk_lines(const uint8_t *data, size_t n, int mode, size_t param, size_t *chunk_off,
+                    size_t *chunk_size, size_t cap, size_t *n_chunks) {
+    if (!n_chunks) return CJX_ERR_INPUT;
+    *n_chunks = 0;
+    if (!data && n) return CJX_ERR_INPUT;
+    // Line starts (load_file.cu:72-81): 0, every byte after '\n', and n if the
+    // last line is unterminated.
+    std::vector<size_t> offs;
+    offs.push_back(0);
+    const uint8_t *p = data, *end = data + n;
+    while (p < end) {
+        const void *hit = std::memchr(p, '\n', (size_t)(end - p));
+        if (!hit) break;
+        p = static_cast<const uint8_t *>(hit) + 1;
+        offs.push_back((size_t)(p - data));
+    }
+    if (offs.back() < n) offs.push_back(n);
+    std::vector<std::pair<size_t, size_t>> out;
+    if (mode == 0) {  // chunk count, load_file.cu:83-103
+        if (param == 0) return CJX_ERR_INPUT;
+        const size_t total = offs.size() - 1, per = total / param, extra = total % param;
+        size_t li = 0;
+        for (size_t i = 0; i < param; i++) {
+            const size_t k = per + (i < extra ? 1 : 0);
+            out.emplace_back(offs[li], offs[li + k] - offs[li]);
+            li += k;
+        }
+    } else if (mode == 1) {  // max bytes per chunk, load_file.cu:153-176
+        size_t cstart = 0, cur = 0;
+        for (size_t i = 1; i < offs.size(); i++) {
+            if (offs[i] - cstart > param) {
+                out.emplace_back(cstart, cur - cstart);
+                cstart = offs[i - 1];
+            }
+            cur = offs[i];
+        }
+        if (cstart < n) out.emplace_back(cstart, n - cstart);
+    } else {
+        return CJX_ERR_INPUT;
+    }
+    *n_chunks = out.size();
+    for (size_t i = 0; i < out.size() && i < cap; i++) {
+        if (chunk_off) chunk_off[i] = out[i].first;
+        if (chunk_size) chunk_size[i] = out[i].second;
+    }
+    return out.size() <= cap ? CJX_OK : CJX_ERR_CAPACITY;
+}
+
+int cjx_parse_standard(cjx_context *ctx, const uint8_t *data, size_t n, unsigned flags,
+                       cjx_result *out) {
+    if (!out) return CJX_ERR_INPUT;
+    zero_result(out);
+    if (!ctx || !data || n == 0) return out->error = CJX_ERR_INPUT;  // PSJ:1597-1600
+    if (flags & CJX_FLAG_UTF8_STRICT) return out->error = CJX_ERR_INPUT;  // not built yet
+    std::vector<uint64_t> offs{0}, sizes{n}, pos{0};
+    int rc = parse_host_units(ctx, data, offs, sizes, pos, flags & ~CJX_FLAG_JSONL, out);
+    out->error = rc;
+    return rc;
+}
+
+int cjx_parse_lines(cjx_context *ctx, const uint8_t *data, size_t n, const size_t *chunk_off,
+                    const size_t *chunk_size, size_t n_chunks, unsigned flags,
+                    cjx_result *out) {
+    if (!out) return CJX_ERR_INPUT;
+    zero_result(out);
+    // Argument checks of parse_json_lines.cu:1103-1150.
+    if (!ctx || n == 0 || !data || n_chunks == 0 || !chunk_off || !chunk_size)
+        return out->error = CJX_ERR_INPUT;
+    if (flags & CJX_FLAG_UTF8_STRICT) return out->error = CJX_ERR_INPUT;
+    std::vector<uint64_t> offs(n_chunks), sizes(n_chunks), pos(n_chunks);
+    uint64_t last_chunk_index = 0;
+    for (size_t i = 0; i < n_chunks; i++) {
+        if (chunk_size[i] == 0 || chunk_size[i] > n || chunk_off[i] > n ||
+            chunk_off[i] + chunk_size[i] > n) {
+            out->error = CJX_ERR_INPUT;
+            out->error_chunk = (int32_t)i;
+            return CJX_ERR_INPUT;
+        }
+        offs[i] = chunk_off[i];
+        sizes[i] = chunk_size[i];
+        pos[i] = last_chunk_index;  // lastChunkIndex, parse_json_lines.cu:1223
+        last_chunk_index += chunk_size[i];
+    }
+    int rc = parse_host_units(ctx, data, offs, sizes, pos, flags | CJX_FLAG_JSONL, out);
+    out->error = rc;
+    return rc;
+}
+
+void cjx_free_result(cjx_result *r) {
+    if (!r) return;
+    if (r->buf) (void)hipHostFree(r->buf);
+    std::free(r->result_sizes);
+    std::free(r->result_sizes_prefix);
+    std::memset(r, 0, sizeof(*r));
+    r->error_chunk = -1;
+}
+
+int cjx_parse_device(cjx_context *ctx, const uint8_t *d_input, size_t n, unsigned flags,
+                     int32_t *d_structural, int32_t *d_pair, size_t cap) {
+    if (!ctx || !d_input || n == 0 || !d_structural || !d_pair) return CJX_ERR_INPUT;
+    if (n > kMaxLaunchBytes) return CJX_ERR_INPUT;
+    if (flags & CJX_FLAG_UTF8_STRICT) return CJX_ERR_INPUT;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    std::vector<UnitDesc> units;
+    uint64_t dev_bytes = 0;
+    const uint32_t ntiles = plan_units({n}, {0}, units, dev_bytes);
+    Shape sh{ntiles, 1, std::min<uint64_t>(cap, n + 16), std::min<uint64_t>(cap, n + 16), n};
+    if ((rc = ensure_work(ctx, sh))) return rc;
+    std::memcpy(ctx->h_units.p, units.data(), sizeof(UnitDesc));
+    if (hipMemcpyAsync(ctx->units.p, ctx->h_units.p, sizeof(UnitDesc), hipMemcpyHostToDevice,
+                       ctx->stream) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    ctx->n_units = 1;
+    (void)hipEventRecord(ctx->ev0, ctx->stream);
+    rc = enqueue_parse(ctx, d_input, sh, flags & ~CJX_FLAG_JSONL,
+                       reinterpret_cast<uint32_t *>(d_structural), d_pair, false);
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
+    ctx->events_recorded = true;
+    return rc;
+}
+
+int cjx_device_wait(cjx_context *ctx, cjx_device_status *st) {
+    if (!ctx) return CJX_ERR_INPUT;
+    int rc = read_status(ctx, ctx->n_units ? ctx->n_units : 1);
+    if (rc) return rc;
+    const UnitOut *uo = static_cast<const UnitOut *>(ctx->h_uout.p);
+    const LaunchState *ls = static_cast<const LaunchState *>(ctx->h_ls.p);
+    int code = unit_code(uo[0]);
+    if (ls->overflow) code = CJX_ERR_CAPACITY;
+    if (st) {
+        st->n_structural = ls->n_total;
+        st->n_brackets = ls->m_total;
+        st->error = code;
+        st->capacity_exceeded = ls->overflow ? 1 : 0;
+    }
+    return code;
+}
+
+}  // extern "C"

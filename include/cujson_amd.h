@@ -1,0 +1,169 @@
+/*
+ * cujson_amd.h - C ABI of the MI355X-native three-phase JSON parse.
+ *
+ * This is the drop-in boundary for cuJSON's parse path.  Every entry point
+ * replaces one reference interface (cited per declaration); the C++ headers
+ * under include/cujson/ rebuild the reference's own C++ API
+ * (cuJSONInput / cuJSONLinesInput / cuJSONResult, loadJSON,
+ * parse_standard_json, parse_json_lines) on top of these functions, so the
+ * reference's query iterator consumes the results unchanged.
+ *
+ * Output layout (identical to the reference, cujson/parse_standard_json.cu:1659-1680):
+ *   one int32 buffer of 2(N+2) cells
+ *   structural = buf;        structural[0] = 0, structural[1..N] = 1-based byte
+ *                            positions of { } [ ] : , (and '\n' for JSON Lines)
+ *                            outside strings, structural[N+1] = N+1
+ *   pair_pos   = buf + N + 1 pair_pos[h] for an opener at structural index h is
+ *                            the structural index of its closer; cells the
+ *                            reference leaves unwritten hold -1.
+ *
+ * Threading: a cjx_context owns one HIP stream and its device buffers; use
+ * one context per host thread.  Calls on one context are synchronous with
+ * respect to the caller unless stated otherwise.
+ */
+#ifndef CUJSON_AMD_H
+#define CUJSON_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  CJX_ERR_UTF8 / CJX_ERR_STRUCTURE are where the reference
+ * calls exit(0) (parse_standard_json.cu:1626, :1569; parse_json_lines.cu:1172,
+ * :1058); CJX_ERR_INPUT is where it returns an empty cuJSONResult
+ * (parse_standard_json.cu:1597-1600; parse_json_lines.cu:1103-1150). */
+enum {
+    CJX_OK = 0,
+    CJX_ERR_INPUT = 1,
+    CJX_ERR_UTF8 = 2,
+    CJX_ERR_STRUCTURE = 3,
+    CJX_ERR_ALLOC = 4,
+    CJX_ERR_DEVICE = 5,
+    CJX_ERR_CAPACITY = 6
+};
+
+/* Flags. */
+#define CJX_FLAG_UTF8_STRICT 0x1u /* RFC 3629 validation (off: the reference's check)  */
+#define CJX_FLAG_DEPTH_U16   0x2u /* 16-bit depth keys (off: the reference's uint8)     */
+#define CJX_FLAG_JSONL       0x4u /* '\n' is structural (set by cjx_parse_lines)        */
+
+typedef struct cjx_context cjx_context;
+
+/* Host-side result; field meanings follow cuJSONResult (cujson/cujson_types.h:22-33). */
+typedef struct cjx_result {
+    int32_t *buf;                 /* pinned host buffer of 2*(n_structural+2) int32  */
+    int32_t *structural;          /* = buf                                          */
+    int32_t *pair_pos;            /* = buf + n_structural + 1                       */
+    int64_t n_structural;         /* N                                              */
+    int64_t n_brackets;           /* M                                              */
+    int32_t chunk_count;          /* cuJSONResult::chunkCount                       */
+    int32_t *result_sizes;        /* per chunk N   (cuJSONResult::resultSizes)       */
+    int32_t *result_sizes_prefix; /* inclusive sum (cuJSONResult::resultSizesPrefix) */
+    int32_t total_result_size;    /* N + 2                                          */
+    int32_t file_size;            /* N + 2 (the reference's fileSize)               */
+    int32_t error;                /* status of the parse                            */
+    int32_t error_chunk;          /* first failing chunk, -1 if none                */
+    double device_ms;             /* device time of the phases (hipEvent)           */
+} cjx_result;
+
+/* Pinned host input (cuJSONInput, cujson/cujson_types.h:9-12). */
+typedef struct cjx_input {
+    uint8_t *data;
+    size_t size;
+} cjx_input;
+
+/* Create / destroy a context on a HIP device (the reference uses device 0
+ * implicitly, parse_standard_json.cu:1617). */
+int cjx_context_create(int device, cjx_context **out);
+void cjx_context_destroy(cjx_context *ctx);
+
+/* Replaces cuJSONInput loadJSON(const std::string&), cujson/load_file.h:9 /
+ * load_file.cu:7-36: read a file into a pinned host buffer. */
+int cjx_load(const char *path, cjx_input *out);
+void cjx_free_input(cjx_input *in);
+
+/* Replaces the line scan of loadJSONLines_chunkCount / _chunkSizeBytes /
+ * _chunkSizeMegaBytes, cujson/load_file.h:10-12 / load_file.cu:71-103,
+ * :137-176: split a JSON Lines buffer into line-aligned chunks.
+ *   mode 0: chunk_param = chunk count; mode 1: chunk_param = max bytes per chunk.
+ * Writes up to cap (offset, size) pairs; *n_chunks receives the count. */
+int cjx_chunk_lines(const uint8_t *data, size_t n, int mode, size_t chunk_param,
+                    size_t *chunk_off, size_t *chunk_size, size_t cap, size_t *n_chunks);
+
+/* Replaces cuJSONResult parse_standard_json(cuJSONInput),
+ * cujson/parse_standard_json.h:9 / parse_standard_json.cu:1584-1687.
+ * data is host memory; the result buffer is pinned host memory owned by the
+ * caller (release with cjx_free_result). */
+int cjx_parse_standard(cjx_context *ctx, const uint8_t *data, size_t n, unsigned flags,
+                       cjx_result *out);
+
+/* Replaces cuJSONResult parse_json_lines(cuJSONLinesInput),
+ * cujson/parse_json_lines.h:9 / parse_json_lines.cu:1092-1264.
+ * Chunks (chunk_off[i], chunk_size[i]) are consecutive line-aligned slices of
+ * data; positions count from data[0] as in the reference (lastChunkIndex). */
+int cjx_parse_lines(cjx_context *ctx, const uint8_t *data, size_t n, const size_t *chunk_off,
+                    const size_t *chunk_size, size_t n_chunks, unsigned flags,
+                    cjx_result *out);
+
+/* Releases a result (the reference's cuJSONIterator::freeJson calls
+ * cudaFreeHost(structural), query_iterator_standard_json.cpp:143-146). */
+void cjx_free_result(cjx_result *r);
+
+const char *cjx_strerror(int code);
+
+/* ---------------------------------------------------------------------- */
+/* Device-resident form (no counterpart in the reference, which always    */
+/* copies through the host): the input is already in HBM and the output   */
+/* rows stay in HBM.  Used by pipelines that keep data on the GPU and by  */
+/* the throughput benchmark.                                              */
+/* ---------------------------------------------------------------------- */
+
+/* Counts and codes of a device-resident parse, read back after sync. */
+typedef struct cjx_device_status {
+    int64_t n_structural;
+    int64_t n_brackets;
+    int32_t error;
+    int32_t capacity_exceeded;
+} cjx_device_status;
+
+/* Parse a device-resident document.
+ *   d_input      device pointer, 16-byte aligned, readable up to n rounded up to 16
+ *   d_structural device int32[cap]: structural row (1-based positions, no sentinels)
+ *   d_pair       device int32[cap]: pair row (structural index of the closer, 1-based
+ *                in the host layout sense, -1 elsewhere)
+ *   cap          capacity of both rows; N > cap reports CJX_ERR_CAPACITY
+ * Launches on the context's stream and returns without synchronizing;
+ * cjx_device_wait() synchronizes and fills the status. */
+int cjx_parse_device(cjx_context *ctx, const uint8_t *d_input, size_t n, unsigned flags,
+                     int32_t *d_structural, int32_t *d_pair, size_t cap);
+int cjx_device_wait(cjx_context *ctx, cjx_device_status *st);
+
+/* Per-phase device times of recent parses on a context, from hipEvents
+ * recorded on the context stream around each phase while timing is enabled
+ * (off by default; enabling clears the history).  cjx_context_timings fills
+ * up to `max` entries for the most recent parses (at most 256), oldest
+ * first, waiting for them to finish, and returns the count (-1 on error).
+ * No counterpart in the reference, whose per-phase timers live only in its
+ * paper drivers (paper_reproduced/src/cuJSON-standardjson.cu:2134-2249). */
+typedef struct cjx_timings {
+    float tokenize_ms; /* phases 1+2: the fused tokenizer kernel            */
+    float phase3_ms;   /* phase 3: depth scan, sort, pairing (all units)     */
+    float total_ms;    /* from the reset kernel to the last phase-3 kernel   */
+} cjx_timings;
+int cjx_context_set_timing(cjx_context *ctx, int enable);
+int cjx_context_timings(cjx_context *ctx, cjx_timings *out, int max);
+
+/* HIP stream of the context (hipStream_t), for callers that order their own
+ * work against the parse. */
+void *cjx_context_stream(cjx_context *ctx);
+
+/* Library build identification. */
+const char *cjx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,75 @@
+"""C-ABI library checks that need no GPU: it loads, exports every symbol the
+public header declares, and its host-only entry points behave."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cujson_amd as cj
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cujson_amd.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cjx_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_loads_and_exports_header_symbols():
+    L = cj.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 12
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(cj.ABI_SYMBOLS)
+
+
+def test_strerror_and_version():
+    assert cj.strerror(cj.CJX_OK) == "ok"
+    assert "UTF-8" in cj.strerror(cj.CJX_ERR_UTF8)
+    assert cj.lib().cjx_version().startswith(b"cujson_amd")
+
+
+@pytest.mark.parametrize("cc", [1, 2, 4, 7, 51, 60])
+def test_chunk_lines_count_mode_matches_loader_rule(twitter_small, cc):
+    offs, sizes = cj.chunk_lines(twitter_small, 0, cc)
+    assert list(zip(offs, sizes)) == oracle.chunks_by_count(twitter_small, cc)
+
+
+@pytest.mark.parametrize("nb", [1, 100, 4096, 65536, 10 ** 7])
+def test_chunk_lines_bytes_mode_matches_loader_rule(twitter_small, nb):
+    offs, sizes = cj.chunk_lines(twitter_small, 1, nb)
+    assert list(zip(offs, sizes)) == oracle.chunks_by_bytes(twitter_small, nb)
+
+
+def test_chunk_lines_unterminated_last_line():
+    data = b'{"a":1}\n{"b":2}'
+    offs, sizes = cj.chunk_lines(data, 0, 2)
+    assert list(zip(offs, sizes)) == [(0, 8), (8, 7)]
+
+
+def test_entry_points_reject_bad_arguments_without_device():
+    r = cj._Result()
+    assert cj.lib().cjx_parse_standard(None, None, 0, 0, ctypes.byref(r)) == cj.CJX_ERR_INPUT
+    assert r.error == cj.CJX_ERR_INPUT and not r.buf
+    assert cj.lib().cjx_parse_lines(None, None, 0, None, None, 0, 0,
+                                    ctypes.byref(r)) == cj.CJX_ERR_INPUT
+
+
+def test_context_without_gpu_fails_loudly():
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a GPU is present")
+    h = ctypes.c_void_p()
+    assert cj.lib().cjx_context_create(0, ctypes.byref(h)) == cj.CJX_ERR_DEVICE
+    with pytest.raises(RuntimeError):
+        cj.Context(0)

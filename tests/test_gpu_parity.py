@@ -1,0 +1,204 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Integer/byte/index work, so the bar is bit-exact: same error code, same N,
+and the same 2(N+2)-cell host buffer (structural row, sentinels, pair row
+with -1 in the cells the reference never writes)."""
+import random
+
+import numpy as np
+import pytest
+
+import cujson_amd as cj
+import oracle
+from cujson_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+ALPHABET = [ord(c) for c in '{}[]:,"\\a \n'] + [0xC3, 0xA9, 0xE2, 0x82, 0xAC, 0xF0]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cj.Context(0)
+    yield c
+    c.close()
+
+
+def check_standard(ctx, data, flags=0, label=""):
+    data = bytes(data) if not isinstance(data, np.ndarray) else data
+    g = cj.parse_standard_json(data, flags, ctx)
+    o = oracle.parse_standard(data, flags)
+    assert g.error == o.code, (label, g.error, o.code)
+    if o.code == oracle.ERR_INPUT:
+        return g
+    assert g.N == o.N, (label, g.N, o.N)
+    if not np.array_equal(g.buf, o.buf):
+        bad = np.flatnonzero(g.buf != o.buf)
+        raise AssertionError(f"{label}: {bad.size} cells differ, first {bad[:8]} "
+                             f"gpu={g.buf[bad[:8]]} oracle={o.buf[bad[:8]]} N={o.N}")
+    return g
+
+
+def check_lines(ctx, data, chunks, flags=0, label=""):
+    offs = [o for o, _ in chunks]
+    sizes = [s for _, s in chunks]
+    inp = cj.cuJSONLinesInput(np.frombuffer(bytes(data), np.uint8), len(chunks), len(data), offs,
+                              sizes)
+    g = cj.parse_json_lines(inp, flags, ctx)
+    o = oracle.parse_lines(data, chunks, flags)
+    assert g.error == o.code, (label, g.error, o.code)
+    if o.code != oracle.OK:
+        return g
+    assert g.N == o.N
+    assert np.array_equal(g.buf, o.buf), label
+    assert g.resultSizes == o.result_sizes
+    assert g.resultSizesPrefix == o.result_sizes_prefix
+    return g
+
+
+def test_twitter_large(ctx, twitter_large):
+    g = check_standard(ctx, twitter_large, label="twitter_large")
+    assert g.N == 22388
+    assert g.pair_pos[0] == g.N + 1
+
+
+def test_config1(ctx):
+    check_standard(ctx, gen.config1(), label="config1")
+
+
+@pytest.mark.parametrize("cc", [1, 2, 4, 8, 51])
+def test_twitter_small_records_chunked(ctx, twitter_small, cc):
+    check_lines(ctx, twitter_small, oracle.chunks_by_count(twitter_small, cc), label=f"cc={cc}")
+
+
+def test_small_sizes_and_tile_edges(ctx):
+    base = bytes(gen.config1()[:70000])
+    for n in [1, 2, 3, 4, 5, 15, 16, 17, 31, 32, 33, 63, 64, 65, 1023, 1024, 1025, 4095, 4096,
+              4097, 16383, 16384, 16385, 32767, 32768, 32769, 65536 + 7]:
+        check_standard(ctx, base[:n], label=f"prefix n={n}")
+
+
+def test_empty_and_null(ctx):
+    g = cj.parse_standard_json(b"", 0, ctx)
+    assert g.error == cj.CJX_ERR_INPUT and g.structural is None
+
+
+@pytest.mark.parametrize("boundary", [16, 1024, 4096, 16384, 32768])
+def test_backslash_runs_across_boundaries(ctx, boundary):
+    for k in range(1, 40):
+        for tail in (b'"]', b'a"]', b'\\"]'):
+            for shift in (-2, -1, 0, 1):
+                pre = b'["' + b"x" * (boundary - 2 - k + shift)
+                check_standard(ctx, pre + b"\\" * k + tail + b',{"a":[1]}]',
+                               label=f"b={boundary} k={k} s={shift} tail={tail}")
+
+
+def test_all_backslash_tiles(ctx):
+    for k in (16384 - 1, 16384, 16384 + 1, 2 * 16384, 2 * 16384 + 1, 3 * 16384 + 5):
+        for head in (b'["', b'[', b'"'):
+            check_standard(ctx, head + b"\\" * k + b'"]",[1]]', label=f"allbs k={k} head={head}")
+
+
+def test_strings_across_tiles(ctx):
+    for L in (16383, 16384, 16385, 50000):
+        doc = b'[{"a":"' + (b"{[:,]}" * (L // 6 + 1))[:L] + b'"},[1,2,{"b":[]}]]'
+        check_standard(ctx, doc, label=f"long string L={L}")
+
+
+def test_random_byte_soup(ctx):
+    rng = random.Random(2024)
+    for it in range(40):
+        n = rng.choice([100, 5000, 20000, 70000, 200000])
+        data = bytes(rng.choice(ALPHABET) for _ in range(n))
+        check_standard(ctx, data, label=f"soup it={it} n={n}")
+        check_standard(ctx, data, oracle.FLAG_DEPTH_U16, label=f"soup u16 it={it}")
+
+
+def test_deep_nesting(ctx):
+    for d in (255, 256, 257, 300, 512, 1000):
+        doc = b"[" * d + b"1" + b"]" * d
+        check_standard(ctx, doc, label=f"depth {d}")
+        check_standard(ctx, doc, oracle.FLAG_DEPTH_U16, label=f"depth {d} u16")
+    doc = b"".join((b"[" if i % 2 == 0 else b'{"d":') for i in range(600)) + b"0" + b"".join(
+        (b"]" if i % 2 == 0 else b"}") for i in reversed(range(600)))
+    check_standard(ctx, doc, label="alternating 600")
+    check_standard(ctx, doc, oracle.FLAG_DEPTH_U16, label="alternating 600 u16")
+
+
+def test_faults(ctx):
+    base = gen.config1()
+    for kind in range(5):
+        for seed in range(3):
+            d = base.copy()
+            assert gen.plant_fault(d, kind, 100 + seed)
+            check_standard(ctx, d, label=f"fault {kind} seed {seed}")
+
+
+def test_utf8_at_word_and_tile_edges(ctx):
+    seqs = [b"\xc3\xa9", b"\xe2\x82\xac", b"\xf0\x9f\x98\x80", b"\xc3", b"\xe2\x82", b"\xf0\x9f",
+            b"\xc0\xaf", b"\xed\xa0\x80"]
+    for edge in (4, 16, 1024, 4096, 16384):
+        for s in seqs:
+            for shift in range(-4, 2):
+                pre = b'["' + b"a" * (edge - 2 + shift)
+                check_standard(ctx, pre + s + b'bcdefgh"]', label=f"utf8 edge={edge} {s} {shift}")
+
+
+def test_config2_shape_64mb(ctx):
+    d = gen.config2(target_bytes=64 << 20)
+    check_standard(ctx, d, label="config2 64MB")
+
+
+def test_jsonl_generated_records(ctx):
+    d = gen.lines(3000)
+    data = bytes(d)
+    for cc in (1, 3, 16):
+        check_lines(ctx, data, oracle.chunks_by_count(data, cc), label=f"lines cc={cc}")
+    check_lines(ctx, data, oracle.chunks_by_bytes(data, 1 << 16), label="lines 64KiB")
+
+
+def test_jsonl_bad_chunks(ctx, twitter_small):
+    ch = [(0, 0)] + oracle.chunks_by_count(twitter_small, 2)
+    g = cj.parse_json_lines(cj.cuJSONLinesInput(np.frombuffer(twitter_small, np.uint8), 3,
+                                                len(twitter_small), [c[0] for c in ch],
+                                                [c[1] for c in ch]), 0, ctx)
+    assert g.error == cj.CJX_ERR_INPUT and g.structural is None
+
+
+def test_device_resident_entry(ctx):
+    from cujson_amd.hip import DeviceBuffer
+    d = gen.config1()
+    n = d.size
+    dev = DeviceBuffer(((n + 15) // 16) * 16)
+    dev.upload(d)
+    cap = n
+    st = DeviceBuffer(cap * 4)
+    pr = DeviceBuffer(cap * 4)
+    assert ctx.parse_device(dev.ptr, n, st.ptr, pr.ptr, cap) == 0
+    rc, N, M, over = ctx.wait()
+    o = oracle.parse_standard(d)
+    assert rc == o.code and N == o.N and not over
+    assert np.array_equal(st.download(np.int32, N), o.structural[1:N + 1])
+    assert np.array_equal(pr.download(np.int32, N), o.pair_pos[1:N + 1])
+    # capacity overflow is reported, never written past
+    small = 1000
+    st2 = DeviceBuffer((small + 64) * 4)
+    pr2 = DeviceBuffer((small + 64) * 4)
+    st2.fill(7)
+    pr2.fill(7)
+    assert ctx.parse_device(dev.ptr, n, st2.ptr, pr2.ptr, small) == 0
+    rc, N, M, over = ctx.wait()
+    assert over and rc == cj.CJX_ERR_CAPACITY
+    assert np.all(st2.download(np.uint8, 64 * 4, small * 4) == 7)
+    assert np.all(pr2.download(np.uint8, (small + 64) * 4) == 7)
+
+
+def test_timings(ctx):
+    ctx.set_timing(True)
+    try:
+        g = cj.parse_standard_json(gen.config1(), 0, ctx)
+        assert g.error == 0
+        tok, p3, tot = ctx.timings()
+        assert tok > 0 and p3 > 0 and tot >= tok
+    finally:
+        ctx.set_timing(False)
