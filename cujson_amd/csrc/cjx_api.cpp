@@ -83,7 +83,7 @@ struct cjx_context {
     uint64_t timing_count = 0;
     bool timing = false;
     DevBuf in, structural, pair;
-    DevBuf units, uout, ls, carry_st, count_st;
+    DevBuf units, uout, ls, tile_st, layout;
     DevBuf oc_char, oc_idx, keys, sorted, tmp, spos, block_delta, hist;
     PinnedBuf h_units, h_uout, h_ls;
     // Shape of the launch in flight (device form).
@@ -109,7 +109,7 @@ uint32_t pair_blocks(uint64_t max_n) {
 
 // Units are given as (host offset, size); they are placed in the device
 // input at 16-byte aligned offsets.  Fills descriptors and returns the tile
-// count, or 0 on invalid shape.
+// count, or 0 on invalid shape.  Output regions are set by set_regions.
 uint32_t plan_units(const std::vector<uint64_t> &sizes, const std::vector<uint64_t> &pos_base,
                     std::vector<UnitDesc> &units, uint64_t &dev_bytes) {
     units.resize(sizes.size());
@@ -129,19 +129,35 @@ uint32_t plan_units(const std::vector<uint64_t> &sizes, const std::vector<uint64
     return (uint32_t)tiles;
 }
 
+// Output regions: unit u owns rows [out_off, out_off + out_cap) of every
+// per-structural and per-bracket buffer.  `full` gives each unit its exact
+// bound (n rows); otherwise a unit gets n/3 + 256 rows (typical JSON has
+// fewer than 0.15 structurals per byte) and the parse is retried with full
+// regions when one overflows.  Returns the total row count.
+uint64_t set_regions(std::vector<UnitDesc> &units, bool full) {
+    uint64_t off = 0;
+    for (UnitDesc &u : units) {
+        const uint64_t cap = full ? u.n : std::min<uint64_t>(u.n, u.n / 3 + 256);
+        u.out_off = (uint32_t)off;
+        u.out_cap = (uint32_t)cap;
+        off += cap;
+    }
+    return off;
+}
+
 struct Shape {
     uint32_t ntiles, n_units;
-    uint64_t n_cap, m_cap, max_unit;
+    uint64_t rows;      // total rows of all output regions
+    uint64_t max_rows;  // largest region
 };
 
 int ensure_work(cjx_context *ctx, const Shape &sh) {
     bool ok = ctx->units.ensure(sh.n_units * sizeof(UnitDesc)) &&
               ctx->uout.ensure(sh.n_units * sizeof(UnitOut)) &&
               ctx->ls.ensure(sizeof(LaunchState)) &&
-              ctx->carry_st.ensure((size_t)sh.ntiles * 4) &&
-              ctx->count_st.ensure((size_t)sh.ntiles * 8) && ctx->oc_char.ensure(sh.m_cap + 16) &&
-              ctx->oc_idx.ensure(sh.m_cap * 4 + 16) && ctx->keys.ensure(sh.m_cap * 2 + 16) &&
-              ctx->sorted.ensure(sh.m_cap * 4 + 16) && ctx->spos.ensure(sh.m_cap * 4 + 16) &&
+              ctx->tile_st.ensure((size_t)sh.ntiles * 16) && ctx->oc_char.ensure(sh.rows + 16) &&
+              ctx->oc_idx.ensure(sh.rows * 4 + 16) && ctx->keys.ensure(sh.rows * 2 + 16) &&
+              ctx->sorted.ensure(sh.rows * 4 + 16) && ctx->spos.ensure(sh.rows * 4 + 16) &&
               ctx->block_delta.ensure(512 * 4) && ctx->hist.ensure(512 * 256 * 4) &&
               ctx->h_units.ensure(sh.n_units * sizeof(UnitDesc)) &&
               ctx->h_uout.ensure(sh.n_units * sizeof(UnitOut)) &&
@@ -154,8 +170,7 @@ LaunchBufs bufs_of(cjx_context *ctx) {
     b.units = ctx->units.as<UnitDesc>();
     b.uout = ctx->uout.as<UnitOut>();
     b.ls = ctx->ls.as<LaunchState>();
-    b.carry_st = ctx->carry_st.as<uint32_t>();
-    b.count_st = ctx->count_st.as<unsigned long long>();
+    b.tile_st = ctx->tile_st.as<unsigned long long>();
     b.oc_char = ctx->oc_char.as<uint8_t>();
     b.oc_idx = ctx->oc_idx.as<uint32_t>();
     b.keys = ctx->keys.as<uint16_t>();
@@ -170,24 +185,25 @@ LaunchBufs bufs_of(cjx_context *ctx) {
 // Enqueues the whole parse of `units` (already in ctx->units on the device)
 // on the context stream.
 int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsigned flags,
-                  uint32_t *d_struct, int32_t *d_pair, bool local_pairs) {
+                  uint32_t *d_struct, int32_t *d_pair) {
     const bool u16 = (flags & CJX_FLAG_DEPTH_U16) != 0;
-    if (u16 && !ctx->tmp.ensure(sh.m_cap * 4 + 16)) return CJX_ERR_ALLOC;
+    if (u16 && !ctx->tmp.ensure(sh.rows * 4 + 16)) return CJX_ERR_ALLOC;
     LaunchBufs b = bufs_of(ctx);
     hipStream_t s = ctx->stream;
     if (launch_reset(s, b, sh.ntiles, sh.n_units) != hipSuccess) return CJX_ERR_DEVICE;
     hipEvent_t *tev = ctx->tev[ctx->timing_count % cjx_context::kTimingRing];
     if (ctx->timing) (void)hipEventRecord(tev[0], s);
     if (launch_tokenize(s, b, d_in, sh.ntiles, sh.n_units, (flags & CJX_FLAG_JSONL) != 0,
-                        d_struct, (uint32_t)std::min<uint64_t>(sh.n_cap, 0xFFFFFFFFull),
-                        (uint32_t)std::min<uint64_t>(sh.m_cap, 0xFFFFFFFFull)) != hipSuccess)
+                        d_struct) != hipSuccess)
         return CJX_ERR_DEVICE;
     if (ctx->timing) (void)hipEventRecord(tev[1], s);
-    const uint32_t G = phase3_blocks(std::min(sh.max_unit, sh.m_cap));
-    const uint32_t PG = pair_blocks(std::min(sh.max_unit, sh.n_cap));
+    // Concatenated results number pairs across units (JSON Lines).
+    if (sh.n_units > 1 && launch_unit_bases(s, b, sh.n_units) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    const uint32_t G = phase3_blocks(sh.max_rows);
+    const uint32_t PG = pair_blocks(sh.max_rows);
     for (uint32_t u = 0; u < sh.n_units; u++)
-        if (launch_phase3(s, b, u, G, PG, u16, d_pair, local_pairs) != hipSuccess)
-            return CJX_ERR_DEVICE;
+        if (launch_phase3(s, b, u, G, PG, u16, d_pair) != hipSuccess) return CJX_ERR_DEVICE;
     if (ctx->timing) {
         (void)hipEventRecord(tev[2], s);
         ctx->timing_count++;
@@ -229,11 +245,8 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
     std::vector<UnitDesc> units;
     uint64_t dev_bytes = 0;
     const uint32_t ntiles = plan_units(sizes, pos_base, units, dev_bytes);
-    uint64_t total = 0, max_unit = 0;
-    for (uint64_t z : sizes) {
-        total += z;
-        max_unit = std::max(max_unit, z);
-    }
+    uint64_t total = 0;
+    for (uint64_t z : sizes) total += z;
     if (ntiles == 0 || total > kMaxLaunchBytes) return CJX_ERR_INPUT;
     if (!ctx->in.ensure(dev_bytes)) return CJX_ERR_ALLOC;
     hipStream_t s = ctx->stream;
@@ -242,13 +255,13 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
                            hipMemcpyHostToDevice, s) != hipSuccess)
             return CJX_ERR_DEVICE;
 
-    Shape sh{ntiles, (uint32_t)units.size(), 0, 0, max_unit};
-    uint64_t guess = total / 3 + 4096;  // typical JSON has rho < 0.15
+    Shape sh{ntiles, (uint32_t)units.size(), 0, 0};
     for (int attempt = 0; attempt < 2; attempt++) {
-        sh.n_cap = attempt == 0 ? std::min<uint64_t>(guess, total + 16) : total + 16;
-        sh.m_cap = sh.n_cap;
+        sh.rows = set_regions(units, attempt == 1);
+        sh.max_rows = 0;
+        for (const UnitDesc &u : units) sh.max_rows = std::max<uint64_t>(sh.max_rows, u.out_cap);
         if ((rc = ensure_work(ctx, sh))) return rc;
-        if (!ctx->structural.ensure(sh.n_cap * 4 + 16) || !ctx->pair.ensure(sh.n_cap * 4 + 16))
+        if (!ctx->structural.ensure(sh.rows * 4 + 16) || !ctx->pair.ensure(sh.rows * 4 + 16))
             return CJX_ERR_ALLOC;
         std::memcpy(ctx->h_units.p, units.data(), units.size() * sizeof(UnitDesc));
         if (hipMemcpyAsync(ctx->units.p, ctx->h_units.p, units.size() * sizeof(UnitDesc),
@@ -256,7 +269,7 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
             return CJX_ERR_DEVICE;
         (void)hipEventRecord(ctx->ev0, s);
         rc = enqueue_parse(ctx, ctx->in.as<uint8_t>(), sh, flags, ctx->structural.as<uint32_t>(),
-                           ctx->pair.as<int32_t>(), false);
+                           ctx->pair.as<int32_t>());
         if (rc) return rc;
         (void)hipEventRecord(ctx->ev1, s);
         if ((rc = read_status(ctx, sh.n_units))) return rc;
@@ -289,18 +302,23 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
     const size_t cells = 2 * (N + 2);
     if (hipHostMalloc((void **)&buf, cells * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
         return CJX_ERR_ALLOC;
-    if (N) {
-        if (hipMemcpyAsync(buf + 1, ctx->structural.p, N * 4, hipMemcpyDeviceToHost, s) !=
-                hipSuccess ||
-            hipMemcpyAsync(buf + N + 2, ctx->pair.p, N * 4, hipMemcpyDeviceToHost, s) !=
-                hipSuccess) {
-            (void)hipHostFree(buf);
-            return CJX_ERR_DEVICE;
-        }
+    bool ok = true;
+    if (units.size() == 1) {  // one region starting at row 0: copy it as is
+        if (N)
+            ok = hipMemcpyAsync(buf + 1, ctx->structural.p, N * 4, hipMemcpyDeviceToHost, s) ==
+                     hipSuccess &&
+                 hipMemcpyAsync(buf + N + 2, ctx->pair.p, N * 4, hipMemcpyDeviceToHost, s) ==
+                     hipSuccess;
+    } else {  // pack the regions on the device, then one copy
+        ok = ctx->layout.ensure(cells * 4) &&
+             launch_assemble(s, bufs_of(ctx), (uint32_t)units.size(),
+                             ctx->structural.as<uint32_t>(), ctx->pair.as<int32_t>(),
+                             ctx->layout.as<int32_t>(), (uint32_t)N) == hipSuccess &&
+             hipMemcpyAsync(buf, ctx->layout.p, cells * 4, hipMemcpyDeviceToHost, s) == hipSuccess;
     }
-    if (hipStreamSynchronize(s) != hipSuccess) {
+    if (!ok || hipStreamSynchronize(s) != hipSuccess) {
         (void)hipHostFree(buf);
-        return CJX_ERR_DEVICE;
+        return ok ? CJX_ERR_DEVICE : CJX_ERR_ALLOC;
     }
     buf[0] = 0;
     buf[N + 1] = (int32_t)(N + 1);
@@ -313,9 +331,12 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
     out->file_size = (int32_t)(N + 2);
     out->result_sizes = (int32_t *)std::malloc(units.size() * sizeof(int32_t));
     out->result_sizes_prefix = (int32_t *)std::malloc(units.size() * sizeof(int32_t));
+    uint32_t prefix = 0;
     for (size_t i = 0; i < units.size(); i++) {
-        out->result_sizes[i] = (int32_t)(uo[i].n_end - uo[i].n_base);
-        out->result_sizes_prefix[i] = (int32_t)uo[i].n_end;
+        const uint32_t nu = uo[i].n_end - uo[i].n_base;
+        prefix += nu;
+        out->result_sizes[i] = (int32_t)nu;
+        out->result_sizes_prefix[i] = (int32_t)prefix;
     }
     return out->error;
 }
@@ -360,10 +381,10 @@ void cjx_context_destroy(cjx_context *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->in,       &ctx->structural, &ctx->pair,     &ctx->units,
-                      &ctx->uout,     &ctx->ls,         &ctx->carry_st, &ctx->count_st,
-                      &ctx->oc_char,  &ctx->oc_idx,     &ctx->keys,     &ctx->sorted,
-                      &ctx->tmp,      &ctx->spos,       &ctx->block_delta, &ctx->hist};
+    DevBuf *bufs[] = {&ctx->in,      &ctx->structural, &ctx->pair,   &ctx->units,
+                      &ctx->uout,    &ctx->ls,         &ctx->tile_st, &ctx->layout,
+                      &ctx->oc_char, &ctx->oc_idx,     &ctx->keys,   &ctx->sorted,
+                      &ctx->tmp,     &ctx->spos,       &ctx->block_delta, &ctx->hist};
     for (DevBuf *b : bufs) b->release();
     ctx->h_units.release();
     ctx->h_uout.release();
@@ -550,7 +571,9 @@ int cjx_parse_device(cjx_context *ctx, const uint8_t *d_input, size_t n, unsigne
     std::vector<UnitDesc> units;
     uint64_t dev_bytes = 0;
     const uint32_t ntiles = plan_units({n}, {0}, units, dev_bytes);
-    Shape sh{ntiles, 1, std::min<uint64_t>(cap, n + 16), std::min<uint64_t>(cap, n + 16), n};
+    units[0].out_off = 0;
+    units[0].out_cap = (uint32_t)std::min<uint64_t>(cap, n);
+    Shape sh{ntiles, 1, units[0].out_cap, units[0].out_cap};
     if ((rc = ensure_work(ctx, sh))) return rc;
     std::memcpy(ctx->h_units.p, units.data(), sizeof(UnitDesc));
     if (hipMemcpyAsync(ctx->units.p, ctx->h_units.p, sizeof(UnitDesc), hipMemcpyHostToDevice,
@@ -559,7 +582,7 @@ int cjx_parse_device(cjx_context *ctx, const uint8_t *d_input, size_t n, unsigne
     ctx->n_units = 1;
     (void)hipEventRecord(ctx->ev0, ctx->stream);
     rc = enqueue_parse(ctx, d_input, sh, flags & ~CJX_FLAG_JSONL,
-                       reinterpret_cast<uint32_t *>(d_structural), d_pair, false);
+                       reinterpret_cast<uint32_t *>(d_structural), d_pair);
     (void)hipEventRecord(ctx->ev1, ctx->stream);
     ctx->events_recorded = true;
     return rc;

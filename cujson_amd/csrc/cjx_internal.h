@@ -7,36 +7,49 @@
 namespace cjx {
 
 // Tokenizer tile geometry: a tile is one 256-thread workgroup (4 wave64s);
-// each lane owns 4 coalesced 16-byte segments, one per "round", so a wave
-// covers 4 KiB and a tile 16 KiB of input.
-constexpr int kTokThreads = 256;
+// each lane owns 32 contiguous bytes per round, and a wave runs
+// CJX_TOK_ROUNDS rounds, so a wave covers 2 KiB per round.
+#ifndef CJX_TOK_ROUNDS
+#define CJX_TOK_ROUNDS 2
+#endif
+#ifndef CJX_TOK_WAVES
+#define CJX_TOK_WAVES 4
+#endif
+constexpr int kTokThreads = 64 * CJX_TOK_WAVES;
 constexpr int kWaves = kTokThreads / 64;
-constexpr int kRounds = 4;
-constexpr int kWaveBytes = 64 * 16 * kRounds;       // 4096
-constexpr int kTileBytes = kTokThreads * 16 * kRounds;  // 16384
+constexpr int kLaneBytes = 32;
+constexpr int kTokRounds = CJX_TOK_ROUNDS;
+constexpr int kRoundBytes = 64 * kLaneBytes;              // 2 KiB per wave round
+constexpr int kWaveBytes = kRoundBytes * kTokRounds;
+constexpr int kTileBytes = kWaveBytes * kWaves;
 
 // Phase-3 geometry.
 constexpr int kP3Threads = 256;
 constexpr int kPairChunk = 4096;  // structural indices per pair-writer chunk (LDS staging)
 
 // One parse unit inside a launch: a standard document, a JSON Lines chunk or
-// one document of a batch.  Units are concatenated: structural and bracket
-// indices run on across units (JSON Lines lastStructuralIndex semantics).
+// one document of a batch.  Escape/string state, depth and counts restart at
+// every unit.  The unit's outputs (structural row, pair row and bracket
+// arrays) live in its own region [out_off, out_off + out_cap) of the output
+// buffers; out_cap >= n always suffices since M <= N <= n.
 struct UnitDesc {
     uint64_t in_off;    // byte offset of the unit in the device input (16-byte aligned)
     uint64_t n;         // unit length in bytes
     uint64_t pos_base;  // added to the 1-based byte positions (lastChunkIndex)
     uint32_t tile0;     // first tokenizer tile of the unit
     uint32_t ntiles;    // tiles of the unit
+    uint32_t out_off;   // first row index of the unit's output region
+    uint32_t out_cap;   // rows available in the region
 };
 
 // Per-unit results written on the device.
 struct UnitOut {
-    uint32_t n_base, m_base;  // first structural / bracket index of the unit
-    uint32_t n_end, m_end;    // one past the last
+    uint32_t n_base, m_base;  // region start (row index) of the unit's rows
+    uint32_t n_end, m_end;    // one past the unit's last structural / bracket
     uint32_t utf8_err;        // phase-1 flag
     uint32_t struct_err;      // phase-3 flag
-    uint32_t pad0, pad1;
+    uint32_t pair_base;       // added to pair values (lastStructuralIndex), 0 if local
+    uint32_t pad;
 };
 
 // Per-launch counters, cleared by k_reset before every parse.
@@ -46,7 +59,7 @@ struct LaunchState {
     uint32_t n_total, m_total;
 };
 
-// Look-back status encoding.
+// Look-back status encoding (two 64-bit words per tile, see cjx_tokenize.hip).
 constexpr uint32_t kStInvalid = 0, kStAgg = 1, kStInc = 2;
 
 }  // namespace cjx
@@ -58,8 +71,7 @@ struct LaunchBufs {
     UnitDesc *units;
     UnitOut *uout;
     LaunchState *ls;
-    uint32_t *carry_st;
-    unsigned long long *count_st;
+    unsigned long long *tile_st;  // 2 words per tile
     uint8_t *oc_char;
     uint32_t *oc_idx;
     uint16_t *keys;
@@ -72,9 +84,12 @@ struct LaunchBufs {
 
 hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uint32_t n_units);
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                           uint32_t ntiles, uint32_t n_units, bool jsonl, uint32_t *out_struct,
-                           uint32_t n_cap, uint32_t m_cap);
+                           uint32_t ntiles, uint32_t n_units, bool jsonl, uint32_t *out_struct);
+hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units);
+hipError_t launch_assemble(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
+                           const uint32_t *d_struct, const int32_t *d_pair, int32_t *dst,
+                           uint32_t N);
 hipError_t launch_phase3(hipStream_t s, const LaunchBufs &b, uint32_t u, uint32_t G,
-                         uint32_t pair_grid, bool depth_u16, int32_t *pair_row, bool local_pairs);
+                         uint32_t pair_grid, bool depth_u16, int32_t *pair_row);
 
 }  // namespace cjx

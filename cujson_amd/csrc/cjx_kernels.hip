@@ -1,449 +1,96 @@
 // cjx_kernels.hip - hand-written gfx950 kernels of the three-phase JSON parse.
 //
-// Phase 1+2 run fused in one pass over the input (k_tokenize): every byte is
-// read once from HBM, classified in registers, and only the compact outputs
-// (structural positions, bracket chars and indices) are written back.  The
-// sequential dependencies of the reference's separate kernels and Thrust scans
-// (cujson/parse_standard_json.cu:446-1343) are carried by a decoupled
-// look-back over tiles:
-//   * escape / string state as an affine map over GF(2) per tile (4 bits),
-//   * structural and bracket counts as one packed 64-bit prefix sum.
+// Phases 1+2 (the fused tokenizer) live in cjx_tokenize.hip.
 // Phase 3 (parse_standard_json.cu:1345-1582) runs over the compact bracket
 // array: a segmented +1/-1 depth scan (k_depth_totals, k_depth_keys), a
 // stable LDS counting sort by depth key (k_scan_hist, k_scatter), and a
 // dense pair-row writer that applies the reference's group-of-four pairing
 // rule (k_pairs).
-#include "cjx_internal.h"
+#include "cjx_device.h"
 
 namespace cjx {
-
-// ---------------------------------------------------------------------------
-// Wave helpers (wave64).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
-
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= (uint32_t)d) x += y;
-    }
-    return x;
-}
 
 __device__ __forceinline__ int32_t wave_incl_scan_i(int32_t x) {
     return (int32_t)wave_incl_scan((uint32_t)x);
 }
 
-__device__ __forceinline__ uint32_t wave_read(uint32_t x, int lane) { return __shfl(x, lane, 64); }
-
-__device__ __forceinline__ uint32_t agent_load(const uint32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void agent_store(uint32_t *p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned long long agent_load64(const unsigned long long *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void agent_store64(unsigned long long *p, unsigned long long v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ---------------------------------------------------------------------------
-// Byte classification (SWAR over 32-bit words).
-// ---------------------------------------------------------------------------
-
-// High bit of each byte set iff that byte of t is zero (exact, no borrow leak).
-__device__ __forceinline__ uint32_t zero_bytes_hi(uint32_t t) {
-    return ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t) & 0x80808080u;
-}
-// Gather bits 7,15,23,31 into bits 0..3.
-__device__ __forceinline__ uint32_t hi_to_nib(uint32_t h) { return (h * 0x00204081u) >> 28; }
-
-// 16-bit masks of one 16-byte segment: bit j <-> byte j.  The character sets
-// are those of bitMapCreatorSimd (parse_standard_json.cu:473-490; JSON Lines
-// adds '\n', parse_json_lines.cu:453-456).
-template <bool JSONL>
-__device__ __forceinline__ void classify16(const uint4 v, uint32_t &bs, uint32_t &q, uint32_t &op,
-                                           uint32_t &oc) {
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-    bs = q = op = oc = 0;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const uint32_t x = w[i];
-        const uint32_t x20 = x | 0x20202020u;  // folds '[' ']' onto '{' '}'
-        const uint32_t hbs = zero_bytes_hi(x ^ 0x5C5C5C5Cu);
-        const uint32_t hq = zero_bytes_hi(x ^ 0x22222222u);
-        const uint32_t hoc = zero_bytes_hi(x20 ^ 0x7B7B7B7Bu) | zero_bytes_hi(x20 ^ 0x7D7D7D7Du);
-        uint32_t hop = hoc | zero_bytes_hi(x ^ 0x3A3A3A3Au) | zero_bytes_hi(x ^ 0x2C2C2C2Cu);
-        if (JSONL) hop |= zero_bytes_hi(x ^ 0x0A0A0A0Au);
-        bs |= hi_to_nib(hbs) << (4 * i);
-        q |= hi_to_nib(hq) << (4 * i);
-        op |= hi_to_nib(hop) << (4 * i);
-        oc |= hi_to_nib(hoc) << (4 * i);
-    }
-}
-
-// Reference UTF-8 check of one 32-bit word against its predecessor
-// (checkUTF8, parse_standard_json.cu:237-257, reduced to what survives the
-// flag reassignment at :122-123): for a word without high bits, the previous
-// word must not end in an unfinished lead (:219-223, :239-242); otherwise a
-// byte errs iff [prev1 and byte are continuations] != [prev2 >= E0 or prev3 >= F0].
-__device__ __forceinline__ uint32_t utf8_word_err(uint32_t cur, uint32_t prev) {
-    if ((cur & 0x80808080u) == 0) {
-        return (prev >= 0xC0000000u) | (((prev >> 16) & 0xE0u) == 0xE0u) |
-               (((prev >> 8) & 0xF0u) == 0xF0u);
-    }
-    const uint32_t cc = cur & ~(cur << 1) & 0x80808080u;
-    const uint32_t cp = prev & ~(prev << 1) & 0x80808080u;
-    const uint32_t ec = cur & (cur << 1) & (cur << 2) & 0x80808080u;
-    const uint32_t ep = prev & (prev << 1) & (prev << 2) & 0x80808080u;
-    const uint32_t fc = ec & (cur << 3);
-    const uint32_t fp = ep & (prev << 3);
-    const uint32_t p1 = (uint32_t)((((uint64_t)cc << 32) | cp) >> 24);
-    const uint32_t p2 = (uint32_t)((((uint64_t)ec << 32) | ep) >> 16);
-    const uint32_t p3 = (uint32_t)((((uint64_t)fc << 32) | fp) >> 8);
-    return ((p1 & cc) ^ (p2 | p3)) & 0x80808080u;
-}
-
-// Unescaped quotes of a 16-bit segment given the parity `ov` of the backslash
-// run ending just before it (findEscapedQuoteMerge_NEW,
-// parse_standard_json.cu:820-830, on 16-bit words).
-__device__ __forceinline__ uint32_t unescaped16(uint32_t bs, uint32_t q, uint32_t ov) {
-    const uint32_t even = 0x5555u, odd = 0xAAAAu;
-    bs &= ~ov;
-    const uint32_t app = ((bs << 1) | ov) & 0xFFFFu;
-    const uint32_t oddseq = bs & odd & ~app;
-    const uint32_t seq = oddseq + bs;
-    const uint32_t inv = (seq << 1) & 0xFFFFu;
-    const uint32_t esc = (even ^ inv) & app;
-    return q & ~esc & 0xFFFFu;
-}
-
-// Inclusive prefix xor over 16 bits (prefix_xor, cujson/utils.cu:141-149).
-__device__ __forceinline__ uint32_t prefix_xor16(uint32_t x) {
-    x ^= x << 1;
-    x ^= x << 2;
-    x ^= x << 4;
-    x ^= x << 8;
-    return x & 0xFFFFu;
-}
-
-__device__ __forceinline__ uint4 load_segment(const uint8_t *base, uint64_t o, uint64_t n) {
-    if (o >= n) return make_uint4(0, 0, 0, 0);
-    uint4 r = *reinterpret_cast<const uint4 *>(base + o);
-    const uint64_t rem = n - o;
-    if (rem < 16) {
-        uint32_t w[4] = {r.x, r.y, r.z, r.w};
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int64_t keep = (int64_t)rem - 4 * k;
-            if (keep <= 0) w[k] = 0;
-            else if (keep < 4) w[k] &= (1u << (8 * keep)) - 1u;
-        }
-        r = make_uint4(w[0], w[1], w[2], w[3]);
-    }
-    return r;
-}
-
-__device__ __forceinline__ uint32_t seg_byte(const uint4 v, int j) {
-    const uint32_t w = (j < 8) ? ((j < 4) ? v.x : v.y) : ((j < 12) ? v.z : v.w);
-    return (w >> (8 * (j & 3))) & 0xFFu;
-}
-
-// ---------------------------------------------------------------------------
-// Escape / string carry as an affine map over GF(2).
-// A span of input maps the state (c = parity of the backslash run ending at
-// its start, s = in-string parity at its start) to the state at its end:
-//   c' = a.c ^ b,   s' = s ^ q ^ d.c
-// a: the span is all backslashes; b: its trailing run parity; q: parity of
-// its unescaped quotes computed with c = s = 0; d: its first non-backslash
-// byte is a quote (only that quote's escape status depends on c).
-// Bits: a=1, b=2, d=4, q=8.
-// ---------------------------------------------------------------------------
-constexpr uint32_t kMapIdentity = 1u;
-
-__device__ __forceinline__ uint32_t map_compose(uint32_t t1, uint32_t t2) {  // t1 then t2
-    const uint32_t a1 = t1 & 1, b1 = (t1 >> 1) & 1, d1 = (t1 >> 2) & 1, q1 = (t1 >> 3) & 1;
-    const uint32_t a2 = t2 & 1, b2 = (t2 >> 1) & 1, d2 = (t2 >> 2) & 1, q2 = (t2 >> 3) & 1;
-    const uint32_t a = a1 & a2, b = (a2 & b1) ^ b2, d = d1 ^ (d2 & a1), q = q1 ^ q2 ^ (d2 & b1);
-    return a | (b << 1) | (d << 2) | (q << 3);
-}
-__device__ __forceinline__ uint32_t map_apply(uint32_t t, uint32_t st) {  // st: c=1, s=2
-    const uint32_t a = t & 1, b = (t >> 1) & 1, d = (t >> 2) & 1, q = (t >> 3) & 1;
-    const uint32_t c = st & 1, s = (st >> 1) & 1;
-    return ((a & c) ^ b) | ((s ^ q ^ (d & c)) << 1);
-}
-
 // ---------------------------------------------------------------------------
 // k_reset: clears look-back state and per-unit results before a parse.
 // ---------------------------------------------------------------------------
-__global__ void k_reset(uint32_t *carry_st, unsigned long long *count_st, uint32_t ntiles,
-                        LaunchState *ls, UnitOut *uo, uint32_t n_units) {
+__global__ void k_reset(unsigned long long *tile_st, uint32_t ntiles, LaunchState *ls,
+                        UnitOut *uo, uint32_t n_units) {
     const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = i0; i < ntiles; i += stride) {
-        carry_st[i] = 0;
-        count_st[i] = 0;
-    }
+    for (uint32_t i = i0; i < 2 * ntiles; i += stride) tile_st[i] = 0;
     for (uint32_t i = i0; i < n_units; i += stride) uo[i] = UnitOut{};
     if (i0 == 0) *ls = LaunchState{};
 }
 
-// ---------------------------------------------------------------------------
-// k_tokenize: fused phase 1 + phase 2 over 16 KiB tiles.
-// ---------------------------------------------------------------------------
-template <bool JSONL>
-__global__ __launch_bounds__(kTokThreads) void k_tokenize(
-    const uint8_t *__restrict__ in, const UnitDesc *__restrict__ units, uint32_t n_units,
-    uint32_t *__restrict__ carry_st, unsigned long long *__restrict__ count_st,
-    LaunchState *__restrict__ ls, UnitOut *__restrict__ uo, uint32_t *__restrict__ out_struct,
-    uint32_t n_cap, uint8_t *__restrict__ oc_char, uint32_t *__restrict__ oc_idx,
-    uint32_t m_cap) {
-    __shared__ uint32_t s_tile, s_unit;
-    __shared__ uint32_t s_map[kWaves];
-    __shared__ uint32_t s_seff[kWaves];
-    __shared__ uint32_t s_wcnt[kWaves];
-    __shared__ uint32_t s_nexcl, s_mexcl;
-
-    const uint32_t tid = threadIdx.x;
-    const uint32_t w = tid >> 6, lane = tid & 63;
-    const uint64_t lt = lanemask_lt();
-
-    if (tid == 0) {
-        const uint32_t t = atomicAdd(&ls->tile_counter, 1u);
-        uint32_t lo = 0, hi = n_units - 1;  // last unit with tile0 <= t
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (units[mid].tile0 <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        s_tile = t;
-        s_unit = lo;
-    }
+// Per-unit pair bases for concatenated results: unit u's pair values are
+// offset by the structural count of all earlier units (the JSON Lines
+// lastStructuralIndex carry, parse_json_lines.cu:1222).
+__global__ __launch_bounds__(256) void k_unit_bases(UnitOut *__restrict__ uo, uint32_t n_units) {
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint32_t u = s_unit;
-    const UnitDesc U = units[u];
-    const uint32_t tile_in_unit = tile - U.tile0;
-    const uint8_t *ubase = in + U.in_off;
-    const uint64_t wave_off = (uint64_t)tile_in_unit * kTileBytes + (uint64_t)w * kWaveBytes;
-
-    uint4 v[kRounds];
-#pragma unroll
-    for (int i = 0; i < kRounds; i++)
-        v[i] = load_segment(ubase, wave_off + (uint64_t)i * 1024 + (uint64_t)lane * 16, U.n);
-
-    // Dword preceding the wave, for the UTF-8 check of its first word.
-    uint32_t prev_dw = 0;
-    if (wave_off > 0 && wave_off < U.n)
-        prev_dw = *reinterpret_cast<const uint32_t *>(ubase + wave_off - 4);
-
-    // ---- local pass: masks with the wave entered at (c = 0, s = 0) -----------
-    uint64_t instr64 = 0, op64 = 0, oc64 = 0;
-    uint32_t c_run = 0, s_run = 0, seen = 0, dflag = 0;
-    uint32_t utf8_bad = 0;
-    uint32_t last_dw = prev_dw;  // last dword of the previous round (wave-uniform)
-#pragma unroll
-    for (int i = 0; i < kRounds; i++) {
-        uint32_t bs, q, op, oc;
-        classify16<JSONL>(v[i], bs, q, op, oc);
-
-        // Phase 1: UTF-8 over the segment's four words (words at or past n are
-        // outside W4 and not checked, parse_standard_json.cu:236).
-        {
-            const uint32_t up = __shfl_up(v[i].w, 1, 64);
-            const uint32_t p0 = (lane == 0) ? last_dw : up;
-            const uint64_t seg_off = wave_off + (uint64_t)i * 1024 + (uint64_t)lane * 16;
-            const uint32_t anyhigh = ((v[i].x | v[i].y | v[i].z | v[i].w | p0) & 0x80808080u) != 0;
-            if (__ballot(anyhigh)) {
-                uint32_t e = 0;
-                if (seg_off < U.n) e |= utf8_word_err(v[i].x, p0);
-                if (seg_off + 4 < U.n) e |= utf8_word_err(v[i].y, v[i].x);
-                if (seg_off + 8 < U.n) e |= utf8_word_err(v[i].z, v[i].y);
-                if (seg_off + 12 < U.n) e |= utf8_word_err(v[i].w, v[i].z);
-                utf8_bad |= (e != 0);
-            }
-            last_dw = wave_read(v[i].w, 63);
+    for (uint32_t base = 0; base < n_units; base += 256) {
+        const uint32_t u = base + threadIdx.x;
+        const uint32_t n = u < n_units ? uo[u].n_end - uo[u].n_base : 0u;
+        const uint32_t incl = wave_incl_scan(n);
+        if (lane_id() == 63) wsum[threadIdx.x >> 6] = incl;
+        __syncthreads();
+        uint32_t pre = carry, tot = 0;
+        for (uint32_t k = 0; k < 4; k++) {
+            if (k < (threadIdx.x >> 6)) pre += wsum[k];
+            tot += wsum[k];
         }
-
-        // Backslash-run carry across lanes: the run entering lane l ends in the
-        // nearest lower lane that is not all backslashes (16 is even, so
-        // all-backslash lanes do not change the parity).
-        const uint32_t all_bs = (bs == 0xFFFFu);
-        const uint32_t run = __clz(~(bs << 16));
-        const uint64_t NB = __ballot(!all_bs);
-        const uint64_t TP = __ballot(run & 1u);
-        const uint64_t mm = NB & lt;
-        const uint32_t ov = mm ? (uint32_t)((TP >> (63 - __clzll(mm))) & 1ull) : c_run;
-        const uint32_t rq = unescaped16(bs, q, ov);
-
-        // In-string parity across lanes.
-        const uint32_t px = prefix_xor16(rq);
-        const uint64_t QP = __ballot(__popc(rq) & 1u);
-        const uint32_t s_lane = s_run ^ (__popcll(QP & lt) & 1u);
-        const uint32_t instr = (px ^ (s_lane ? 0xFFFFu : 0u)) & 0xFFFFu;
-
-        if (!seen && NB) {  // first non-backslash byte of the wave: is it a quote?
-            const uint32_t k0 = (uint32_t)__ffsll((long long)NB) - 1;
-            const uint32_t nbpos = (uint32_t)__ffs(~bs & 0xFFFFu) - 1;
-            const uint32_t fq = all_bs ? 0u : ((q >> nbpos) & 1u);
-            const uint64_t FQ = __ballot(fq);
-            dflag = (uint32_t)((FQ >> k0) & 1ull);
-            seen = 1;
-        }
-        if (NB) c_run = (uint32_t)((TP >> (63 - __clzll(NB))) & 1ull);
-        s_run ^= (__popcll(QP) & 1u);
-
-        instr64 |= (uint64_t)instr << (16 * i);
-        op64 |= (uint64_t)op << (16 * i);
-        oc64 |= (uint64_t)oc << (16 * i);
+        if (u < n_units) uo[u].pair_base = pre + incl - n;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
     }
+}
 
-    const uint64_t bad_lanes = __ballot(utf8_bad);
-    if (lane == 0) {
-        if (bad_lanes) atomicOr(&uo[u].utf8_err, 1u);
-        const uint32_t a = seen ? 0u : 1u;
-        s_map[w] = a | (c_run << 1) | (dflag << 2) | (s_run << 3);
+// Packs the per-unit output regions into the reference's host layout
+// (parse_json_lines.cu:1070-1089): dst[0] = 0, dst[1 + p_u + i] = structural
+// row i of unit u, dst[N + 1] = N + 1, dst[N + 2 + p_u + i] = its pair cell,
+// and the two trailing cells -1, where p_u is the unit's pair_base (the
+// structural count of the units before it).  Each thread moves kAsmRows
+// consecutive rows and finds their unit once by binary search.
+constexpr uint32_t kAsmRows = 16;
+__global__ __launch_bounds__(256) void k_assemble(const UnitOut *__restrict__ uo, uint32_t n_units,
+                                                  const uint32_t *__restrict__ st,
+                                                  const int32_t *__restrict__ pr,
+                                                  int32_t *__restrict__ dst, uint32_t N) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t r0 = t * kAsmRows;
+    if (t == 0) {
+        dst[0] = 0;
+        dst[N + 1] = (int32_t)(N + 1);
+        dst[2 * (uint64_t)N + 2] = -1;
+        dst[2 * (uint64_t)N + 3] = -1;
     }
-    __syncthreads();
-
-    // ---- carry look-back over tiles (restarts at each unit) ------------------
-    if (tid == 0) {
-        uint32_t T = kMapIdentity;
-        for (int k = 0; k < kWaves; k++) T = map_compose(T, s_map[k]);
-        uint32_t entry;  // (c, s) entering the tile
-        if (tile_in_unit == 0) {
-            entry = 0;
-            agent_store(&carry_st[tile], (kStInc << 30) | map_apply(T, 0));
-        } else {
-            agent_store(&carry_st[tile], (kStAgg << 30) | T);
-            uint32_t acc = kMapIdentity;
-            uint32_t j = tile - 1;
-            for (;;) {
-                uint32_t st;
-                while (((st = agent_load(&carry_st[j])) >> 30) == kStInvalid)
-                    __builtin_amdgcn_s_sleep(1);
-                if ((st >> 30) == kStInc) {
-                    entry = map_apply(acc, st & 3u);
-                    break;
-                }
-                acc = map_compose(st & 0xFu, acc);
-                j--;
-            }
-            agent_store(&carry_st[tile], (kStInc << 30) | map_apply(T, entry));
-        }
-        uint32_t state = entry;
-        for (int k = 0; k < kWaves; k++) {
-            const uint32_t m = s_map[k];
-            s_seff[k] = ((state >> 1) & 1u) ^ ((state & 1u) & ((m >> 2) & 1u));
-            state = map_apply(m, state);
-        }
+    if (r0 >= N) return;
+    uint32_t lo = 0, hi = n_units;  // last unit with pair_base <= r0
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (uo[mid].pair_base <= r0) lo = mid;
+        else hi = mid;
     }
-    __syncthreads();
-
-    // ---- select structurals, count ------------------------------------------
-    const uint64_t fix = s_seff[w] ? ~0ull : 0ull;
-    instr64 ^= fix;
-    const uint64_t st64 = op64 & ~instr64;
-    const uint64_t ocm64 = oc64 & ~instr64;
-    uint32_t excl[kRounds], rtot[kRounds];
-    uint32_t wtot = 0;
-#pragma unroll
-    for (int i = 0; i < kRounds; i++) {
-        const uint32_t sti = (uint32_t)(st64 >> (16 * i)) & 0xFFFFu;
-        const uint32_t oci = (uint32_t)(ocm64 >> (16 * i)) & 0xFFFFu;
-        const uint32_t cnt = (uint32_t)__popc(sti) | ((uint32_t)__popc(oci) << 16);
-        const uint32_t incl = wave_incl_scan(cnt);
-        excl[i] = incl - cnt;
-        rtot[i] = wave_read(incl, 63);
-        wtot += rtot[i];
+    uint32_t u = lo;
+    uint32_t ub = uo[u].pair_base, ue = ub + (uo[u].n_end - uo[u].n_base);
+    const uint64_t r1 = r0 + kAsmRows < N ? r0 + kAsmRows : N;
+    for (uint64_t r = r0; r < r1; r++) {
+        while (r >= ue) {  // skips units without structurals
+            u++;
+            ub = uo[u].pair_base;
+            ue = ub + (uo[u].n_end - uo[u].n_base);
+        }
+        const uint64_t src = uo[u].n_base + (r - ub);
+        dst[1 + r] = (int32_t)st[src];
+        dst[(uint64_t)N + 2 + r] = pr[src];
     }
-    if (lane == 0) s_wcnt[w] = wtot;
-    __syncthreads();
-
-    // ---- count look-back over all tiles of the launch ------------------------
-    if (tid == 0) {
-        uint32_t nt = 0, mt = 0;
-        for (int k = 0; k < kWaves; k++) {
-            nt += s_wcnt[k] & 0xFFFFu;
-            mt += s_wcnt[k] >> 16;
-        }
-        const unsigned long long agg = ((unsigned long long)nt << 31) | mt;
-        const unsigned long long fl_mask = 3ull << 62;
-        unsigned long long excl_sum = 0;
-        if (tile == 0) {
-            agent_store64(&count_st[tile], ((unsigned long long)kStInc << 62) | agg);
-        } else {
-            agent_store64(&count_st[tile], ((unsigned long long)kStAgg << 62) | agg);
-            uint32_t j = tile - 1;
-            for (;;) {
-                unsigned long long st;
-                while (((st = agent_load64(&count_st[j])) >> 62) == kStInvalid)
-                    __builtin_amdgcn_s_sleep(1);
-                excl_sum += st & ~fl_mask;
-                if ((st >> 62) == kStInc) break;
-                j--;
-            }
-            agent_store64(&count_st[tile], ((unsigned long long)kStInc << 62) | (excl_sum + agg));
-        }
-        const uint32_t nex = (uint32_t)(excl_sum >> 31);
-        const uint32_t mex = (uint32_t)(excl_sum & 0x7FFFFFFFull);
-        s_nexcl = nex;
-        s_mexcl = mex;
-        if (tile_in_unit == 0) {
-            uo[u].n_base = nex;
-            uo[u].m_base = mex;
-        }
-        if (tile_in_unit == U.ntiles - 1) {
-            uo[u].n_end = nex + nt;
-            uo[u].m_end = mex + mt;
-        }
-        if (tile == gridDim.x - 1) {
-            ls->n_total = nex + nt;
-            ls->m_total = mex + mt;
-        }
-    }
-    __syncthreads();
-
-    // ---- compaction: write positions and the bracket arrays ------------------
-    uint32_t nb = s_nexcl, mb = s_mexcl;
-    for (uint32_t k = 0; k < w; k++) {
-        nb += s_wcnt[k] & 0xFFFFu;
-        mb += s_wcnt[k] >> 16;
-    }
-    uint32_t overflow = 0;
-#pragma unroll
-    for (int i = 0; i < kRounds; i++) {
-        uint32_t sti = (uint32_t)(st64 >> (16 * i)) & 0xFFFFu;
-        const uint32_t oci = (uint32_t)(ocm64 >> (16 * i)) & 0xFFFFu;
-        uint32_t idx = nb + (excl[i] & 0xFFFFu);
-        uint32_t midx = mb + (excl[i] >> 16);
-        const uint64_t pos0 =
-            U.pos_base + wave_off + (uint64_t)i * 1024 + (uint64_t)lane * 16 + 1;
-        while (sti) {
-            const int j = __ffs(sti) - 1;
-            sti &= sti - 1;
-            if (idx < n_cap) out_struct[idx] = (uint32_t)(pos0 + (uint64_t)j);
-            else overflow = 1;
-            if ((oci >> j) & 1u) {
-                if (midx < m_cap) {
-                    oc_char[midx] = (uint8_t)seg_byte(v[i], j);
-                    oc_idx[midx] = idx;
-                } else {
-                    overflow = 1;
-                }
-                midx++;
-            }
-            idx++;
-        }
-        nb += rtot[i] & 0xFFFFu;
-        mb += rtot[i] >> 16;
-    }
-    if (__ballot(overflow) && lane == 0) atomicOr(&ls->overflow, 1u);
 }
 
 // ---------------------------------------------------------------------------
@@ -639,7 +286,7 @@ __global__ __launch_bounds__(kP3Threads) void k_scatter(
 __global__ __launch_bounds__(kP3Threads) void k_pairs(
     const uint8_t *__restrict__ oc_char, const uint32_t *__restrict__ oc_idx,
     const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ spos,
-    UnitOut *__restrict__ uo, uint32_t u, int32_t *__restrict__ pair_row, uint32_t local_pairs,
+    UnitOut *__restrict__ uo, uint32_t u, int32_t *__restrict__ pair_row,
     const LaunchState *__restrict__ ls) {
     __shared__ int32_t row[kPairChunk];
     if (ls->overflow) return;
@@ -650,7 +297,7 @@ __global__ __launch_bounds__(kP3Threads) void k_pairs(
     const uint32_t mb = uo[u].m_base, me = uo[u].m_end;
     const uint32_t N = ne - nb, M = me - mb;
     const uint32_t last_group = M ? 4u * ((M + 3u) / 4u - 1u) : 0u;
-    const uint32_t sub = local_pairs ? nb : 0u;
+    const uint32_t pbase = uo[u].pair_base + 1u;
     const uint32_t nchunks = (N + kPairChunk - 1) / kPairChunk;
     int32_t err = 0;
     for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
@@ -679,7 +326,7 @@ __global__ __launch_bounds__(kP3Threads) void k_pairs(
             const int c1 = (int)(signed char)oc_char[m2];
             if ((c0 ^ c1) != 6) err = 1;
             if (s < last_group && c0 > c1) err = 1;
-            row[oc_idx[m] - t0] = (int32_t)(oc_idx[m2] + 1u - sub);
+            row[oc_idx[m] - t0] = (int32_t)(oc_idx[m2] - nb + pbase);
         }
         __syncthreads();
         for (uint32_t i = tid; i < t1 - t0; i += kP3Threads) pair_row[t0 + i] = row[i];
@@ -690,14 +337,6 @@ __global__ __launch_bounds__(kP3Threads) void k_pairs(
     if (tid == 0 && err) atomicOr(&uo[u].struct_err, 1u);
 }
 
-// Explicit instantiations used by the host driver.
-template __global__ void k_tokenize<false>(const uint8_t *, const UnitDesc *, uint32_t, uint32_t *,
-                                           unsigned long long *, LaunchState *, UnitOut *,
-                                           uint32_t *, uint32_t, uint8_t *, uint32_t *, uint32_t);
-template __global__ void k_tokenize<true>(const uint8_t *, const UnitDesc *, uint32_t, uint32_t *,
-                                          unsigned long long *, LaunchState *, UnitOut *,
-                                          uint32_t *, uint32_t, uint8_t *, uint32_t *, uint32_t);
-
 }  // namespace cjx
 
 // ---------------------------------------------------------------------------
@@ -706,32 +345,32 @@ template __global__ void k_tokenize<true>(const uint8_t *, const UnitDesc *, uin
 namespace cjx {
 
 hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uint32_t n_units) {
-    const uint32_t work = ntiles > n_units ? ntiles : n_units;
+    const uint32_t work = 2 * ntiles > n_units ? 2 * ntiles : n_units;
     uint32_t grid = (work + 255) / 256;
     if (grid < 1) grid = 1;
     if (grid > 4096) grid = 4096;
-    hipLaunchKernelGGL(k_reset, dim3(grid), dim3(256), 0, s, b.carry_st, b.count_st, ntiles, b.ls,
-                       b.uout, n_units);
+    hipLaunchKernelGGL(k_reset, dim3(grid), dim3(256), 0, s, b.tile_st, ntiles, b.ls, b.uout,
+                       n_units);
     return hipGetLastError();
 }
 
-hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                           uint32_t ntiles, uint32_t n_units, bool jsonl, uint32_t *out_struct,
-                           uint32_t n_cap, uint32_t m_cap) {
-    if (jsonl)
-        hipLaunchKernelGGL(k_tokenize<true>, dim3(ntiles), dim3(kTokThreads), 0, s, d_in, b.units,
-                           n_units, b.carry_st, b.count_st, b.ls, b.uout, out_struct, n_cap,
-                           b.oc_char, b.oc_idx, m_cap);
-    else
-        hipLaunchKernelGGL(k_tokenize<false>, dim3(ntiles), dim3(kTokThreads), 0, s, d_in,
-                           b.units, n_units, b.carry_st, b.count_st, b.ls, b.uout, out_struct,
-                           n_cap, b.oc_char, b.oc_idx, m_cap);
+hipError_t launch_assemble(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
+                           const uint32_t *d_struct, const int32_t *d_pair, int32_t *dst,
+                           uint32_t N) {
+    const uint64_t threads = ((uint64_t)N + kAsmRows - 1) / kAsmRows;
+    const uint32_t grid = (uint32_t)((threads + 255) / 256) + 1;
+    hipLaunchKernelGGL(k_assemble, dim3(grid), dim3(256), 0, s, b.uout, n_units, d_struct, d_pair,
+                       dst, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units) {
+    hipLaunchKernelGGL(k_unit_bases, dim3(1), dim3(256), 0, s, b.uout, n_units);
     return hipGetLastError();
 }
 
 hipError_t launch_phase3(hipStream_t s, const LaunchBufs &b, uint32_t u, uint32_t G,
-                         uint32_t pair_grid, bool depth_u16, int32_t *pair_row,
-                         bool local_pairs) {
+                         uint32_t pair_grid, bool depth_u16, int32_t *pair_row) {
     const uint32_t kmask = depth_u16 ? 0xFFFFu : 0xFFu;
     hipLaunchKernelGGL(k_depth_totals, dim3(G), dim3(kP3Threads), 0, s, b.oc_char, b.uout, u,
                        b.block_delta, b.ls);
@@ -753,7 +392,7 @@ hipError_t launch_phase3(hipStream_t s, const LaunchBufs &b, uint32_t u, uint32_
                            b.uout, u, 8u, b.hist, b.tmp, b.sorted, b.spos, b.ls);
     }
     hipLaunchKernelGGL(k_pairs, dim3(pair_grid), dim3(kP3Threads), 0, s, b.oc_char, b.oc_idx,
-                       b.sorted, b.spos, b.uout, u, pair_row, local_pairs ? 1u : 0u, b.ls);
+                       b.sorted, b.spos, b.uout, u, pair_row, b.ls);
     return hipGetLastError();
 }
 

@@ -1,0 +1,547 @@
+// cjx_tokenize.hip - fused phase 1 + phase 2 (UTF-8 check and tokenization).
+//
+// One pass over the input: every byte is read once from HBM, classified in
+// registers, and only the compact outputs (structural positions, bracket chars
+// and bracket structural indices) are written back.  The sequential
+// dependencies that the reference spreads over separate kernels and Thrust
+// scans (cujson/parse_standard_json.cu:446-1343) are carried by
+//   * in-lane recurrences on 32-bit masks,
+//   * ballots across the 64 lanes of a wave,
+//   * a decoupled look-back across 16 KiB tiles: the escape/string state as an
+//     affine map over GF(2) (4 bits), and the structural and bracket counts
+//     as one packed 64-bit prefix sum.
+//
+// Lane layout: each lane owns 32 contiguous input bytes per round.  The eight
+// words are byte-transposed with v_perm (word d, byte b <- input byte 8b+d),
+// so that shifting one class bit per word and OR-ing gives the linear 32-bit
+// mask, and so that "previous byte" is the previous word in the UTF-8 rule.
+//
+// Classification is three v_perm table lookups ANDed together, indexed by
+// byte bits 0-2, 3-5 and 6-7.  Every class needed is a product set in those
+// three coordinates, so the AND is exact:
+//   bit 0 ':' (and '\n' for JSON Lines)   bit 4 E    (>= 0xE0)
+//   bit 1 ','                              bit 5 F    (>= 0xF0)
+//   bit 2 '[' ']' '{' '}'                  bit 6 '\\'
+//   bit 3 CONT (0x80-0xBF)                 bit 7 '"'
+#include "cjx_device.h"
+
+namespace cjx {
+
+// Table entries, byte i = value for index i.
+constexpr uint64_t kLutBits02 = 0x38383C7A3CB93838ull;        // by bits 0-2
+constexpr uint64_t kLutBits35Std = 0x3D381A984C080808ull;     // by bits 3-5
+constexpr uint64_t kLutBits35Lines = 0x3D381A984C080908ull;   // '\n' joins ':'
+constexpr uint64_t kLutBits67 = 0x0000000030084483ull;        // by bits 6-7
+
+
+__device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) {
+    return __builtin_amdgcn_perm(hi, lo, sel);
+}
+
+template <bool JSONL>
+__device__ __forceinline__ uint32_t classify(uint32_t r) {
+    constexpr uint64_t t35 = JSONL ? kLutBits35Lines : kLutBits35Std;
+    const uint32_t a = perm((uint32_t)(kLutBits02 >> 32), (uint32_t)kLutBits02, r & 0x07070707u);
+    const uint32_t b = perm((uint32_t)(t35 >> 32), (uint32_t)t35, (r >> 3) & 0x07070707u);
+    const uint32_t c = perm(0u, (uint32_t)kLutBits67, (r >> 6) & 0x03030303u);
+    return a & b & c;
+}
+
+// 4x4 byte transpose: o_k = [a.k, b.k, c.k, e.k].
+__device__ __forceinline__ void transpose4(uint32_t a, uint32_t b, uint32_t c, uint32_t e,
+                                           uint32_t &o0, uint32_t &o1, uint32_t &o2,
+                                           uint32_t &o3) {
+    const uint32_t ab_lo = perm(b, a, 0x05010400u), ab_hi = perm(b, a, 0x07030602u);
+    const uint32_t ce_lo = perm(e, c, 0x05010400u), ce_hi = perm(e, c, 0x07030602u);
+    o0 = perm(ce_lo, ab_lo, 0x05040100u);
+    o1 = perm(ce_lo, ab_lo, 0x07060302u);
+    o2 = perm(ce_hi, ab_hi, 0x05040100u);
+    o3 = perm(ce_hi, ab_hi, 0x07060302u);
+}
+
+// Linear 32-bit mask (bit i <-> byte i of the lane) of class bit K.
+template <int K>
+__device__ __forceinline__ uint32_t extract(const uint32_t (&t)[8]) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+        const uint32_t v = (d >= K) ? (t[d] << (d - K)) : (t[d] >> (K - d));
+        acc |= v & (0x01010101u << d);
+    }
+    return acc;
+}
+
+// Unescaped quotes given the parity `ov` of the backslash run ending just
+// before the lane (findEscapedQuoteMerge_NEW, parse_standard_json.cu:820-830).
+__device__ __forceinline__ uint32_t unescaped32(uint32_t bs, uint32_t q, uint32_t ov) {
+    const uint32_t even = 0x55555555u, odd = 0xAAAAAAAAu;
+    bs &= ~ov;
+    const uint32_t app = (bs << 1) | ov;
+    const uint32_t oddseq = bs & odd & ~app;
+    const uint32_t seq = oddseq + bs;
+    const uint32_t esc = (even ^ (seq << 1)) & app;
+    return q & ~esc;
+}
+
+// Inclusive prefix xor (prefix_xor, cujson/utils.cu:141-149).
+__device__ __forceinline__ uint32_t prefix_xor32(uint32_t x) {
+    x ^= x << 1;
+    x ^= x << 2;
+    x ^= x << 4;
+    x ^= x << 8;
+    x ^= x << 16;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t alignbyte3(uint32_t hi, uint32_t lo) {
+    return __builtin_amdgcn_alignbyte(hi, lo, 3u);
+}
+
+__device__ __forceinline__ void load_lane(const uint8_t *base, uint64_t o, uint64_t n,
+                                          uint32_t (&x)[8]) {
+    if (o >= n) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) x[k] = 0;
+        return;
+    }
+    const uint4 v0 = *reinterpret_cast<const uint4 *>(base + o);
+    uint4 v1 = make_uint4(0, 0, 0, 0);
+    if (o + 16 < n) v1 = *reinterpret_cast<const uint4 *>(base + o + 16);
+    x[0] = v0.x; x[1] = v0.y; x[2] = v0.z; x[3] = v0.w;
+    x[4] = v1.x; x[5] = v1.y; x[6] = v1.z; x[7] = v1.w;
+    const uint64_t rem = n - o;
+    if (rem < 32) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int64_t keep = (int64_t)rem - 4 * k;
+            if (keep <= 0) x[k] = 0;
+            else if (keep < 4) x[k] &= (1u << (8 * keep)) - 1u;
+        }
+    }
+}
+
+// Reference UTF-8 rule (checkUTF8, parse_standard_json.cu:210-262, reduced to
+// what survives :122-123) over the lane's eight words, on transposed data:
+// word 2s is slot s of transposed words 0-3, word 2s+1 slot s of words 4-7.
+//  * a word with a high byte: byte p errs iff
+//      [p-1 and p are continuations] != [p-2 >= E0 or p-3 >= F0]
+//  * a word without one errs iff the previous word ends in a byte >= C0,
+//    or its third byte is >= E0, or its second >= F0 (:219-223, :239-242)
+// pk5/pk6/pk7/pr7 carry the previous lane's transposed words 5, 6, 7 (classes)
+// and 7 (raw).  vA/vB: 0xFF in slot s if word 2s / 2s+1 lies inside the unit.
+__device__ __forceinline__ uint32_t utf8_lane_error(const uint32_t (&r)[8], const uint32_t (&k)[8],
+                                                    uint32_t pk5, uint32_t pk6, uint32_t pk7,
+                                                    uint32_t pr7, uint32_t vA, uint32_t vB) {
+    const uint32_t ks5 = alignbyte3(k[5], pk5), ks6 = alignbyte3(k[6], pk6);
+    const uint32_t ks7 = alignbyte3(k[7], pk7), rs7 = alignbyte3(r[7], pr7);
+    const uint32_t e0 = (ks7 & k[0]) ^ ((ks6 >> 1) | (ks5 >> 2));
+    const uint32_t e1 = (k[0] & k[1]) ^ ((ks7 >> 1) | (ks6 >> 2));
+    const uint32_t e2 = (k[1] & k[2]) ^ ((k[0] >> 1) | (ks7 >> 2));
+    const uint32_t e3 = (k[2] & k[3]) ^ ((k[1] >> 1) | (k[0] >> 2));
+    const uint32_t e4 = (k[3] & k[4]) ^ ((k[2] >> 1) | (k[1] >> 2));
+    const uint32_t e5 = (k[4] & k[5]) ^ ((k[3] >> 1) | (k[2] >> 2));
+    const uint32_t e6 = (k[5] & k[6]) ^ ((k[4] >> 1) | (k[3] >> 2));
+    const uint32_t e7 = (k[6] & k[7]) ^ ((k[5] >> 1) | (k[4] >> 2));
+    const uint32_t errA = (e0 | e1 | e2 | e3) & 0x08080808u;
+    const uint32_t errB = (e4 | e5 | e6 | e7) & 0x08080808u;
+    const uint32_t gA = (r[0] | r[1] | r[2] | r[3]) & 0x80808080u;
+    const uint32_t gB = (r[4] | r[5] | r[6] | r[7]) & 0x80808080u;
+    const uint32_t gen = ((errA & (gA >> 4)) & vA) | ((errB & (gB >> 4)) & vB);
+    const uint32_t ascA = ((rs7 & ~(ks7 << 4)) | (ks6 << 3) | (ks5 << 2)) & ~gA & vA;
+    const uint32_t ascB = ((r[3] & ~(k[3] << 4)) | (k[2] << 3) | (k[1] << 2)) & ~gB & vB;
+    return gen | ((ascA | ascB) & 0x80808080u);
+}
+
+// ---------------------------------------------------------------------------
+// Tile status for the decoupled look-back: two 64-bit words per tile.
+//   aggregate  A = 1<<62 | map<<58 | n0<<29 | m0     B = 1<<62 | opc<<29 | brc
+//   inclusive  A = 2<<62 | state<<60 | n_incl        B = 2<<62 | m_incl
+// n0/m0 are the tile's structural/bracket counts when it is entered with
+// e = 0, where e = s ^ (c & d) for entry state (c, s) and the tile's d bit;
+// with e = 1 every wave's string mask flips, so the counts are opc - n0 and
+// brc - m0 (opc/brc count the structural/bracket characters regardless of
+// strings).  Words are written B then A; a reader accepts a tile only when
+// both words carry the same flag.
+// ---------------------------------------------------------------------------
+constexpr unsigned long long kMask29 = (1ull << 29) - 1, kMask31 = (1ull << 31) - 1;
+
+__device__ __forceinline__ void publish_agg(unsigned long long *st, uint32_t tile, uint32_t map,
+                                            uint32_t n0, uint32_t m0, uint32_t opc,
+                                            uint32_t brc) {
+    agent_store64(&st[2 * (uint64_t)tile + 1],
+                  (1ull << 62) | ((unsigned long long)opc << 29) | brc);
+    agent_store64(&st[2 * (uint64_t)tile],
+                  (1ull << 62) | ((unsigned long long)map << 58) |
+                      ((unsigned long long)n0 << 29) | m0);
+}
+
+__device__ __forceinline__ void publish_inc(unsigned long long *st, uint32_t tile, uint32_t state,
+                                            uint32_t n_incl, uint32_t m_incl) {
+    agent_store64(&st[2 * (uint64_t)tile + 1], (2ull << 62) | m_incl);
+    agent_store64(&st[2 * (uint64_t)tile],
+                  (2ull << 62) | ((unsigned long long)state << 60) | n_incl);
+}
+
+// One wave walks back from `tile`: lane i reads tile (tile - 1 - i).  When an
+// inclusive tile k lies in the window and every tile after it is an
+// aggregate, the maps of those tiles are scanned from the oldest to give each
+// one's entry state, hence its e bit and its counts; their sum plus tile k's
+// inclusive counts is this tile's exclusive prefix.  Tiles before the unit
+// read as inclusive with state 0 and counts 0.  The walk waits while no
+// inclusive tile is within 64 tiles (the oldest running tile always finds
+// one, so waiting cannot deadlock).
+__device__ __forceinline__ void lookback_tile(const unsigned long long *st, uint32_t tile,
+                                              uint32_t unit_tile0, uint32_t &entry,
+                                              uint32_t &n_excl, uint32_t &m_excl) {
+    const uint32_t lane = lane_id();
+    const int64_t j = (int64_t)tile - 1 - (int64_t)lane;
+    for (;;) {
+        unsigned long long a = 2ull << 62, b = 2ull << 62;
+        if (j >= (int64_t)unit_tile0) {
+            a = agent_load64(&st[2 * j]);
+            b = agent_load64(&st[2 * j + 1]);
+        }
+        const uint32_t fa = (uint32_t)(a >> 62), fb = (uint32_t)(b >> 62);
+        const uint32_t kind = (fa == fb) ? fa : kStInvalid;
+        const uint64_t inc = __ballot(kind == kStInc);
+        const uint64_t notagg = __ballot(kind != kStAgg);
+        if (!inc || (notagg & ((inc & (0ull - inc)) - 1ull))) {
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        const uint32_t k = (uint32_t)__ffsll((long long)inc) - 1;
+        const uint32_t map = (lane < k) ? (uint32_t)((a >> 58) & 0xFu) : kMapIdentity;
+        // P = map(lane 63) then ... then map(lane): inclusive scan from the oldest tile.
+        uint32_t P = map;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t other = (uint32_t)__shfl_down((int)P, off, 64);
+            if (lane + off < 64) P = map_compose(other, P);
+        }
+        uint32_t E = (uint32_t)__shfl_down((int)P, 1, 64);
+        if (lane == 63) E = kMapIdentity;
+        const uint32_t sk = wave_read((uint32_t)(a >> 60) & 3u, (int)k);
+        const uint32_t stj = map_apply(E, sk);
+        const uint32_t e = ((stj >> 1) ^ (stj & (map >> 2))) & 1u;
+        const uint32_t n0 = (uint32_t)((a >> 29) & kMask29), m0 = (uint32_t)(a & kMask29);
+        const uint32_t opc = (uint32_t)((b >> 29) & kMask29), brc = (uint32_t)(b & kMask29);
+        uint32_t cn = 0, cm = 0;
+        if (lane < k) {
+            cn = e ? opc - n0 : n0;
+            cm = e ? brc - m0 : m0;
+        } else if (lane == k) {
+            cn = (uint32_t)(a & kMask31);
+            cm = (uint32_t)(b & kMask31);
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            cn += (uint32_t)__shfl_xor((int)cn, off, 64);
+            cm += (uint32_t)__shfl_xor((int)cm, off, 64);
+        }
+        n_excl = cn;
+        m_excl = cm;
+        entry = map_apply(wave_read(P, 0), sk);
+        return;
+    }
+}
+
+__device__ __forceinline__ uint32_t find_unit(const UnitDesc *__restrict__ units, uint32_t n_units,
+                                              uint32_t t) {
+    uint32_t lo = 0, hi = n_units - 1;  // last unit with tile0 <= t
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (units[mid].tile0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+    return wave_read(wave_incl_scan(x), 63);
+}
+
+// One workgroup per tile; tiles are taken from a ticket counter in launch
+// order so that every look-back waits only on running workgroups.
+template <bool JSONL>
+__global__ __launch_bounds__(kTokThreads) void k_tokenize(
+    const uint8_t *__restrict__ in, const UnitDesc *__restrict__ units, uint32_t n_units,
+    uint32_t ntiles, unsigned long long *__restrict__ tile_st, LaunchState *__restrict__ ls,
+    UnitOut *__restrict__ uo, uint32_t *__restrict__ out_struct, uint8_t *__restrict__ oc_char,
+    uint32_t *__restrict__ oc_idx) {
+    __shared__ uint32_t s_tile, s_unit;
+    __shared__ uint32_t s_map[kWaves];
+    __shared__ uint32_t s_cnt0[kWaves], s_cnt_all[kWaves];
+    __shared__ uint32_t s_fix[kWaves], s_nbase[kWaves], s_mbase[kWaves];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t w = tid >> 6, lane = tid & 63;
+    const uint64_t lt = lanemask_lt();
+
+    if (tid == 0) {
+        const uint32_t t = atomicAdd(&ls->tile_counter, 1u);
+        s_tile = t;
+        s_unit = find_unit(units, n_units, t);
+    }
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t u = s_unit;
+    const UnitDesc U = units[u];
+    const uint32_t tile_in_unit = tile - U.tile0;
+    const uint8_t *ubase = in + U.in_off;
+    const uint64_t wave_off = (uint64_t)tile_in_unit * kTileBytes + (uint64_t)w * kWaveBytes;
+
+    // Rounds stream through a double buffer: round i+2 is loaded as soon as
+    // round i has been transposed.
+    uint32_t x[2][8];
+    load_lane(ubase, wave_off + (uint64_t)lane * kLaneBytes, U.n, x[0]);
+    if (kTokRounds > 1)
+        load_lane(ubase, wave_off + kRoundBytes + (uint64_t)lane * kLaneBytes, U.n, x[1]);
+    uint32_t prev_dw = 0;
+    if (wave_off > 0 && wave_off < U.n)
+        prev_dw = *reinterpret_cast<const uint32_t *>(ubase + wave_off - 4);
+
+    // Carries entering the wave: the four bytes before it (UTF-8 context).
+    uint32_t c_k5, c_k6, c_k7, c_r7;
+    {
+        const uint32_t cp = classify<JSONL>(prev_dw);
+        c_k7 = cp;
+        c_k6 = cp << 8;
+        c_k5 = cp << 16;
+        c_r7 = prev_dw;
+    }
+
+    uint32_t instr_r[kTokRounds], op_r[kTokRounds], br_r[kTokRounds];
+    uint32_t c_run = 0, s_run = 0, seen = 0, dflag = 0;
+    uint32_t utf8_bad = 0;
+    uint32_t lane_cnt0 = 0, lane_all = 0;  // packed n | m << 16, string fix 0 / regardless
+
+#pragma unroll
+    for (int i = 0; i < kTokRounds; i++) {
+        const uint64_t lane_off = wave_off + (uint64_t)i * kRoundBytes + (uint64_t)lane * kLaneBytes;
+        uint32_t r[8], k[8];
+        transpose4(x[i & 1][0], x[i & 1][2], x[i & 1][4], x[i & 1][6], r[0], r[1], r[2], r[3]);
+        transpose4(x[i & 1][1], x[i & 1][3], x[i & 1][5], x[i & 1][7], r[4], r[5], r[6], r[7]);
+        if (i + 2 < kTokRounds)
+            load_lane(ubase, lane_off + 2 * kRoundBytes, U.n, x[i & 1]);
+#pragma unroll
+        for (int d = 0; d < 8; d++) k[d] = classify<JSONL>(r[d]);
+
+        // ---- phase 1: UTF-8 ---------------------------------------------------
+        {
+            const uint32_t pk5 = wave_shr1(k[5], c_k5), pk6 = wave_shr1(k[6], c_k6);
+            const uint32_t pk7 = wave_shr1(k[7], c_k7), pr7 = wave_shr1(r[7], c_r7);
+            const uint32_t high = ((r[0] | r[1] | r[2] | r[3] | r[4] | r[5] | r[6] | r[7]) &
+                                   0x80808080u) |
+                                  ((pk5 | pk6 | pk7) & 0x38000000u) | (pr7 & 0x80000000u);
+            if (__ballot(high != 0)) {
+                uint32_t vA = ~0u, vB = ~0u;
+                if (lane_off + 32 > U.n) {
+                    vA = vB = 0;
+#pragma unroll
+                    for (int s = 0; s < 4; s++) {
+                        if (lane_off + 8 * s < U.n) vA |= 0xFFu << (8 * s);
+                        if (lane_off + 8 * s + 4 < U.n) vB |= 0xFFu << (8 * s);
+                    }
+                }
+                if (high && utf8_lane_error(r, k, pk5, pk6, pk7, pr7, vA, vB)) utf8_bad = 1;
+            }
+            c_k5 = wave_read(k[5], 63);
+            c_k6 = wave_read(k[6], 63);
+            c_k7 = wave_read(k[7], 63);
+            c_r7 = wave_read(r[7], 63);
+        }
+
+        // ---- phase 2: masks, escapes, strings ------------------------------
+        uint32_t u7[8];
+#pragma unroll
+        for (int d = 0; d < 8; d++) u7[d] = (k[d] & 0x07070707u) + 0x7F7F7F7Fu;
+        const uint32_t bs = extract<6>(k);
+        const uint32_t q = extract<7>(k);
+        op_r[i] = extract<7>(u7);
+        br_r[i] = extract<2>(k) & op_r[i];
+
+        const uint32_t all_bs = (bs == 0xFFFFFFFFu);
+        const uint32_t run = (uint32_t)__clz(~bs);  // trailing backslash run of the lane
+        const uint64_t NB = __ballot(!all_bs);
+        const uint64_t TP = __ballot(run & 1u);
+        const uint64_t mm = NB & lt;
+        const uint32_t ov = mm ? (uint32_t)((TP >> (63 - __clzll(mm))) & 1ull) : c_run;
+        const uint32_t rq = unescaped32(bs, q, ov);
+        const uint32_t px = prefix_xor32(rq);
+        const uint64_t QP = __ballot(__popc(rq) & 1u);
+        const uint32_t s_lane = s_run ^ (uint32_t)(__popcll(QP & lt) & 1);
+        instr_r[i] = px ^ (s_lane ? ~0u : 0u);
+        lane_cnt0 += (uint32_t)__popc(op_r[i] & ~instr_r[i]) |
+                     ((uint32_t)__popc(br_r[i] & ~instr_r[i]) << 16);
+        lane_all += (uint32_t)__popc(op_r[i]) | ((uint32_t)__popc(br_r[i]) << 16);
+
+        if (!seen && NB) {  // first non-backslash byte of the wave: is it a quote?
+            const uint32_t k0 = (uint32_t)__ffsll((long long)NB) - 1;
+            const uint32_t nbpos = (uint32_t)__ffs(~bs) - 1;
+            const uint32_t fq = all_bs ? 0u : ((q >> nbpos) & 1u);
+            const uint64_t FQ = __ballot(fq);
+            dflag = (uint32_t)((FQ >> k0) & 1ull);
+            seen = 1;
+        }
+        if (NB) c_run = (uint32_t)((TP >> (63 - __clzll(NB))) & 1ull);
+        s_run ^= (uint32_t)(__popcll(QP) & 1);
+    }
+
+    const uint64_t bad_lanes = __ballot(utf8_bad);
+    const uint32_t wcnt0 = wave_sum(lane_cnt0), wall = wave_sum(lane_all);
+    if (lane == 0) {
+        if (bad_lanes) atomicOr(&uo[u].utf8_err, 1u);
+        s_map[w] = (seen ? 0u : 1u) | (c_run << 1) | (dflag << 2) | (s_run << 3);
+        s_cnt0[w] = wcnt0;
+        s_cnt_all[w] = wall;
+    }
+    __syncthreads();
+
+    // ---- one look-back: entry state and output offsets -----------------------
+    if (w == 0) {
+        // The tile entered at (c, s) = (0, 0): wave string fixes and counts.
+        uint32_t T = kMapIdentity, state = 0, n0 = 0, m0 = 0, opc = 0, brc = 0;
+        uint32_t fix0[kWaves];
+#pragma unroll
+        for (int kk = 0; kk < kWaves; kk++) {
+            const uint32_t m = s_map[kk];
+            fix0[kk] = ((state >> 1) & 1u) ^ ((state & 1u) & ((m >> 2) & 1u));
+            state = map_apply(m, state);
+            T = map_compose(T, m);
+            const uint32_t c0 = s_cnt0[kk], ca = s_cnt_all[kk];
+            n0 += fix0[kk] ? (ca & 0xFFFFu) - (c0 & 0xFFFFu) : (c0 & 0xFFFFu);
+            m0 += fix0[kk] ? (ca >> 16) - (c0 >> 16) : (c0 >> 16);
+            opc += ca & 0xFFFFu;
+            brc += ca >> 16;
+        }
+        uint32_t entry = 0, n_excl = 0, m_excl = 0;
+        if (tile_in_unit != 0) {
+            if (lane == 0) publish_agg(tile_st, tile, T, n0, m0, opc, brc);
+            lookback_tile(tile_st, tile, U.tile0, entry, n_excl, m_excl);
+        }
+        const uint32_t e = ((entry >> 1) ^ (entry & (T >> 2))) & 1u;
+        const uint32_t nt = e ? opc - n0 : n0, mt = e ? brc - m0 : m0;
+        if (lane == 0) {
+            publish_inc(tile_st, tile, map_apply(T, entry), n_excl + nt, m_excl + mt);
+            uint32_t nb = U.out_off + n_excl, mb = U.out_off + m_excl;
+#pragma unroll
+            for (int kk = 0; kk < kWaves; kk++) {
+                const uint32_t f = fix0[kk] ^ e;
+                const uint32_t c0 = s_cnt0[kk], ca = s_cnt_all[kk];
+                s_fix[kk] = f;
+                s_nbase[kk] = nb;
+                s_mbase[kk] = mb;
+                nb += f ? (ca & 0xFFFFu) - (c0 & 0xFFFFu) : (c0 & 0xFFFFu);
+                mb += f ? (ca >> 16) - (c0 >> 16) : (c0 >> 16);
+            }
+            if (tile_in_unit == 0) {
+                uo[u].n_base = U.out_off;
+                uo[u].m_base = U.out_off;
+            }
+            if (tile_in_unit == U.ntiles - 1) {
+                const uint32_t nu = n_excl + nt, mu = m_excl + mt;
+                uo[u].n_end = U.out_off + nu;
+                uo[u].m_end = U.out_off + mu;
+                if (nu > U.out_cap) atomicOr(&ls->overflow, 1u);
+                atomicAdd(&ls->n_total, nu);
+                atomicAdd(&ls->m_total, mu);
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- select structurals and compact them ---------------------------------
+    const uint32_t fix = s_fix[w] ? ~0u : 0u;
+    uint32_t nb = s_nbase[w], mb = s_mbase[w];
+    const uint32_t cap_end = U.out_off + U.out_cap;
+    uint32_t overflow = 0;
+#pragma unroll
+    for (int i = 0; i < kTokRounds; i++) {
+        const uint64_t lane_off = wave_off + (uint64_t)i * kRoundBytes + (uint64_t)lane * kLaneBytes;
+        const uint32_t sti = op_r[i] & ~(instr_r[i] ^ fix);
+        const uint32_t oci = br_r[i] & sti;
+        const uint32_t cnt = (uint32_t)__popc(sti) | ((uint32_t)__popc(oci) << 16);
+        const uint32_t incl = wave_incl_scan(cnt);
+        const uint32_t excl = incl - cnt;
+        const uint32_t rtot = wave_read(incl, 63);
+        const uint32_t idx0 = nb + (excl & 0xFFFFu);
+        uint32_t midx = mb + (excl >> 16);
+        const uint32_t pos0 = (uint32_t)(U.pos_base + lane_off + 1);
+        uint32_t m = sti;
+        if ((uint64_t)idx0 + (uint32_t)__popc(sti) <= cap_end) {
+            uint32_t *p = out_struct + idx0;
+            while (m) {
+                const uint32_t j = (uint32_t)__builtin_ctz(m);
+                m &= m - 1;
+                *p++ = pos0 + j;
+            }
+        } else if (m) {
+            overflow = 1;
+            uint32_t idx = idx0;
+            while (m) {
+                const uint32_t j = (uint32_t)__builtin_ctz(m);
+                m &= m - 1;
+                if (idx < cap_end) out_struct[idx] = pos0 + j;
+                idx++;
+            }
+        }
+        m = oci;
+        while (m) {
+            const uint32_t j = (uint32_t)__builtin_ctz(m);
+            m &= m - 1;
+            const uint32_t t = idx0 + (uint32_t)__popc(sti & ((1u << j) - 1u));
+            if (midx < cap_end) {
+                oc_idx[midx] = t;
+                oc_char[midx] = ubase[lane_off + j];
+            } else {
+                overflow = 1;
+            }
+            midx++;
+        }
+        nb += rtot & 0xFFFFu;
+        mb += rtot >> 16;
+    }
+    if (__ballot(overflow) && lane == 0) atomicOr(&ls->overflow, 1u);
+}
+
+hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
+                           uint32_t ntiles, uint32_t n_units, bool jsonl, uint32_t *out_struct) {
+    if (jsonl)
+        hipLaunchKernelGGL(k_tokenize<true>, dim3(ntiles), dim3(kTokThreads), 0, s, d_in, b.units,
+                           n_units, ntiles, b.tile_st, b.ls, b.uout, out_struct, b.oc_char,
+                           b.oc_idx);
+    else
+        hipLaunchKernelGGL(k_tokenize<false>, dim3(ntiles), dim3(kTokThreads), 0, s, d_in, b.units,
+                           n_units, ntiles, b.tile_st, b.ls, b.uout, out_struct, b.oc_char,
+                           b.oc_idx);
+    return hipGetLastError();
+}
+
+}  // namespace cjx
+
+// Host-side model of the byte classifier (same tables, v_perm emulated), for
+// the table self-test that runs without a GPU.
+static uint32_t host_perm(uint32_t hi, uint32_t lo, uint32_t sel) {
+    const uint64_t data = ((uint64_t)hi << 32) | lo;
+    uint32_t out = 0;
+    for (int i = 0; i < 4; i++) {
+        const uint32_t s = (sel >> (8 * i)) & 0xFFu;
+        uint32_t v;
+        if (s >= 13) v = 0xFF;
+        else if (s == 12) v = 0;
+        else if (s >= 8) v = ((data >> (15 + 16 * (s - 8))) & 1) ? 0xFF : 0;
+        else v = (uint32_t)(data >> (8 * s)) & 0xFFu;
+        out |= v << (8 * i);
+    }
+    return out;
+}
+
+extern "C" uint32_t cjx_debug_byte_class(uint32_t byte, int jsonl) {
+    using namespace cjx;
+    const uint64_t t35 = jsonl ? kLutBits35Lines : kLutBits35Std;
+    const uint32_t r = byte & 0xFFu;
+    const uint32_t a = host_perm((uint32_t)(kLutBits02 >> 32), (uint32_t)kLutBits02, r & 7u);
+    const uint32_t b = host_perm((uint32_t)(t35 >> 32), (uint32_t)t35, (r >> 3) & 7u);
+    const uint32_t c = host_perm(0u, (uint32_t)kLutBits67, (r >> 6) & 3u);
+    return (a & b & c) & 0xFFu;
+}

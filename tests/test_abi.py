@@ -73,3 +73,31 @@ def test_context_without_gpu_fails_loudly():
     assert cj.lib().cjx_context_create(0, ctypes.byref(h)) == cj.CJX_ERR_DEVICE
     with pytest.raises(RuntimeError):
         cj.Context(0)
+
+
+def test_byte_class_tables():
+    """The tokenizer's three-lookup byte classifier (host model of the same
+    tables and of v_perm) against the class definitions, for all 256 bytes."""
+    f = cj.lib().cjx_debug_byte_class
+    f.argtypes = [ctypes.c_uint32, ctypes.c_int]
+    f.restype = ctypes.c_uint32
+    for jsonl in (0, 1):
+        for b in range(256):
+            want = 0
+            if b == 0x3A or (jsonl and b == 0x0A):
+                want |= 0x01
+            if b == 0x2C:
+                want |= 0x02
+            if b in (0x5B, 0x5D, 0x7B, 0x7D):
+                want |= 0x04
+            if 0x80 <= b <= 0xBF:
+                want |= 0x08
+            if b >= 0xE0:
+                want |= 0x10
+            if b >= 0xF0:
+                want |= 0x20
+            if b == 0x5C:
+                want |= 0x40
+            if b == 0x22:
+                want |= 0x80
+            assert f(b, jsonl) == want, (jsonl, hex(b), hex(f(b, jsonl)), hex(want))
