@@ -85,6 +85,7 @@ struct cjx_context {
     DevBuf in, structural, pair;
     DevBuf units, uout, ls, tile_st, layout;
     DevBuf oc_char, oc_idx, keys, sorted, tmp, spos, block_delta, hist;
+    DevBuf ck, gmin, gmin1, gmin2, lows, highs;
     PinnedBuf h_units, h_uout, h_ls;
     // Shape of the launch in flight (device form).
     uint32_t n_units = 0;
@@ -149,7 +150,18 @@ struct Shape {
     uint32_t ntiles, n_units;
     uint64_t rows;      // total rows of all output regions
     uint64_t max_rows;  // largest region
+    uint64_t chunks;    // bound on the matching chunks of all units
 };
+
+// Fills the row totals of a shape from the units' regions.
+void shape_rows(Shape &sh, const std::vector<UnitDesc> &units) {
+    sh.rows = sh.max_rows = sh.chunks = 0;
+    for (const UnitDesc &u : units) {
+        sh.rows += u.out_cap;
+        sh.max_rows = std::max<uint64_t>(sh.max_rows, u.out_cap);
+        sh.chunks += (u.out_cap + kMatchChunk - 1) / kMatchChunk;
+    }
+}
 
 int ensure_work(cjx_context *ctx, const Shape &sh) {
     bool ok = ctx->units.ensure(sh.n_units * sizeof(UnitDesc)) &&
@@ -159,6 +171,11 @@ int ensure_work(cjx_context *ctx, const Shape &sh) {
               ctx->oc_idx.ensure(sh.rows * 4 + 16) && ctx->keys.ensure(sh.rows * 2 + 16) &&
               ctx->sorted.ensure(sh.rows * 4 + 16) && ctx->spos.ensure(sh.rows * 4 + 16) &&
               ctx->block_delta.ensure(512 * 4) && ctx->hist.ensure(512 * 256 * 4) &&
+              ctx->ck.ensure(sh.chunks * sizeof(ChunkStat) + 16) &&
+              ctx->gmin.ensure(sh.chunks * 4 + 16) && ctx->gmin1.ensure(sh.chunks / 64 * 4 + 16) &&
+              ctx->gmin2.ensure(sh.chunks / 4096 * 4 + 16) &&
+              ctx->lows.ensure(sh.chunks * kMatchKeys * 4 + 16) &&
+              ctx->highs.ensure(sh.chunks * kMatchKeys * 4 + 16) &&
               ctx->h_units.ensure(sh.n_units * sizeof(UnitDesc)) &&
               ctx->h_uout.ensure(sh.n_units * sizeof(UnitOut)) &&
               ctx->h_ls.ensure(sizeof(LaunchState));
@@ -179,6 +196,12 @@ LaunchBufs bufs_of(cjx_context *ctx) {
     b.spos = ctx->spos.as<uint32_t>();
     b.block_delta = ctx->block_delta.as<int32_t>();
     b.hist = ctx->hist.as<uint32_t>();
+    b.ck = ctx->ck.as<ChunkStat>();
+    b.gmin = ctx->gmin.as<int32_t>();
+    b.gmin1 = ctx->gmin1.as<int32_t>();
+    b.gmin2 = ctx->gmin2.as<int32_t>();
+    b.lows = ctx->lows.as<uint32_t>();
+    b.highs = ctx->highs.as<uint32_t>();
     return b;
 }
 
@@ -194,11 +217,13 @@ int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsign
     hipEvent_t *tev = ctx->tev[ctx->timing_count % cjx_context::kTimingRing];
     if (ctx->timing) (void)hipEventRecord(tev[0], s);
     if (launch_tokenize(s, b, d_in, sh.ntiles, sh.n_units, (flags & CJX_FLAG_JSONL) != 0,
-                        d_struct) != hipSuccess)
+                        d_struct, d_pair) != hipSuccess)
         return CJX_ERR_DEVICE;
     if (ctx->timing) (void)hipEventRecord(tev[1], s);
     // Concatenated results number pairs across units (JSON Lines).
-    if (sh.n_units > 1 && launch_unit_bases(s, b, sh.n_units) != hipSuccess)
+    if (launch_unit_bases(s, b, sh.n_units, sh.n_units > 1) != hipSuccess) return CJX_ERR_DEVICE;
+    // Well-nested units are matched directly; the rest take the sort path.
+    if (launch_match(s, b, sh.n_units, (uint32_t)sh.chunks, d_pair) != hipSuccess)
         return CJX_ERR_DEVICE;
     const uint32_t G = phase3_blocks(sh.max_rows);
     const uint32_t PG = pair_blocks(sh.max_rows);
@@ -255,11 +280,10 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
                            hipMemcpyHostToDevice, s) != hipSuccess)
             return CJX_ERR_DEVICE;
 
-    Shape sh{ntiles, (uint32_t)units.size(), 0, 0};
+    Shape sh{ntiles, (uint32_t)units.size(), 0, 0, 0};
     for (int attempt = 0; attempt < 2; attempt++) {
-        sh.rows = set_regions(units, attempt == 1);
-        sh.max_rows = 0;
-        for (const UnitDesc &u : units) sh.max_rows = std::max<uint64_t>(sh.max_rows, u.out_cap);
+        set_regions(units, attempt == 1);
+        shape_rows(sh, units);
         if ((rc = ensure_work(ctx, sh))) return rc;
         if (!ctx->structural.ensure(sh.rows * 4 + 16) || !ctx->pair.ensure(sh.rows * 4 + 16))
             return CJX_ERR_ALLOC;
@@ -384,7 +408,9 @@ void cjx_context_destroy(cjx_context *ctx) {
     DevBuf *bufs[] = {&ctx->in,      &ctx->structural, &ctx->pair,   &ctx->units,
                       &ctx->uout,    &ctx->ls,         &ctx->tile_st, &ctx->layout,
                       &ctx->oc_char, &ctx->oc_idx,     &ctx->keys,   &ctx->sorted,
-                      &ctx->tmp,     &ctx->spos,       &ctx->block_delta, &ctx->hist};
+                      &ctx->tmp,     &ctx->spos,       &ctx->block_delta, &ctx->hist,
+                      &ctx->ck,      &ctx->gmin,       &ctx->gmin1,  &ctx->gmin2,
+                      &ctx->lows,    &ctx->highs};
     for (DevBuf *b : bufs) b->release();
     ctx->h_units.release();
     ctx->h_uout.release();
@@ -573,7 +599,8 @@ int cjx_parse_device(cjx_context *ctx, const uint8_t *d_input, size_t n, unsigne
     const uint32_t ntiles = plan_units({n}, {0}, units, dev_bytes);
     units[0].out_off = 0;
     units[0].out_cap = (uint32_t)std::min<uint64_t>(cap, n);
-    Shape sh{ntiles, 1, units[0].out_cap, units[0].out_cap};
+    Shape sh{ntiles, 1, 0, 0, 0};
+    shape_rows(sh, units);
     if ((rc = ensure_work(ctx, sh))) return rc;
     std::memcpy(ctx->h_units.p, units.data(), sizeof(UnitDesc));
     if (hipMemcpyAsync(ctx->units.p, ctx->h_units.p, sizeof(UnitDesc), hipMemcpyHostToDevice,

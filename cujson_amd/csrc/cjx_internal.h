@@ -6,14 +6,15 @@
 
 namespace cjx {
 
-// Tokenizer tile geometry: a tile is one 256-thread workgroup (4 wave64s);
-// each lane owns 32 contiguous bytes per round, and a wave runs
-// CJX_TOK_ROUNDS rounds, so a wave covers 2 KiB per round.
+// Tokenizer tile geometry: a tile is one workgroup of CJX_TOK_WAVES wave64s;
+// each lane owns 32 contiguous bytes per round, so a wave covers 2 KiB per
+// round and runs CJX_TOK_ROUNDS rounds.  8 x 8 (64 KiB tiles) measured
+// fastest on a 2 GB document: fewer tiles shorten the look-back chain.
 #ifndef CJX_TOK_ROUNDS
-#define CJX_TOK_ROUNDS 2
+#define CJX_TOK_ROUNDS 8
 #endif
 #ifndef CJX_TOK_WAVES
-#define CJX_TOK_WAVES 4
+#define CJX_TOK_WAVES 8
 #endif
 constexpr int kTokThreads = 64 * CJX_TOK_WAVES;
 constexpr int kWaves = kTokThreads / 64;
@@ -26,6 +27,14 @@ constexpr int kTileBytes = kWaveBytes * kWaves;
 // Phase-3 geometry.
 constexpr int kP3Threads = 256;
 constexpr int kPairChunk = 4096;  // structural indices per pair-writer chunk (LDS staging)
+
+// Bracket matching of well-nested units (cjx_match.hip): brackets are cut
+// into chunks of kMatchChunk; a chunk keeps one slot per depth key for its
+// unmatched openers and closers.
+constexpr int kMatchChunk = 4096;
+constexpr int kMatchThreads = 256;
+constexpr int kMatchKeys = 256;
+constexpr int kMatchMaxDepth = 256;  // deeper units take the sort path
 
 // One parse unit inside a launch: a standard document, a JSON Lines chunk or
 // one document of a batch.  Escape/string state, depth and counts restart at
@@ -49,7 +58,21 @@ struct UnitOut {
     uint32_t utf8_err;        // phase-1 flag
     uint32_t struct_err;      // phase-3 flag
     uint32_t pair_base;       // added to pair values (lastStructuralIndex), 0 if local
-    uint32_t pad;
+    uint32_t chunk0;          // first matching chunk of the unit
+    uint32_t fast;            // well nested: paired by cjx_match.hip, not by the sort path
+    uint32_t pad[3];
+};
+
+// Depth walk of a run of brackets: net change, and the minimum and maximum
+// of the running depth, the start (0) included.
+struct DepthAgg {
+    int32_t d, mn, mx;
+};
+
+// Per matching chunk: its depth walk and the depth where it starts.
+struct ChunkStat {
+    int32_t d, mn, mx;
+    int32_t d0;
 };
 
 // Per-launch counters, cleared by k_reset before every parse.
@@ -57,6 +80,8 @@ struct LaunchState {
     uint32_t tile_counter;  // dynamic tile ticket
     uint32_t overflow;      // an output exceeded its capacity
     uint32_t n_total, m_total;
+    uint32_t n_chunks;  // matching chunks of all units
+    uint32_t pad[3];
 };
 
 // Look-back status encoding (two 64-bit words per tile, see cjx_tokenize.hip).
@@ -80,12 +105,21 @@ struct LaunchBufs {
     uint32_t *spos;
     int32_t *block_delta;
     uint32_t *hist;
+    ChunkStat *ck;    // per matching chunk
+    int32_t *gmin;    // per chunk: lowest depth reached (d0 + mn)
+    int32_t *gmin1;   // per 64 chunks (complete groups inside a unit)
+    int32_t *gmin2;   // per 4096 chunks
+    uint32_t *lows;   // per chunk x key: first closer reaching that depth (bracket index)
+    uint32_t *highs;  // per chunk x key: last opener left open
 };
 
 hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uint32_t n_units);
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                           uint32_t ntiles, uint32_t n_units, bool jsonl, uint32_t *out_struct);
-hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units);
+                           uint32_t ntiles, uint32_t n_units, bool jsonl, uint32_t *out_struct,
+                           int32_t *out_pair);
+hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool concat);
+hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, uint32_t chunk_bound,
+                        int32_t *pair_row);
 hipError_t launch_assemble(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
                            const uint32_t *d_struct, const int32_t *d_pair, int32_t *dst,
                            uint32_t N);

@@ -26,30 +26,51 @@ __global__ void k_reset(unsigned long long *tile_st, uint32_t ntiles, LaunchStat
     if (i0 == 0) *ls = LaunchState{};
 }
 
-// Per-unit pair bases for concatenated results: unit u's pair values are
-// offset by the structural count of all earlier units (the JSON Lines
-// lastStructuralIndex carry, parse_json_lines.cu:1222).
-__global__ __launch_bounds__(256) void k_unit_bases(UnitOut *__restrict__ uo, uint32_t n_units) {
-    __shared__ uint32_t wsum[4];
-    __shared__ uint32_t carry;
-    if (threadIdx.x == 0) carry = 0;
+// Per-unit bases, one block: pair_base offsets unit u's pair values by the
+// structural count of all earlier units when results are concatenated (the
+// JSON Lines lastStructuralIndex carry, parse_json_lines.cu:1222), and chunk0
+// numbers the units' matching chunks consecutively.
+__global__ __launch_bounds__(256) void k_unit_bases(UnitOut *__restrict__ uo, uint32_t n_units,
+                                                    uint32_t concat, LaunchState *__restrict__ ls) {
+    __shared__ uint32_t wsum[2][4];
+    __shared__ uint32_t carry[2];
+    if (threadIdx.x < 2) carry[threadIdx.x] = 0;
     __syncthreads();
+    const uint32_t w = threadIdx.x >> 6;
     for (uint32_t base = 0; base < n_units; base += 256) {
         const uint32_t u = base + threadIdx.x;
-        const uint32_t n = u < n_units ? uo[u].n_end - uo[u].n_base : 0u;
-        const uint32_t incl = wave_incl_scan(n);
-        if (lane_id() == 63) wsum[threadIdx.x >> 6] = incl;
-        __syncthreads();
-        uint32_t pre = carry, tot = 0;
-        for (uint32_t k = 0; k < 4; k++) {
-            if (k < (threadIdx.x >> 6)) pre += wsum[k];
-            tot += wsum[k];
+        uint32_t n = 0, c = 0;
+        if (u < n_units) {
+            n = uo[u].n_end - uo[u].n_base;
+            c = (uo[u].m_end - uo[u].m_base + kMatchChunk - 1) / kMatchChunk;
         }
-        if (u < n_units) uo[u].pair_base = pre + incl - n;
+        const uint32_t in = wave_incl_scan(n), ic = wave_incl_scan(c);
+        if (lane_id() == 63) {
+            wsum[0][w] = in;
+            wsum[1][w] = ic;
+        }
         __syncthreads();
-        if (threadIdx.x == 0) carry += tot;
+        uint32_t pn = carry[0], pc = carry[1], tn = 0, tc = 0;
+        for (uint32_t k = 0; k < 4; k++) {
+            if (k < w) {
+                pn += wsum[0][k];
+                pc += wsum[1][k];
+            }
+            tn += wsum[0][k];
+            tc += wsum[1][k];
+        }
+        if (u < n_units) {
+            uo[u].pair_base = concat ? pn + in - n : 0u;
+            uo[u].chunk0 = pc + ic - c;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            carry[0] += tn;
+            carry[1] += tc;
+        }
         __syncthreads();
     }
+    if (threadIdx.x == 0) ls->n_chunks = carry[1];
 }
 
 // Packs the per-unit output regions into the reference's host layout
@@ -129,7 +150,9 @@ __global__ __launch_bounds__(kP3Threads) void k_depth_totals(const uint8_t *__re
                                                              int32_t *__restrict__ block_delta,
                                                              const LaunchState *__restrict__ ls) {
     __shared__ int32_t red[kP3Threads / 64];
-    if (ls->overflow) return;  // capacity exceeded: ranges are not backed by buffers
+    // Capacity exceeded (ranges are not backed by buffers), or a well-nested
+    // unit that the matching kernels pair.
+    if (ls->overflow || uo[u].fast) return;
     uint32_t b0, b1;
     block_range(uo[u].m_base, uo[u].m_end, blockIdx.x, gridDim.x, b0, b1);
     int32_t s = 0;
@@ -146,7 +169,7 @@ __global__ __launch_bounds__(kP3Threads) void k_depth_keys(
     uint32_t kmask, const int32_t *__restrict__ block_delta, uint16_t *__restrict__ keys,
     uint32_t *__restrict__ hist, const LaunchState *__restrict__ ls) {
     __shared__ int32_t red[kP3Threads / 64];
-    if (ls->overflow) return;
+    if (ls->overflow || uo[u].fast) return;
     __shared__ int32_t wsum[kP3Threads / 64];
     __shared__ uint32_t h[256];
     const uint32_t tid = threadIdx.x, w = tid >> 6;
@@ -190,7 +213,7 @@ __global__ __launch_bounds__(kP3Threads) void k_hist(const uint16_t *__restrict_
                                                      uint32_t shift, uint32_t *__restrict__ hist,
                                                      const LaunchState *__restrict__ ls) {
     __shared__ uint32_t h[256];
-    if (ls->overflow) return;
+    if (ls->overflow || uo[u].fast) return;
     const uint32_t tid = threadIdx.x;
     uint32_t b0, b1;
     block_range(uo[u].m_base, uo[u].m_end, blockIdx.x, gridDim.x, b0, b1);
@@ -205,9 +228,10 @@ __global__ __launch_bounds__(kP3Threads) void k_hist(const uint16_t *__restrict_
 // Exclusive offsets per (block, digit): digit-major, block-minor, so that a
 // stable scatter reproduces thrust::stable_sort_by_key (parse_standard_json.cu:1546).
 __global__ __launch_bounds__(256) void k_scan_hist(uint32_t *__restrict__ hist, uint32_t G,
+                                                  const UnitOut *__restrict__ uo, uint32_t u,
                                                   const LaunchState *__restrict__ ls) {
     __shared__ uint32_t wsum[4];
-    if (ls->overflow) return;
+    if (ls->overflow || uo[u].fast) return;
     const uint32_t k = threadIdx.x, w = k >> 6;
     uint32_t total = 0;
     for (uint32_t b = 0; b < G; b++) total += hist[b * 256 + k];
@@ -233,7 +257,7 @@ __global__ __launch_bounds__(kP3Threads) void k_scatter(
     uint32_t *__restrict__ perm_out, uint32_t *__restrict__ spos,
     const LaunchState *__restrict__ ls) {
     __shared__ uint32_t run[256];
-    if (ls->overflow) return;
+    if (ls->overflow || uo[u].fast) return;
     __shared__ uint32_t wcnt[kP3Threads / 64][256];
     const uint32_t tid = threadIdx.x, w = tid >> 6;
     const uint64_t lt = lanemask_lt();
@@ -289,7 +313,7 @@ __global__ __launch_bounds__(kP3Threads) void k_pairs(
     UnitOut *__restrict__ uo, uint32_t u, int32_t *__restrict__ pair_row,
     const LaunchState *__restrict__ ls) {
     __shared__ int32_t row[kPairChunk];
-    if (ls->overflow) return;
+    if (ls->overflow || uo[u].fast) return;
     __shared__ uint32_t s_m0, s_m1;
     __shared__ int32_t red[kP3Threads / 64];
     const uint32_t tid = threadIdx.x;
@@ -364,8 +388,9 @@ hipError_t launch_assemble(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
     return hipGetLastError();
 }
 
-hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units) {
-    hipLaunchKernelGGL(k_unit_bases, dim3(1), dim3(256), 0, s, b.uout, n_units);
+hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool concat) {
+    hipLaunchKernelGGL(k_unit_bases, dim3(1), dim3(256), 0, s, b.uout, n_units, concat ? 1u : 0u,
+                       b.ls);
     return hipGetLastError();
 }
 
@@ -376,7 +401,7 @@ hipError_t launch_phase3(hipStream_t s, const LaunchBufs &b, uint32_t u, uint32_
                        b.block_delta, b.ls);
     hipLaunchKernelGGL(k_depth_keys, dim3(G), dim3(kP3Threads), 0, s, b.oc_char, b.uout, u, kmask,
                        b.block_delta, b.keys, b.hist, b.ls);
-    hipLaunchKernelGGL(k_scan_hist, dim3(1), dim3(256), 0, s, b.hist, G, b.ls);
+    hipLaunchKernelGGL(k_scan_hist, dim3(1), dim3(256), 0, s, b.hist, G, b.uout, u, b.ls);
     if (!depth_u16) {
         hipLaunchKernelGGL((k_scatter<true, true>), dim3(G), dim3(kP3Threads), 0, s, b.keys,
                            b.uout, u, 0u, b.hist, (const uint32_t *)nullptr, b.sorted, b.spos,
@@ -387,7 +412,7 @@ hipError_t launch_phase3(hipStream_t s, const LaunchBufs &b, uint32_t u, uint32_
                            b.ls);
         hipLaunchKernelGGL(k_hist, dim3(G), dim3(kP3Threads), 0, s, b.keys, b.uout, u, b.tmp, 8u,
                            b.hist, b.ls);
-        hipLaunchKernelGGL(k_scan_hist, dim3(1), dim3(256), 0, s, b.hist, G, b.ls);
+        hipLaunchKernelGGL(k_scan_hist, dim3(1), dim3(256), 0, s, b.hist, G, b.uout, u, b.ls);
         hipLaunchKernelGGL((k_scatter<false, true>), dim3(G), dim3(kP3Threads), 0, s, b.keys,
                            b.uout, u, 8u, b.hist, b.tmp, b.sorted, b.spos, b.ls);
     }

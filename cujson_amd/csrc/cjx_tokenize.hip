@@ -266,8 +266,8 @@ template <bool JSONL>
 __global__ __launch_bounds__(kTokThreads) void k_tokenize(
     const uint8_t *__restrict__ in, const UnitDesc *__restrict__ units, uint32_t n_units,
     uint32_t ntiles, unsigned long long *__restrict__ tile_st, LaunchState *__restrict__ ls,
-    UnitOut *__restrict__ uo, uint32_t *__restrict__ out_struct, uint8_t *__restrict__ oc_char,
-    uint32_t *__restrict__ oc_idx) {
+    UnitOut *__restrict__ uo, uint32_t *__restrict__ out_struct, int32_t *__restrict__ out_pair,
+    uint8_t *__restrict__ oc_char, uint32_t *__restrict__ oc_idx) {
     __shared__ uint32_t s_tile, s_unit;
     __shared__ uint32_t s_map[kWaves];
     __shared__ uint32_t s_cnt0[kWaves], s_cnt_all[kWaves];
@@ -468,12 +468,16 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
         uint32_t midx = mb + (excl >> 16);
         const uint32_t pos0 = (uint32_t)(U.pos_base + lane_off + 1);
         uint32_t m = sti;
+        // Structural positions, and -1 in the same pair-row cells (phase 3
+        // overwrites the openers' cells).
         if ((uint64_t)idx0 + (uint32_t)__popc(sti) <= cap_end) {
             uint32_t *p = out_struct + idx0;
+            int32_t *q = out_pair + idx0;
             while (m) {
                 const uint32_t j = (uint32_t)__builtin_ctz(m);
                 m &= m - 1;
                 *p++ = pos0 + j;
+                *q++ = -1;
             }
         } else if (m) {
             overflow = 1;
@@ -481,7 +485,10 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
             while (m) {
                 const uint32_t j = (uint32_t)__builtin_ctz(m);
                 m &= m - 1;
-                if (idx < cap_end) out_struct[idx] = pos0 + j;
+                if (idx < cap_end) {
+                    out_struct[idx] = pos0 + j;
+                    out_pair[idx] = -1;
+                }
                 idx++;
             }
         }
@@ -505,15 +512,16 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
 }
 
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                           uint32_t ntiles, uint32_t n_units, bool jsonl, uint32_t *out_struct) {
+                           uint32_t ntiles, uint32_t n_units, bool jsonl, uint32_t *out_struct,
+                           int32_t *out_pair) {
     if (jsonl)
         hipLaunchKernelGGL(k_tokenize<true>, dim3(ntiles), dim3(kTokThreads), 0, s, d_in, b.units,
-                           n_units, ntiles, b.tile_st, b.ls, b.uout, out_struct, b.oc_char,
-                           b.oc_idx);
+                           n_units, ntiles, b.tile_st, b.ls, b.uout, out_struct, out_pair,
+                           b.oc_char, b.oc_idx);
     else
         hipLaunchKernelGGL(k_tokenize<false>, dim3(ntiles), dim3(kTokThreads), 0, s, d_in, b.units,
-                           n_units, ntiles, b.tile_st, b.ls, b.uout, out_struct, b.oc_char,
-                           b.oc_idx);
+                           n_units, ntiles, b.tile_st, b.ls, b.uout, out_struct, out_pair,
+                           b.oc_char, b.oc_idx);
     return hipGetLastError();
 }
 

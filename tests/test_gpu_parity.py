@@ -72,9 +72,11 @@ def test_twitter_small_records_chunked(ctx, twitter_small, cc):
 
 
 def test_small_sizes_and_tile_edges(ctx):
-    base = bytes(gen.config1()[:70000])
-    for n in [1, 2, 3, 4, 5, 15, 16, 17, 31, 32, 33, 63, 64, 65, 1023, 1024, 1025, 4095, 4096,
-              4097, 16383, 16384, 16385, 32767, 32768, 32769, 65536 + 7]:
+    # lane (32 B), wave round (2 KiB), wave (16 KiB) and tile (64 KiB) edges
+    base = bytes(gen.config1()[:140000])
+    for n in [1, 2, 3, 4, 5, 15, 16, 17, 31, 32, 33, 63, 64, 65, 1023, 1024, 1025, 2047, 2048,
+              2049, 4095, 4096, 4097, 16383, 16384, 16385, 32767, 32768, 32769, 65535, 65536,
+              65537, 131072 + 7]:
         check_standard(ctx, base[:n], label=f"prefix n={n}")
 
 
@@ -83,7 +85,7 @@ def test_empty_and_null(ctx):
     assert g.error == cj.CJX_ERR_INPUT and g.structural is None
 
 
-@pytest.mark.parametrize("boundary", [16, 1024, 4096, 16384, 32768])
+@pytest.mark.parametrize("boundary", [16, 1024, 2048, 4096, 16384, 32768, 65536, 131072])
 def test_backslash_runs_across_boundaries(ctx, boundary):
     for k in range(1, 40):
         for tail in (b'"]', b'a"]', b'\\"]'):
@@ -94,13 +96,14 @@ def test_backslash_runs_across_boundaries(ctx, boundary):
 
 
 def test_all_backslash_tiles(ctx):
-    for k in (16384 - 1, 16384, 16384 + 1, 2 * 16384, 2 * 16384 + 1, 3 * 16384 + 5):
+    for k in (16384 - 1, 16384, 16384 + 1, 2 * 16384, 2 * 16384 + 1, 3 * 16384 + 5, 65535,
+              65536, 65537, 2 * 65536 + 1, 3 * 65536 + 5):
         for head in (b'["', b'[', b'"'):
             check_standard(ctx, head + b"\\" * k + b'"]",[1]]', label=f"allbs k={k} head={head}")
 
 
 def test_strings_across_tiles(ctx):
-    for L in (16383, 16384, 16385, 50000):
+    for L in (16383, 16384, 16385, 50000, 65535, 65536, 65537, 200000):
         doc = b'[{"a":"' + (b"{[:,]}" * (L // 6 + 1))[:L] + b'"},[1,2,{"b":[]}]]'
         check_standard(ctx, doc, label=f"long string L={L}")
 
@@ -137,7 +140,7 @@ def test_faults(ctx):
 def test_utf8_at_word_and_tile_edges(ctx):
     seqs = [b"\xc3\xa9", b"\xe2\x82\xac", b"\xf0\x9f\x98\x80", b"\xc3", b"\xe2\x82", b"\xf0\x9f",
             b"\xc0\xaf", b"\xed\xa0\x80"]
-    for edge in (4, 16, 1024, 4096, 16384):
+    for edge in (4, 16, 1024, 2048, 4096, 16384, 65536):
         for s in seqs:
             for shift in range(-4, 2):
                 pre = b'["' + b"a" * (edge - 2 + shift)
@@ -155,6 +158,51 @@ def test_jsonl_generated_records(ctx):
     for cc in (1, 3, 16):
         check_lines(ctx, data, oracle.chunks_by_count(data, cc), label=f"lines cc={cc}")
     check_lines(ctx, data, oracle.chunks_by_bytes(data, 1 << 16), label="lines 64KiB")
+
+
+def nested_doc(rng, n_brackets, max_depth, mismatches=0):
+    """Random well-nested document of about n_brackets brackets whose depth
+    stays within max_depth, with `mismatches` closers of the wrong type."""
+    out, stack = [b"["], [b"]"]
+    made = 1
+    while made < n_brackets or len(stack) > 0:
+        if made >= n_brackets:
+            out.append(stack.pop())
+            continue
+        up = len(stack) < max_depth and (not stack or rng.random() < 0.5)
+        if up:
+            kind = rng.random() < 0.5
+            out.append(b'{"k":' if kind else b"[1,")
+            stack.append(b"}" if kind else b"]")
+        else:
+            out.append(stack.pop() + (b"," if stack else b""))
+            if stack and out[-1].endswith(b","):
+                out.append(b"0")
+        made += 1
+    doc = bytearray(b"".join(out))
+    closers = [i for i, c in enumerate(doc) if c in (0x5D, 0x7D)]
+    for i in rng.sample(closers, mismatches):
+        doc[i] = 0x7D if doc[i] == 0x5D else 0x5D
+    return bytes(doc)
+
+
+@pytest.mark.parametrize("max_depth", [3, 64, 255, 256, 257])
+def test_nested_depth_limits(ctx, max_depth):
+    rng = random.Random(max_depth)
+    for n, bad in ((3000, 0), (200000, 0), (200000, 1), (50000, 3)):
+        doc = nested_doc(rng, n, max_depth, bad)
+        check_standard(ctx, doc, label=f"nested d<={max_depth} n={n} bad={bad}")
+        if max_depth >= 255:
+            check_standard(ctx, doc, oracle.FLAG_DEPTH_U16, label=f"u16 d<={max_depth} n={n}")
+
+
+def test_many_match_chunks(ctx):
+    """More than 4096 chunks of 4096 brackets under one top-level array."""
+    rng = random.Random(7)
+    parts = [b"[[[]]]", b'{"a":[{}]}', b"[{},[1]]", b'[[{"b":[[]]}]]']
+    body = b",".join(rng.choice(parts) for _ in range(3_000_000))
+    doc = b'{"top":[' + body + b"]}"
+    check_standard(ctx, doc, label="many chunks")
 
 
 def test_jsonl_bad_chunks(ctx, twitter_small):
@@ -190,7 +238,7 @@ def test_device_resident_entry(ctx):
     rc, N, M, over = ctx.wait()
     assert over and rc == cj.CJX_ERR_CAPACITY
     assert np.all(st2.download(np.uint8, 64 * 4, small * 4) == 7)
-    assert np.all(pr2.download(np.uint8, (small + 64) * 4) == 7)
+    assert np.all(pr2.download(np.uint8, 64 * 4, small * 4) == 7)
 
 
 def test_timings(ctx):
