@@ -85,7 +85,7 @@ struct cjx_context {
     DevBuf in, structural, pair;
     DevBuf units, uout, ls, tile_st, layout;
     DevBuf oc_char, oc_idx, keys, sorted, tmp, spos, block_delta, hist;
-    DevBuf ck, gmin, gmin1, gmin2, lows, highs;
+    DevBuf ck, gmin, gmin1, gmin2, gagg, gd0, lows, highs;
     PinnedBuf h_units, h_uout, h_ls;
     // Shape of the launch in flight (device form).
     uint32_t n_units = 0;
@@ -151,6 +151,7 @@ struct Shape {
     uint64_t rows;      // total rows of all output regions
     uint64_t max_rows;  // largest region
     uint64_t chunks;    // bound on the matching chunks of all units
+    uint64_t groups;    // bound on their 64-chunk groups (per unit, rounded up)
 };
 
 // Fills the row totals of a shape from the units' regions.
@@ -159,8 +160,9 @@ void shape_rows(Shape &sh, const std::vector<UnitDesc> &units) {
     for (const UnitDesc &u : units) {
         sh.rows += u.out_cap;
         sh.max_rows = std::max<uint64_t>(sh.max_rows, u.out_cap);
-        sh.chunks += (u.out_cap + kMatchChunk - 1) / kMatchChunk;
+        sh.chunks += (u.out_cap + kMatchChunk - 1) / kMatchChunk + 1;
     }
+    sh.groups = sh.chunks / 64 + units.size();
 }
 
 int ensure_work(cjx_context *ctx, const Shape &sh) {
@@ -172,8 +174,10 @@ int ensure_work(cjx_context *ctx, const Shape &sh) {
               ctx->sorted.ensure(sh.rows * 4 + 16) && ctx->spos.ensure(sh.rows * 4 + 16) &&
               ctx->block_delta.ensure(512 * 4) && ctx->hist.ensure(512 * 256 * 4) &&
               ctx->ck.ensure(sh.chunks * sizeof(ChunkStat) + 16) &&
-              ctx->gmin.ensure(sh.chunks * 4 + 16) && ctx->gmin1.ensure(sh.chunks / 64 * 4 + 16) &&
-              ctx->gmin2.ensure(sh.chunks / 4096 * 4 + 16) &&
+              ctx->gmin.ensure(sh.chunks * 4 + 16) && ctx->gmin1.ensure(sh.groups * 4 + 16) &&
+              ctx->gmin2.ensure(sh.groups * 4 + 16) &&
+              ctx->gagg.ensure(sh.groups * sizeof(DepthAgg) + 16) &&
+              ctx->gd0.ensure(sh.groups * 4 + 16) &&
               ctx->lows.ensure(sh.chunks * kMatchKeys * 4 + 16) &&
               ctx->highs.ensure(sh.chunks * kMatchKeys * 4 + 16) &&
               ctx->h_units.ensure(sh.n_units * sizeof(UnitDesc)) &&
@@ -200,6 +204,8 @@ LaunchBufs bufs_of(cjx_context *ctx) {
     b.gmin = ctx->gmin.as<int32_t>();
     b.gmin1 = ctx->gmin1.as<int32_t>();
     b.gmin2 = ctx->gmin2.as<int32_t>();
+    b.gagg = ctx->gagg.as<DepthAgg>();
+    b.gd0 = ctx->gd0.as<int32_t>();
     b.lows = ctx->lows.as<uint32_t>();
     b.highs = ctx->highs.as<uint32_t>();
     return b;
@@ -217,13 +223,14 @@ int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsign
     hipEvent_t *tev = ctx->tev[ctx->timing_count % cjx_context::kTimingRing];
     if (ctx->timing) (void)hipEventRecord(tev[0], s);
     if (launch_tokenize(s, b, d_in, sh.ntiles, sh.n_units, (flags & CJX_FLAG_JSONL) != 0,
-                        d_struct, d_pair) != hipSuccess)
+                        d_struct) != hipSuccess)
         return CJX_ERR_DEVICE;
     if (ctx->timing) (void)hipEventRecord(tev[1], s);
     // Concatenated results number pairs across units (JSON Lines).
     if (launch_unit_bases(s, b, sh.n_units, sh.n_units > 1) != hipSuccess) return CJX_ERR_DEVICE;
     // Well-nested units are matched directly; the rest take the sort path.
-    if (launch_match(s, b, sh.n_units, (uint32_t)sh.chunks, d_pair) != hipSuccess)
+    if (launch_match(s, b, sh.n_units, (uint32_t)sh.chunks, (uint32_t)sh.groups, d_pair) !=
+        hipSuccess)
         return CJX_ERR_DEVICE;
     const uint32_t G = phase3_blocks(sh.max_rows);
     const uint32_t PG = pair_blocks(sh.max_rows);
@@ -280,7 +287,7 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
                            hipMemcpyHostToDevice, s) != hipSuccess)
             return CJX_ERR_DEVICE;
 
-    Shape sh{ntiles, (uint32_t)units.size(), 0, 0, 0};
+    Shape sh{ntiles, (uint32_t)units.size(), 0, 0, 0, 0};
     for (int attempt = 0; attempt < 2; attempt++) {
         set_regions(units, attempt == 1);
         shape_rows(sh, units);
@@ -410,7 +417,7 @@ void cjx_context_destroy(cjx_context *ctx) {
                       &ctx->oc_char, &ctx->oc_idx,     &ctx->keys,   &ctx->sorted,
                       &ctx->tmp,     &ctx->spos,       &ctx->block_delta, &ctx->hist,
                       &ctx->ck,      &ctx->gmin,       &ctx->gmin1,  &ctx->gmin2,
-                      &ctx->lows,    &ctx->highs};
+                      &ctx->gagg,    &ctx->gd0,        &ctx->lows,   &ctx->highs};
     for (DevBuf *b : bufs) b->release();
     ctx->h_units.release();
     ctx->h_uout.release();
@@ -599,7 +606,7 @@ int cjx_parse_device(cjx_context *ctx, const uint8_t *d_input, size_t n, unsigne
     const uint32_t ntiles = plan_units({n}, {0}, units, dev_bytes);
     units[0].out_off = 0;
     units[0].out_cap = (uint32_t)std::min<uint64_t>(cap, n);
-    Shape sh{ntiles, 1, 0, 0, 0};
+    Shape sh{ntiles, 1, 0, 0, 0, 0};
     shape_rows(sh, units);
     if ((rc = ensure_work(ctx, sh))) return rc;
     std::memcpy(ctx->h_units.p, units.data(), sizeof(UnitDesc));

@@ -23,6 +23,7 @@ constexpr int kTokRounds = CJX_TOK_ROUNDS;
 constexpr int kRoundBytes = 64 * kLaneBytes;              // 2 KiB per wave round
 constexpr int kWaveBytes = kRoundBytes * kTokRounds;
 constexpr int kTileBytes = kWaveBytes * kWaves;
+constexpr int kStageRows = 1024;  // LDS rows per wave for coalesced structural stores
 
 // Phase-3 geometry.
 constexpr int kP3Threads = 256;
@@ -58,9 +59,11 @@ struct UnitOut {
     uint32_t utf8_err;        // phase-1 flag
     uint32_t struct_err;      // phase-3 flag
     uint32_t pair_base;       // added to pair values (lastStructuralIndex), 0 if local
-    uint32_t chunk0;          // first matching chunk of the unit
     uint32_t fast;            // well nested: paired by cjx_match.hip, not by the sort path
-    uint32_t pad[3];
+    uint32_t chunk0;          // first matching chunk of the unit
+    uint32_t nchunks;         // matching chunks (at least one when the unit has structurals)
+    uint32_t grp0;            // first 64-chunk group (unit-local groups)
+    uint32_t sgrp0;           // first 4096-chunk group
 };
 
 // Depth walk of a run of brackets: net change, and the minimum and maximum
@@ -107,19 +110,20 @@ struct LaunchBufs {
     uint32_t *hist;
     ChunkStat *ck;    // per matching chunk
     int32_t *gmin;    // per chunk: lowest depth reached (d0 + mn)
-    int32_t *gmin1;   // per 64 chunks (complete groups inside a unit)
-    int32_t *gmin2;   // per 4096 chunks
+    int32_t *gmin1;   // per 64 chunks of a unit
+    int32_t *gmin2;   // per 4096 chunks of a unit
+    DepthAgg *gagg;   // per 64 chunks: their depth walk
+    int32_t *gd0;     // per 64 chunks: start depth
     uint32_t *lows;   // per chunk x key: first closer reaching that depth (bracket index)
     uint32_t *highs;  // per chunk x key: last opener left open
 };
 
 hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uint32_t n_units);
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                           uint32_t ntiles, uint32_t n_units, bool jsonl, uint32_t *out_struct,
-                           int32_t *out_pair);
+                           uint32_t ntiles, uint32_t n_units, bool jsonl, uint32_t *out_struct);
 hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool concat);
 hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, uint32_t chunk_bound,
-                        int32_t *pair_row);
+                        uint32_t group_bound, int32_t *pair_row);
 hipError_t launch_assemble(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
                            const uint32_t *d_struct, const int32_t *d_pair, int32_t *dst,
                            uint32_t N);

@@ -28,46 +28,54 @@ __global__ void k_reset(unsigned long long *tile_st, uint32_t ntiles, LaunchStat
 
 // Per-unit bases, one block: pair_base offsets unit u's pair values by the
 // structural count of all earlier units when results are concatenated (the
-// JSON Lines lastStructuralIndex carry, parse_json_lines.cu:1222), and chunk0
-// numbers the units' matching chunks consecutively.
+// JSON Lines lastStructuralIndex carry, parse_json_lines.cu:1222); chunk0,
+// grp0 and sgrp0 number the units' matching chunks and their 64- and
+// 4096-chunk groups consecutively.  A unit with structurals has at least one
+// chunk, so that its pair row is written even without brackets.
 __global__ __launch_bounds__(256) void k_unit_bases(UnitOut *__restrict__ uo, uint32_t n_units,
                                                     uint32_t concat, LaunchState *__restrict__ ls) {
-    __shared__ uint32_t wsum[2][4];
-    __shared__ uint32_t carry[2];
-    if (threadIdx.x < 2) carry[threadIdx.x] = 0;
+    __shared__ uint32_t wsum[4][4];
+    __shared__ uint32_t carry[4];
+    if (threadIdx.x < 4) carry[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t w = threadIdx.x >> 6;
     for (uint32_t base = 0; base < n_units; base += 256) {
         const uint32_t u = base + threadIdx.x;
-        uint32_t n = 0, c = 0;
+        uint32_t v[4] = {0, 0, 0, 0};  // structurals, chunks, groups, super groups
         if (u < n_units) {
-            n = uo[u].n_end - uo[u].n_base;
-            c = (uo[u].m_end - uo[u].m_base + kMatchChunk - 1) / kMatchChunk;
+            v[0] = uo[u].n_end - uo[u].n_base;
+            const uint32_t m = uo[u].m_end - uo[u].m_base;
+            v[1] = m ? (m + kMatchChunk - 1) / kMatchChunk : (v[0] ? 1u : 0u);
+            v[2] = (v[1] + 63) / 64;
+            v[3] = (v[1] + 4095) / 4096;
         }
-        const uint32_t in = wave_incl_scan(n), ic = wave_incl_scan(c);
-        if (lane_id() == 63) {
-            wsum[0][w] = in;
-            wsum[1][w] = ic;
+        uint32_t in[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            in[k] = wave_incl_scan(v[k]);
+            if (lane_id() == 63) wsum[k][w] = in[k];
         }
         __syncthreads();
-        uint32_t pn = carry[0], pc = carry[1], tn = 0, tc = 0;
-        for (uint32_t k = 0; k < 4; k++) {
-            if (k < w) {
-                pn += wsum[0][k];
-                pc += wsum[1][k];
+        uint32_t pre[4], tot[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            pre[k] = carry[k];
+            tot[k] = 0;
+            for (uint32_t j = 0; j < 4; j++) {
+                if (j < w) pre[k] += wsum[k][j];
+                tot[k] += wsum[k][j];
             }
-            tn += wsum[0][k];
-            tc += wsum[1][k];
+            pre[k] += in[k] - v[k];
         }
         if (u < n_units) {
-            uo[u].pair_base = concat ? pn + in - n : 0u;
-            uo[u].chunk0 = pc + ic - c;
+            uo[u].pair_base = concat ? pre[0] : 0u;
+            uo[u].chunk0 = pre[1];
+            uo[u].nchunks = v[1];
+            uo[u].grp0 = pre[2];
+            uo[u].sgrp0 = pre[3];
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            carry[0] += tn;
-            carry[1] += tc;
-        }
+        if (threadIdx.x < 4) carry[threadIdx.x] += tot[threadIdx.x];
         __syncthreads();
     }
     if (threadIdx.x == 0) ls->n_chunks = carry[1];

@@ -16,15 +16,17 @@
 // unbalanced) goes through the sort path of cjx_kernels.hip, which follows the
 // reference step by step.
 //
-// Brackets are cut into chunks of kMatchChunk.  A chunk matches its own
-// brackets with a local stable counting sort in LDS; an opener left open at
-// the end of chunk b with key d is closed by the first later chunk whose
-// depth walk reaches d, at that chunk's first closer of key d.  Both ends are
-// recorded per (chunk, key), so the cross-chunk step is a search over the
-// chunks' minimum depths (with 64- and 4096-chunk group minima).
+// The same two-level rule runs at two scales.  Brackets are cut into chunks
+// of kMatchChunk, and a chunk into 16-bracket lanes.  A lane matches its own
+// brackets with a stack and is left with some closers that reach new lows
+// and some openers left open, at most one per depth.  An opener left open at
+// depth d is closed by the first later lane (or chunk) whose walk reaches d,
+// at that lane's (chunk's) first closer to reach d.  The search runs over the
+// lanes' (chunks') minimum depths, skipping whole groups by their minima.
 //
-// The pair row cells of all structurals are filled with -1 by the tokenizer;
-// these kernels write only the openers' cells.
+// Each chunk writes the pair-row cells of its rows (from its first bracket's
+// row to the next chunk's) densely, -1 where no pair starts, through LDS
+// tiles; openers closed in a later chunk are patched afterwards.
 #include "cjx_device.h"
 
 namespace cjx {
@@ -44,7 +46,8 @@ __device__ __forceinline__ int32_t delta_of(uint8_t c) {
     return (c & 2u) ? 1 : -1;
 }
 
-// Last unit whose chunk0 <= c.
+// Last unit whose chunk0 <= c.  A unit without chunks has the same chunk0 as
+// the next unit, so the last unit with chunk0 <= c is the one that owns c.
 __device__ __forceinline__ uint32_t unit_of_chunk(const UnitOut *__restrict__ uo,
                                                   uint32_t n_units, uint32_t c) {
     uint32_t lo = 0, hi = n_units - 1;
@@ -53,9 +56,15 @@ __device__ __forceinline__ uint32_t unit_of_chunk(const UnitOut *__restrict__ uo
         if (uo[mid].chunk0 <= c) lo = mid;
         else hi = mid - 1;
     }
-    // A unit without brackets has no chunks and the same chunk0 as the next
-    // unit, so the last unit with chunk0 <= c is the one that owns chunk c.
     return lo;
+}
+
+__device__ __forceinline__ DepthAgg shfl_agg_up(const DepthAgg &a, int off) {
+    DepthAgg o;
+    o.d = __shfl_up(a.d, off, 64);
+    o.mn = __shfl_up(a.mn, off, 64);
+    o.mx = __shfl_up(a.mx, off, 64);
+    return o;
 }
 
 // Ordered reduction of a DepthAgg over a wave (lane order = walk order).
@@ -94,253 +103,393 @@ __global__ __launch_bounds__(kMatchThreads) void k_chunk_stats(const uint8_t *__
                                                                uint32_t n_units,
                                                                ChunkStat *__restrict__ ck,
                                                                const LaunchState *__restrict__ ls) {
-    __shared__ DepthAgg part[kMatchThreads / 64];
-    if (ls->overflow) return;
-    const uint32_t c = blockIdx.x;
-    if (c >= ls->n_chunks) return;
-    const uint32_t u = unit_of_chunk(uo, n_units, c);
-    const uint32_t first = uo[u].m_base + (c - uo[u].chunk0) * kMatchChunk;
-    const uint32_t cnt = min((uint32_t)kMatchChunk, uo[u].m_end - first);
     constexpr int kPer = kMatchChunk / kMatchThreads;
-    DepthAgg a{0, 0, 0};
-    const uint32_t e0 = threadIdx.x * kPer;
+    __shared__ DepthAgg part[kMatchThreads / 64];
+    __shared__ __align__(16) uint8_t s_ch[kMatchChunk];
+    if (ls->overflow) return;
+    const uint32_t n_chunks = ls->n_chunks;
+    for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        const uint32_t u = unit_of_chunk(uo, n_units, c);
+        const uint32_t first = uo[u].m_base + (c - uo[u].chunk0) * kMatchChunk;
+        const uint32_t cnt =
+            uo[u].m_end > first ? min((uint32_t)kMatchChunk, uo[u].m_end - first) : 0u;
+        uint8_t vc[kPer];
 #pragma unroll
-    for (int k = 0; k < kPer; k++) {
-        if (e0 + k < cnt) {
-            a.d += delta_of(oc_char[first + e0 + k]);
-            a.mn = min(a.mn, a.d);
-            a.mx = max(a.mx, a.d);
+        for (int k = 0; k < kPer; k++) {
+            const uint32_t e = threadIdx.x + k * kMatchThreads;
+            vc[k] = e < cnt ? oc_char[first + e] : 0;
         }
-    }
-    a = wave_reduce_agg(a);
-    if (lane_id() == 0) part[threadIdx.x >> 6] = a;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        DepthAgg t = part[0];
-        for (int k = 1; k < kMatchThreads / 64; k++) t = agg_compose(t, part[k]);
-        ck[c] = ChunkStat{t.d, t.mn, t.mx, 0};
+#pragma unroll
+        for (int k = 0; k < kPer; k++) s_ch[threadIdx.x + k * kMatchThreads] = vc[k];
+        __syncthreads();
+        DepthAgg a{0, 0, 0};
+        const uint32_t e0 = threadIdx.x * kPer;
+        const uint4 cw = *reinterpret_cast<const uint4 *>(&s_ch[e0]);
+        const uint32_t cv[4] = {cw.x, cw.y, cw.z, cw.w};
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            if (e0 + k < cnt) {
+                a.d += delta_of((uint8_t)(cv[k >> 2] >> (8 * (k & 3))));
+                a.mn = min(a.mn, a.d);
+                a.mx = max(a.mx, a.d);
+            }
+        }
+        a = wave_reduce_agg(a);
+        if (lane_id() == 0) part[threadIdx.x >> 6] = a;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            DepthAgg t = part[0];
+            for (int k = 1; k < kMatchThreads / 64; k++) t = agg_compose(t, part[k]);
+            ck[c] = ChunkStat{t.d, t.mn, t.mx, 0};
+        }
+        __syncthreads();
     }
 }
 
-// K2, one block per unit: chunk start depths, chunk minima and their group
-// minima, and whether the unit is well nested.
-__global__ __launch_bounds__(256) void k_chunk_scan(UnitOut *__restrict__ uo,
-                                                    ChunkStat *__restrict__ ck,
-                                                    int32_t *__restrict__ gmin,
-                                                    int32_t *__restrict__ gmin1,
-                                                    int32_t *__restrict__ gmin2,
-                                                    const LaunchState *__restrict__ ls) {
-    __shared__ DepthAgg part[4];
+// K2, one block per unit, in three wave-parallel passes over the unit's
+// 64-chunk groups: (a) each group's depth walk, (b) an ordered scan of the
+// groups giving their start depths and minima, the 4096-chunk minima and
+// whether the unit is well nested, (c) each chunk's start depth and minimum.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_chunk_scan(
+    UnitOut *__restrict__ uo, ChunkStat *__restrict__ ck, int32_t *__restrict__ gmin,
+    int32_t *__restrict__ gmin1, int32_t *__restrict__ gmin2, DepthAgg *__restrict__ gagg,
+    int32_t *__restrict__ gd0, const LaunchState *__restrict__ ls) {
+    constexpr int kW = kScanThreads / 64;
+    __shared__ DepthAgg part[kW];
     if (ls->overflow) return;
-    const uint32_t u = blockIdx.x, tid = threadIdx.x;
-    const uint32_t c0 = uo[u].chunk0;
-    const uint32_t nch = (uo[u].m_end - uo[u].m_base + kMatchChunk - 1) / kMatchChunk;
-    const uint32_t per = (nch + 255) / 256;
-    const uint32_t a0 = c0 + min(nch, tid * per), a1 = c0 + min(nch, (tid + 1) * per);
-    DepthAgg a{0, 0, 0};
-    for (uint32_t c = a0; c < a1; c++) {
-        const ChunkStat s = ck[c];
-        a = agg_compose(a, DepthAgg{s.d, s.mn, s.mx});
+    const uint32_t u = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = lane_id();
+    const uint32_t c0 = uo[u].chunk0, nch = uo[u].nchunks;
+    const uint32_t g0 = uo[u].grp0, h0 = uo[u].sgrp0;
+    const uint32_t G = (nch + 63) / 64, H = (nch + 4095) / 4096;
+    if (nch == 0) {
+        if (tid == 0) uo[u].fast = 1;
+        return;
     }
-    DepthAgg incl = wave_scan_agg(a);
-    if (lane_id() == 63) part[tid >> 6] = incl;
+    // (a) group walks, two groups per wave step.
+    for (uint32_t j = w; j < G; j += 2 * kW) {
+        const uint32_t j2 = j + kW;
+        const uint32_t x1 = 64 * j + lane, x2 = 64 * j2 + lane;
+        ChunkStat s1{0, 0, 0, 0}, s2{0, 0, 0, 0};
+        if (x1 < nch) s1 = ck[c0 + x1];
+        if (j2 < G && x2 < nch) s2 = ck[c0 + x2];
+        const DepthAgg r1 = wave_reduce_agg(DepthAgg{s1.d, s1.mn, s1.mx});
+        const DepthAgg r2 = wave_reduce_agg(DepthAgg{s2.d, s2.mn, s2.mx});
+        if (lane == 0) {
+            gagg[g0 + j] = r1;
+            if (j2 < G) gagg[g0 + j2] = r2;
+        }
+    }
+    __syncthreads();
+    // (b) ordered scan over the groups, up to 8 per thread.
+    const uint32_t per = (G + kScanThreads - 1) / kScanThreads;
+    const uint32_t a0 = min(G, tid * per), a1 = min(G, (tid + 1) * per);
+    DepthAgg a{0, 0, 0};
+    for (uint32_t jb = a0; jb < a1; jb += 8) {
+        DepthAgg v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = gagg[g0 + (jb + k < a1 ? jb + k : a0)];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (jb + k < a1) a = agg_compose(a, v[k]);
+    }
+    const DepthAgg incl = wave_scan_agg(a);
+    if (lane == 63) part[w] = incl;
     __syncthreads();
     DepthAgg pre{0, 0, 0};
-    for (uint32_t k = 0; k < (tid >> 6); k++) pre = agg_compose(pre, part[k]);
-    DepthAgg excl;
-    {
-        DepthAgg o;
-        o.d = __shfl_up(incl.d, 1, 64);
-        o.mn = __shfl_up(incl.mn, 1, 64);
-        o.mx = __shfl_up(incl.mx, 1, 64);
-        excl = lane_id() ? agg_compose(pre, o) : pre;
-    }
+    for (uint32_t k = 0; k < w; k++) pre = agg_compose(pre, part[k]);
+    const DepthAgg up = shfl_agg_up(incl, 1);
+    const DepthAgg excl = lane ? agg_compose(pre, up) : pre;
     int32_t depth = excl.d;
-    for (uint32_t c = a0; c < a1; c++) {
-        const ChunkStat s = ck[c];
-        ck[c].d0 = depth;
-        gmin[c] = depth + s.mn;
-        depth += s.d;
+    for (uint32_t jb = a0; jb < a1; jb += 8) {
+        DepthAgg v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = gagg[g0 + (jb + k < a1 ? jb + k : a0)];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (jb + k < a1) {
+                gd0[g0 + jb + k] = depth;
+                gmin1[g0 + jb + k] = depth + v[k].mn;
+                depth += v[k].d;
+            }
+        }
     }
-    if (tid == 255) {
+    if (tid == kScanThreads - 1) {
         const DepthAgg t = agg_compose(pre, incl);
         uo[u].fast = (t.d == 0 && t.mn >= 0 && t.mx <= kMatchMaxDepth) ? 1u : 0u;
     }
     __syncthreads();
-    // Group minima for groups that lie inside the unit.
-    const uint32_t g1a = (c0 + 63) / 64, g1b = (c0 + nch) / 64;
-    for (uint32_t g = g1a + tid; g < g1b; g += 256) {
-        int32_t m = gmin[64 * g];
-        for (uint32_t k = 1; k < 64; k++) m = min(m, gmin[64 * g + k]);
-        gmin1[g] = m;
+    // 4096-chunk minima.
+    for (uint32_t h = w; h < H; h += kW) {
+        int32_t m = (64 * h + lane < G) ? gmin1[g0 + 64 * h + lane] : INT32_MAX;
+        for (int off = 32; off >= 1; off >>= 1) m = min(m, __shfl_xor(m, off, 64));
+        if (lane == 0) gmin2[h0 + h] = m;
     }
-    __syncthreads();
-    const uint32_t g2a = (c0 + 4095) / 4096, g2b = (c0 + nch) / 4096;
-    for (uint32_t g = g2a + tid; g < g2b; g += 256) {
-        int32_t m = gmin1[64 * g];
-        for (uint32_t k = 1; k < 64; k++) m = min(m, gmin1[64 * g + k]);
-        gmin2[g] = m;
+    // (c) chunk start depths and minima, two groups per wave step.
+    for (uint32_t j = w; j < G; j += 2 * kW) {
+        const uint32_t j2 = j + kW;
+        const uint32_t x1 = 64 * j + lane, x2 = 64 * j2 + lane;
+        ChunkStat s1{0, 0, 0, 0}, s2{0, 0, 0, 0};
+        int32_t b1 = 0, b2 = 0;
+        b1 = gd0[g0 + j];
+        if (x1 < nch) s1 = ck[c0 + x1];
+        if (j2 < G) {
+            b2 = gd0[g0 + j2];
+            if (x2 < nch) s2 = ck[c0 + x2];
+        }
+        const int32_t i1 = (int32_t)wave_incl_scan((uint32_t)s1.d);
+        const int32_t i2 = (int32_t)wave_incl_scan((uint32_t)s2.d);
+        if (x1 < nch) {
+            const int32_t d0 = b1 + i1 - s1.d;
+            ck[c0 + x1].d0 = d0;
+            gmin[c0 + x1] = d0 + s1.mn;
+        }
+        if (j2 < G && x2 < nch) {
+            const int32_t d0 = b2 + i2 - s2.d;
+            ck[c0 + x2].d0 = d0;
+            gmin[c0 + x2] = d0 + s2.mn;
+        }
     }
 }
 
-// K3: local matching of one chunk.  The chunk's brackets are stably
-// counting-sorted by key in LDS; an opener followed in its key class by
-// another bracket is matched to it, an opener last in its class is recorded
-// in highs, and a closer first in its class in lows.
+// K3: matching inside one chunk.  Thread t owns the 16 brackets
+// [16t, 16t + 16) of the chunk and matches them with a stack kept in LDS
+// (column t, so the threads' pushes and pops do not conflict).  What is left
+// is, per lane, closers that reach the lane's successive new lows (s_low) and
+// openers left open at the lane minimum, minimum + 1, ... (the stack).  Each
+// left-open opener is closed by the first later lane whose minimum reaches
+// its depth; one that no lane of the chunk closes goes to highs, and a low
+// closer that no earlier lane of the chunk opened goes to lows.  Partners are
+// kept in LDS (s_part) and the chunk's pair-row cells are then written as
+// dense 4096-row tiles.
+constexpr int kLaneBrackets = kMatchChunk / kMatchThreads;  // 16
+constexpr int kRowTile = 4096;
+constexpr uint16_t kNoPart = 0xFFFF;
 __global__ __launch_bounds__(kMatchThreads) void k_match_local(
     const uint8_t *__restrict__ oc_char, const uint32_t *__restrict__ oc_idx,
     UnitOut *__restrict__ uo, uint32_t n_units, const ChunkStat *__restrict__ ck,
     uint32_t *__restrict__ lows, uint32_t *__restrict__ highs, int32_t *__restrict__ pair_row,
     const LaunchState *__restrict__ ls) {
-    constexpr int kPer = kMatchChunk / kMatchThreads;
     constexpr int kW = kMatchThreads / 64;
-    __shared__ uint8_t s_ch[kMatchChunk];
-    __shared__ uint8_t s_key[kMatchChunk];
-    __shared__ uint16_t s_sidx[kMatchChunk];  // sorted position -> element
-    __shared__ uint16_t s_spos[kMatchChunk];  // element -> sorted position
+    constexpr int kGroups = kMatchThreads / 16;
+    __shared__ __align__(16) uint8_t s_ch[kMatchChunk];
     __shared__ uint32_t s_idx[kMatchChunk];
-    __shared__ uint32_t s_start[kMatchKeys], s_count[kMatchKeys], s_run[kMatchKeys];
-    __shared__ uint32_t s_wcnt[kW][kMatchKeys];
-    __shared__ int32_t s_wsum[kW];
-    __shared__ int32_t s_red[kW];
+    __shared__ uint16_t s_part[kMatchChunk];
+    // Stack columns and low closers during matching; the row tile afterwards.
+    __shared__ union {
+        struct {
+            uint16_t stk[kLaneBrackets][kMatchThreads];
+            uint16_t low[kLaneBrackets][kMatchThreads];
+        } m;
+        int32_t row[kRowTile];
+    } s_u;
+    __shared__ int32_t s_lmin[kMatchThreads];  // lane minimum depth (start included)
+    __shared__ int32_t s_lst[kMatchThreads];   // lane start depth
+    __shared__ int32_t s_gmin[kGroups];        // minimum over 16 lanes
+    __shared__ int32_t s_wsum[kW], s_wmin[kW];
 
     if (ls->overflow) return;
-    const uint32_t c = blockIdx.x;
-    if (c >= ls->n_chunks) return;
-    const uint32_t u = unit_of_chunk(uo, n_units, c);
-    if (!uo[u].fast) return;
-    const uint32_t tid = threadIdx.x, w = tid >> 6;
-    const uint32_t first = uo[u].m_base + (c - uo[u].chunk0) * kMatchChunk;
-    const uint32_t cnt = min((uint32_t)kMatchChunk, uo[u].m_end - first);
-    const uint32_t nb = uo[u].n_base, pbase = uo[u].pair_base + 1u;
-    const int32_t d0 = ck[c].d0;
-
-    for (uint32_t e = tid; e < kMatchChunk; e += kMatchThreads) {
-        const bool v = e < cnt;
-        s_ch[e] = v ? oc_char[first + e] : 0;
-        s_idx[e] = v ? oc_idx[first + e] : 0;
-    }
-    s_count[tid] = 0;
-    __syncthreads();
-
-    // Depth keys: thread t walks elements [16t, 16t + 16).
-    const uint32_t e0 = tid * kPer;
-    int32_t mine = 0;
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = lane_id();
+    const uint32_t n_chunks = ls->n_chunks;
+    for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        const uint32_t u = unit_of_chunk(uo, n_units, c);
+        if (!uo[u].fast) continue;
+        const uint32_t xc = c - uo[u].chunk0;
+        const uint32_t first = uo[u].m_base + xc * kMatchChunk;
+        const uint32_t cnt =
+            uo[u].m_end > first ? min((uint32_t)kMatchChunk, uo[u].m_end - first) : 0u;
+        const uint32_t nb = uo[u].n_base, pbase = uo[u].pair_base + 1u;
+        const bool last_chunk = xc + 1 == uo[u].nchunks;
+        const uint32_t r1 = last_chunk ? uo[u].n_end : oc_idx[first + kMatchChunk];
+        const int32_t d0 = ck[c].d0;
+        // All loads in flight before the first LDS store.
+        uint8_t vc[kLaneBrackets];
+        uint32_t vi[kLaneBrackets];
 #pragma unroll
-    for (int k = 0; k < kPer; k++)
-        if (e0 + k < cnt) mine += delta_of(s_ch[e0 + k]);
-    const int32_t incl = (int32_t)wave_incl_scan((uint32_t)mine);
-    if (lane_id() == 63) s_wsum[w] = incl;
-    __syncthreads();
-    int32_t depth = d0 + incl - mine;
-    for (uint32_t k = 0; k < w; k++) depth += s_wsum[k];
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-        if (e0 + k < cnt) {
-            const int32_t dl = delta_of(s_ch[e0 + k]);
-            const uint32_t key = (uint32_t)(dl > 0 ? depth : depth - 1);  // in [0, 255]
-            depth += dl;
-            s_key[e0 + k] = (uint8_t)key;
-            atomicAdd(&s_count[key], 1u);
+        for (int k = 0; k < kLaneBrackets; k++) {
+            const uint32_t e = tid + k * kMatchThreads;
+            vc[k] = e < cnt ? oc_char[first + e] : 0;
+            vi[k] = e < cnt ? oc_idx[first + e] : 0;
         }
-    }
-    __syncthreads();
-    {
-        const uint32_t v = s_count[tid];
-        const uint32_t in = wave_incl_scan(v);
-        if (lane_id() == 63) s_wsum[w] = (int32_t)in;
-        __syncthreads();
-        uint32_t ex = in - v;
-        for (uint32_t k = 0; k < w; k++) ex += (uint32_t)s_wsum[k];
-        s_start[tid] = ex;
-        s_run[tid] = ex;
-    }
-    __syncthreads();
-
-    // Stable ranks: rounds of 256 consecutive elements; inside a round, ranks
-    // among equal keys come from a wave match (8 ballots) and per-wave counts.
-    const uint64_t lt = lanemask_lt();
-    for (uint32_t base = 0; base < cnt; base += kMatchThreads) {
-        const uint32_t e = base + tid;
-        const bool valid = e < cnt;
-        const uint32_t key = valid ? s_key[e] : 0u;
-        uint64_t peers = __ballot(valid);
+        __syncthreads();  // LDS of the previous chunk is no longer read
 #pragma unroll
-        for (int bit = 0; bit < 8; bit++) {
-            const uint32_t bv = (key >> bit) & 1u;
-            const uint64_t bm = __ballot(bv);
-            peers &= bv ? bm : ~bm;
-        }
-#pragma unroll
-        for (int k = 0; k < kW; k++) s_wcnt[k][tid] = 0;
-        __syncthreads();
-        const uint32_t rank = (uint32_t)__popcll(peers & lt);
-        if (valid && rank == 0) s_wcnt[w][key] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = s_run[key] + rank;
-            for (uint32_t k = 0; k < w; k++) pos += s_wcnt[k][key];
-            s_sidx[pos] = (uint16_t)e;
-            s_spos[e] = (uint16_t)pos;
+        for (int k = 0; k < kLaneBrackets; k++) {
+            const uint32_t e = tid + k * kMatchThreads;
+            s_ch[e] = vc[k];
+            s_idx[e] = vi[k];
+            s_part[e] = kNoPart;
         }
         __syncthreads();
-        uint32_t add = 0;
-#pragma unroll
-        for (int k = 0; k < kW; k++) add += s_wcnt[k][tid];
-        s_run[tid] += add;
-        __syncthreads();
-    }
+        const uint32_t r0 = xc == 0 ? nb : s_idx[0];
 
-    // Pairs, in element order so that the row writes stay close together.
-    int32_t err = 0;
-    for (uint32_t e = tid; e < cnt; e += kMatchThreads) {
-        const uint32_t key = s_key[e], p = s_spos[e];
-        const uint8_t ch = s_ch[e];
-        if (ch & 2u) {
-            if (p + 1 < s_start[key] + s_count[key]) {
-                const uint32_t e2 = s_sidx[p + 1];
-                if ((s_ch[e2] ^ ch) != 6) err = 1;
-                pair_row[s_idx[e]] = (int32_t)(s_idx[e2] - nb + pbase);
+        const uint32_t e0 = tid * kLaneBrackets;
+        const uint32_t ne = cnt > e0 ? min((uint32_t)kLaneBrackets, cnt - e0) : 0u;
+        const uint4 cw = *reinterpret_cast<const uint4 *>(&s_ch[e0]);
+        const uint32_t cv[4] = {cw.x, cw.y, cw.z, cw.w};
+        int32_t mine = 0;
+#pragma unroll
+        for (int k = 0; k < kLaneBrackets; k++)
+            if ((uint32_t)k < ne) mine += delta_of((uint8_t)(cv[k >> 2] >> (8 * (k & 3))));
+        // Lane start depth: exclusive scan of the lanes' net changes.
+        const int32_t incl = (int32_t)wave_incl_scan((uint32_t)mine);
+        if (lane == 63) s_wsum[w] = incl;
+        __syncthreads();
+        int32_t ds = d0 + incl - mine;
+        for (uint32_t k = 0; k < w; k++) ds += s_wsum[k];
+
+        // Stack walk of the lane.
+        uint32_t sp = 0, nlow = 0;
+        int32_t err = 0;
+#pragma unroll
+        for (int k = 0; k < kLaneBrackets; k++) {
+            if ((uint32_t)k >= ne) break;
+            const uint8_t ch = (uint8_t)(cv[k >> 2] >> (8 * (k & 3)));
+            const uint32_t e = e0 + k;
+            if (ch & 2u) {
+                s_u.m.stk[sp][tid] = (uint16_t)e;
+                sp++;
+            } else if (sp) {
+                sp--;
+                const uint32_t i = s_u.m.stk[sp][tid];
+                if ((s_ch[i] ^ ch) != 6) err = 1;
+                s_part[i] = (uint16_t)e;
             } else {
-                highs[(uint64_t)c * kMatchKeys + key] = first + e;
+                s_u.m.low[nlow][tid] = (uint16_t)e;
+                nlow++;
             }
-        } else if (p == s_start[key]) {
-            lows[(uint64_t)c * kMatchKeys + key] = first + e;
         }
-    }
-    for (int off = 32; off >= 1; off >>= 1) err |= __shfl_xor(err, off, 64);
-    if (lane_id() == 0) s_red[w] = err;
-    __syncthreads();
-    if (tid == 0) {
-        int32_t any = 0;
-        for (int k = 0; k < kW; k++) any |= s_red[k];
-        if (any) atomicOr(&uo[u].struct_err, 1u);
+        const int32_t dm = ds - (int32_t)nlow;
+        s_lmin[tid] = ne ? dm : INT32_MAX;
+        s_lst[tid] = ds;
+        // Lowest depth before this lane within the chunk (chunk start included).
+        int32_t pm = ne ? dm : INT32_MAX;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int32_t o = __shfl_up(pm, off, 64);
+            if (lane >= (uint32_t)off) pm = min(pm, o);
+        }
+        if (lane == 63) s_wmin[w] = pm;
+        {
+            const int32_t o = __shfl_up(pm, 1, 64);
+            pm = lane ? o : INT32_MAX;
+        }
+        __syncthreads();
+        pm = min(pm, d0);
+        for (uint32_t k = 0; k < w; k++) pm = min(pm, s_wmin[k]);
+        if (tid < kGroups) {
+            int32_t g = INT32_MAX;
+            for (int k = 0; k < 16; k++) g = min(g, s_lmin[16 * tid + k]);
+            s_gmin[tid] = g;
+        }
+        __syncthreads();
+
+        // Openers left open by the lane.
+        for (uint32_t j = 0; j < sp; j++) {
+            const int32_t d = dm + (int32_t)j;
+            const uint32_t i = s_u.m.stk[j][tid];
+            uint32_t L = tid + 1;
+            while (L < kMatchThreads) {
+                if ((L & 15u) == 0 && s_gmin[L >> 4] > d) {
+                    L += 16;
+                    continue;
+                }
+                if (s_lmin[L] <= d) break;
+                L++;
+            }
+            if (L < kMatchThreads) {
+                const int32_t k = s_lst[L] - 1 - d;
+                const uint32_t e2 = s_u.m.low[k & (kLaneBrackets - 1)][L];
+                if ((s_ch[i] ^ s_ch[e2]) != 6) err = 1;
+                s_part[i] = (uint16_t)e2;
+            } else if ((uint32_t)d < (uint32_t)kMatchKeys) {
+                highs[(uint64_t)c * kMatchKeys + d] = first + i;
+            }
+        }
+        // Low closers that no earlier lane of the chunk opened.
+        for (uint32_t k = 0; k < nlow; k++) {
+            const int32_t d = ds - 1 - (int32_t)k;
+            if (d < pm && (uint32_t)d < (uint32_t)kMatchKeys)
+                lows[(uint64_t)c * kMatchKeys + d] = first + s_u.m.low[k][tid];
+        }
+        if (err) atomicOr(&uo[u].struct_err, 1u);
+
+        // Dense pair-row tiles over the chunk's rows [r0, r1).  Each thread
+        // keeps the rows and values of its openers (elements tid + 256k) in
+        // registers and drops those that fall in the current tile.
+        __syncthreads();  // s_part complete
+        int32_t pv[kLaneBrackets];
+#pragma unroll
+        for (int k = 0; k < kLaneBrackets; k++) {
+            const uint32_t e = tid + k * kMatchThreads;
+            const uint32_t p = s_part[e];
+            pv[k] = p != kNoPart ? (int32_t)(s_idx[p] - nb + pbase) : -1;
+        }
+        for (uint32_t t0 = r0; t0 < r1; t0 += kRowTile) {
+            __syncthreads();  // matching state / previous tile no longer read
+            for (uint32_t i = tid; i < (uint32_t)kRowTile; i += kMatchThreads) s_u.row[i] = -1;
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kLaneBrackets; k++) {
+                const uint32_t off = vi[k] - t0;
+                if (pv[k] >= 0 && off < (uint32_t)kRowTile) s_u.row[off] = pv[k];
+            }
+            __syncthreads();
+            const uint32_t len = min((uint32_t)kRowTile, r1 - t0);
+            for (uint32_t i = tid; i < len; i += kMatchThreads) pair_row[t0 + i] = s_u.row[i];
+        }
     }
 }
 
-// First chunk in (b, last] whose walk reaches depth d or below.  Whole
-// 64- and 4096-chunk groups inside the range are skipped by their minima.
-__device__ __forceinline__ uint32_t next_reaching(const int32_t *__restrict__ gmin,
-                                                  const int32_t *__restrict__ gmin1,
-                                                  const int32_t *__restrict__ gmin2, uint32_t b,
-                                                  uint32_t last, int32_t d) {
-    uint32_t c = b + 1;
-    while (c <= last) {
-        if ((c & 4095u) == 0 && c + 4095 <= last && gmin2[c >> 12] > d) {
-            c += 4096;
+// First chunk (unit-local index) x >= x0 whose walk reaches depth d or below,
+// or nch.  The whole wave runs one search: each step tests 64 entries of one
+// level (chunks, 64-chunk groups, 4096-chunk groups) with a ballot.
+__device__ __forceinline__ uint32_t wave_next_reaching(const int32_t *__restrict__ gmin,
+                                                       const int32_t *__restrict__ gmin1,
+                                                       const int32_t *__restrict__ gmin2,
+                                                       uint32_t c0, uint32_t g0, uint32_t h0,
+                                                       uint32_t nch, uint32_t x, int32_t d) {
+    const uint32_t lane = lane_id();
+    const uint32_t G = (nch + 63) / 64, H = (nch + 4095) / 4096;
+    while (x < nch) {
+        if (x & 63u) {  // chunks up to the next group boundary
+            const uint32_t end = min(nch, (x | 63u) + 1);
+            const uint64_t m = __ballot(x + lane < end && gmin[c0 + x + lane] <= d);
+            if (m) return x + (uint32_t)__ffsll((long long)m) - 1;
+            x = end;
             continue;
         }
-        if ((c & 63u) == 0 && c + 63 <= last && gmin1[c >> 6] > d) {
-            c += 64;
-            continue;
+        uint32_t g = x >> 6;
+        if (g & 63u) {  // groups up to the next super-group boundary
+            const uint32_t gend = min(G, (g | 63u) + 1);
+            const uint64_t m = __ballot(g + lane < gend && gmin1[g0 + g + lane] <= d);
+            if (!m) {
+                x = gend * 64;
+                continue;
+            }
+            g += (uint32_t)__ffsll((long long)m) - 1;
+        } else {
+            const uint32_t h = g >> 6;
+            const uint32_t hend = min(H, h + 64);
+            const uint64_t mh = __ballot(h + lane < hend && gmin2[h0 + h + lane] <= d);
+            if (!mh) {
+                x = hend * 4096;
+                continue;
+            }
+            const uint32_t hh = h + (uint32_t)__ffsll((long long)mh) - 1;
+            const uint64_t mg =
+                __ballot(64 * hh + lane < G && gmin1[g0 + 64 * hh + lane] <= d);
+            g = 64 * hh + (uint32_t)__ffsll((long long)mg) - 1;
         }
-        if (gmin[c] <= d) return c;
-        c++;
+        const uint64_t mc = __ballot(64 * g + lane < nch && gmin[c0 + 64 * g + lane] <= d);
+        return 64 * g + (uint32_t)__ffsll((long long)mc) - 1;
     }
-    return last;  // not reached for a well-nested unit
+    return nch;
 }
 
-// K4: openers left open by their chunk, one wave per chunk.
+// K4: openers left open by their chunk, one wave per chunk and one lane per
+// depth.  A lane first looks at the next few chunks itself; openers closed
+// further away are then searched by the whole wave, one at a time.
+constexpr uint32_t kNearChunks = 8;
 __global__ __launch_bounds__(64) void k_match_cross(
     const uint8_t *__restrict__ oc_char, const uint32_t *__restrict__ oc_idx,
     UnitOut *__restrict__ uo, uint32_t n_units, const ChunkStat *__restrict__ ck,
@@ -349,45 +498,76 @@ __global__ __launch_bounds__(64) void k_match_cross(
     const uint32_t *__restrict__ highs, int32_t *__restrict__ pair_row,
     const LaunchState *__restrict__ ls) {
     if (ls->overflow) return;
-    const uint32_t c = blockIdx.x;
-    if (c >= ls->n_chunks) return;
-    const uint32_t u = unit_of_chunk(uo, n_units, c);
-    if (!uo[u].fast) return;
-    const uint32_t nch = (uo[u].m_end - uo[u].m_base + kMatchChunk - 1) / kMatchChunk;
-    const uint32_t last = uo[u].chunk0 + nch - 1;
-    const uint32_t nb = uo[u].n_base, pbase = uo[u].pair_base + 1u;
-    const uint32_t mb = uo[u].m_base, me = uo[u].m_end;
-    const ChunkStat s = ck[c];
-    const int32_t dend = s.d0 + s.d;
-    int32_t err = 0;
-    for (int32_t d = gmin[c] + (int32_t)threadIdx.x; d < dend; d += 64) {
-        const uint32_t c2 = next_reaching(gmin, gmin1, gmin2, c, last, d);
-        const uint32_t m = highs[(uint64_t)c * kMatchKeys + d];
-        const uint32_t m2 = lows[(uint64_t)c2 * kMatchKeys + d];
-        // Both slots are written for a well-nested unit; the range check keeps
-        // a stale slot from ever indexing outside the unit.
-        if (m < mb || m >= me || m2 < mb || m2 >= me) {
-            err = 1;
-            continue;
+    const uint32_t n_chunks = ls->n_chunks;
+    const uint32_t lane = lane_id();
+    for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        const uint32_t u = unit_of_chunk(uo, n_units, c);
+        if (!uo[u].fast) continue;
+        const uint32_t c0 = uo[u].chunk0, nch = uo[u].nchunks;
+        const uint32_t g0 = uo[u].grp0, h0 = uo[u].sgrp0;
+        const uint32_t xc = c - c0;
+        const uint32_t nb = uo[u].n_base, pbase = uo[u].pair_base + 1u;
+        const uint32_t mb = uo[u].m_base, me = uo[u].m_end;
+        const ChunkStat s = ck[c];
+        const int32_t dend = min(s.d0 + s.d, kMatchKeys);
+        int32_t err = 0;
+        for (int32_t dbase = max(gmin[c], 0); dbase < dend; dbase += 64) {
+            const int32_t d = dbase + (int32_t)lane;
+            const bool active = d < dend;
+            uint32_t found = nch;
+            if (active) {
+                for (uint32_t x = xc + 1; x < nch && x <= xc + kNearChunks; x++) {
+                    if (gmin[c0 + x] <= d) {
+                        found = x;
+                        break;
+                    }
+                }
+            }
+            uint64_t pend = __ballot(active && found == nch);
+            while (pend) {
+                const uint32_t k = (uint32_t)__ffsll((long long)pend) - 1;
+                pend &= pend - 1;
+                const int32_t dk = __shfl(d, (int)k, 64);
+                const uint32_t r = wave_next_reaching(gmin, gmin1, gmin2, c0, g0, h0, nch,
+                                                      xc + kNearChunks + 1, dk);
+                if (lane == k) found = r;
+            }
+            if (!active) continue;
+            if (found >= nch) {  // not reached for a well-nested unit
+                err = 1;
+                continue;
+            }
+            const uint32_t m = highs[(uint64_t)c * kMatchKeys + d];
+            const uint32_t m2 = lows[(uint64_t)(c0 + found) * kMatchKeys + d];
+            // Both slots are written for a well-nested unit; the range check
+            // keeps a stale slot from ever indexing outside the unit.
+            if (m < mb || m >= me || m2 < mb || m2 >= me) {
+                err = 1;
+                continue;
+            }
+            if ((oc_char[m] ^ oc_char[m2]) != 6) err = 1;
+            pair_row[oc_idx[m]] = (int32_t)(oc_idx[m2] - nb + pbase);
         }
-        if ((oc_char[m] ^ oc_char[m2]) != 6) err = 1;
-        pair_row[oc_idx[m]] = (int32_t)(oc_idx[m2] - nb + pbase);
+        if (__ballot(err) && lane == 0) atomicOr(&uo[u].struct_err, 1u);
     }
-    if (__ballot(err) && threadIdx.x == 0) atomicOr(&uo[u].struct_err, 1u);
 }
 
 hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, uint32_t chunk_bound,
-                        int32_t *pair_row) {
+                        uint32_t group_bound, int32_t *pair_row) {
+    (void)group_bound;
     if (chunk_bound == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_chunk_stats, dim3(chunk_bound), dim3(kMatchThreads), 0, s, b.oc_char,
-                       b.uout, n_units, b.ck, b.ls);
-    hipLaunchKernelGGL(k_chunk_scan, dim3(n_units), dim3(256), 0, s, b.uout, b.ck, b.gmin,
-                       b.gmin1, b.gmin2, b.ls);
-    hipLaunchKernelGGL(k_match_local, dim3(chunk_bound), dim3(kMatchThreads), 0, s, b.oc_char,
-                       b.oc_idx, b.uout, n_units, b.ck, b.lows, b.highs, pair_row, b.ls);
-    hipLaunchKernelGGL(k_match_cross, dim3(chunk_bound), dim3(64), 0, s, b.oc_char, b.oc_idx,
-                       b.uout, n_units, b.ck, b.gmin, b.gmin1, b.gmin2, b.lows, b.highs,
-                       pair_row, b.ls);
+    // The chunk count is only known on the device: grid-stride over a grid
+    // that fills the chip (256 CUs) without launching one block per bound.
+    const uint32_t g = chunk_bound < 2048u ? chunk_bound : 2048u;
+    const uint32_t gc = chunk_bound < 8192u ? chunk_bound : 8192u;
+    hipLaunchKernelGGL(k_chunk_stats, dim3(g), dim3(kMatchThreads), 0, s, b.oc_char, b.uout,
+                       n_units, b.ck, b.ls);
+    hipLaunchKernelGGL(k_chunk_scan, dim3(n_units), dim3(kScanThreads), 0, s, b.uout, b.ck,
+                       b.gmin, b.gmin1, b.gmin2, b.gagg, b.gd0, b.ls);
+    hipLaunchKernelGGL(k_match_local, dim3(g), dim3(kMatchThreads), 0, s, b.oc_char, b.oc_idx,
+                       b.uout, n_units, b.ck, b.lows, b.highs, pair_row, b.ls);
+    hipLaunchKernelGGL(k_match_cross, dim3(gc), dim3(64), 0, s, b.oc_char, b.oc_idx, b.uout,
+                       n_units, b.ck, b.gmin, b.gmin1, b.gmin2, b.lows, b.highs, pair_row, b.ls);
     return hipGetLastError();
 }
 

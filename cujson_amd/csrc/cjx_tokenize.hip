@@ -182,37 +182,80 @@ __device__ __forceinline__ void publish_inc(unsigned long long *st, uint32_t til
                   (2ull << 62) | ((unsigned long long)state << 60) | n_incl);
 }
 
-// One wave walks back from `tile`: lane i reads tile (tile - 1 - i).  When an
-// inclusive tile k lies in the window and every tile after it is an
-// aggregate, the maps of those tiles are scanned from the oldest to give each
-// one's entry state, hence its e bit and its counts; their sum plus tile k's
-// inclusive counts is this tile's exclusive prefix.  Tiles before the unit
-// read as inclusive with state 0 and counts 0.  The walk waits while no
-// inclusive tile is within 64 tiles (the oldest running tile always finds
-// one, so waiting cannot deadlock).
+// One wave walks back from `tile` over a window of 64 x kLookPer tiles: lane
+// i reads tiles tile - 1 - (kLookPer * i + k), k = 0..kLookPer-1 (position
+// q = kLookPer * i + k, 0 = the newest).  When an inclusive tile lies in the
+// window and every newer tile is an aggregate, the maps of those tiles are
+// composed from the oldest to give each one's entry state, hence its e bit
+// and its counts; their sum plus the inclusive tile's counts is this tile's
+// exclusive prefix.  Tiles before the unit read as inclusive with state 0 and
+// counts 0.  The walk waits while no inclusive tile is in the window (the
+// oldest running tile always finds one, so waiting cannot deadlock).  The
+// inclusive frontier advances up to a window per memory round trip, which is
+// what bounds a launch of many tiles.
+#ifndef CJX_LOOK_PER
+#define CJX_LOOK_PER 1
+#endif
+#ifndef CJX_LB_SLEEP
+#define CJX_LB_SLEEP 2
+#endif
+constexpr int kLookPer = CJX_LOOK_PER;
 __device__ __forceinline__ void lookback_tile(const unsigned long long *st, uint32_t tile,
                                               uint32_t unit_tile0, uint32_t &entry,
                                               uint32_t &n_excl, uint32_t &m_excl) {
     const uint32_t lane = lane_id();
-    const int64_t j = (int64_t)tile - 1 - (int64_t)lane;
     for (;;) {
-        unsigned long long a = 2ull << 62, b = 2ull << 62;
-        if (j >= (int64_t)unit_tile0) {
-            a = agent_load64(&st[2 * j]);
-            b = agent_load64(&st[2 * j + 1]);
+        unsigned long long a[kLookPer], b[kLookPer];
+#pragma unroll
+        for (int k = 0; k < kLookPer; k++) {
+            const int64_t j = (int64_t)tile - 1 - (int64_t)(kLookPer * lane + k);
+            a[k] = b[k] = 2ull << 62;
+            if (j >= (int64_t)unit_tile0) {
+                a[k] = agent_load64(&st[2 * j]);
+                b[k] = agent_load64(&st[2 * j + 1]);
+            }
         }
-        const uint32_t fa = (uint32_t)(a >> 62), fb = (uint32_t)(b >> 62);
-        const uint32_t kind = (fa == fb) ? fa : kStInvalid;
-        const uint64_t inc = __ballot(kind == kStInc);
-        const uint64_t notagg = __ballot(kind != kStAgg);
-        if (!inc || (notagg & ((inc & (0ull - inc)) - 1ull))) {
-            __builtin_amdgcn_s_sleep(2);
+        // Newest inclusive entry of the lane, and whether every newer entry
+        // of the lane is an aggregate.
+        uint32_t k_inc = kLookPer, inc_state = 0;
+        bool ok = true;
+#pragma unroll
+        for (int k = kLookPer - 1; k >= 0; k--) {
+            const uint32_t fa = (uint32_t)(a[k] >> 62), fb = (uint32_t)(b[k] >> 62);
+            const uint32_t kind = (fa == fb) ? fa : kStInvalid;
+            if (kind == kStInc) {
+                k_inc = (uint32_t)k;
+                inc_state = (uint32_t)(a[k] >> 60) & 3u;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kLookPer; k++) {
+            const uint32_t fa = (uint32_t)(a[k] >> 62), fb = (uint32_t)(b[k] >> 62);
+            const uint32_t kind = (fa == fb) ? fa : kStInvalid;
+            if ((uint32_t)k < k_inc && kind != kStAgg) ok = false;
+        }
+        const uint64_t inc_lanes = __ballot(k_inc < (uint32_t)kLookPer);
+        if (!inc_lanes) {
+            __builtin_amdgcn_s_sleep(CJX_LB_SLEEP);
             continue;
         }
-        const uint32_t k = (uint32_t)__ffsll((long long)inc) - 1;
-        const uint32_t map = (lane < k) ? (uint32_t)((a >> 58) & 0xFu) : kMapIdentity;
-        // P = map(lane 63) then ... then map(lane): inclusive scan from the oldest tile.
-        uint32_t P = map;
+        const uint32_t li = (uint32_t)__ffsll((long long)inc_lanes) - 1;
+        const uint64_t upto = (li == 63) ? ~0ull : ((2ull << li) - 1ull);
+        if (__ballot(!ok) & upto) {  // a tile newer than the inclusive one is not ready
+            __builtin_amdgcn_s_sleep(CJX_LB_SLEEP);
+            continue;
+        }
+        const uint32_t qi = kLookPer * li + wave_read(k_inc, (int)li);
+        const uint32_t sk = wave_read(inc_state, (int)li);
+        // Lane map: its newer-than-inclusive entries, oldest first.
+        uint32_t L = kMapIdentity;
+#pragma unroll
+        for (int k = kLookPer - 1; k >= 0; k--) {
+            const uint32_t q = kLookPer * lane + (uint32_t)k;
+            if (q < qi) L = map_compose(L, (uint32_t)((a[k] >> 58) & 0xFu));
+        }
+        // P = L(lane 63) then ... then L(lane): inclusive scan from the oldest.
+        uint32_t P = L;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t other = (uint32_t)__shfl_down((int)P, off, 64);
@@ -220,18 +263,24 @@ __device__ __forceinline__ void lookback_tile(const unsigned long long *st, uint
         }
         uint32_t E = (uint32_t)__shfl_down((int)P, 1, 64);
         if (lane == 63) E = kMapIdentity;
-        const uint32_t sk = wave_read((uint32_t)(a >> 60) & 3u, (int)k);
-        const uint32_t stj = map_apply(E, sk);
-        const uint32_t e = ((stj >> 1) ^ (stj & (map >> 2))) & 1u;
-        const uint32_t n0 = (uint32_t)((a >> 29) & kMask29), m0 = (uint32_t)(a & kMask29);
-        const uint32_t opc = (uint32_t)((b >> 29) & kMask29), brc = (uint32_t)(b & kMask29);
+        uint32_t stj = map_apply(E, sk);
         uint32_t cn = 0, cm = 0;
-        if (lane < k) {
-            cn = e ? opc - n0 : n0;
-            cm = e ? brc - m0 : m0;
-        } else if (lane == k) {
-            cn = (uint32_t)(a & kMask31);
-            cm = (uint32_t)(b & kMask31);
+#pragma unroll
+        for (int k = kLookPer - 1; k >= 0; k--) {
+            const uint32_t q = kLookPer * lane + (uint32_t)k;
+            if (q < qi) {
+                const uint32_t map = (uint32_t)((a[k] >> 58) & 0xFu);
+                const uint32_t e = ((stj >> 1) ^ (stj & (map >> 2))) & 1u;
+                const uint32_t n0 = (uint32_t)((a[k] >> 29) & kMask29), m0 = (uint32_t)(a[k] & kMask29);
+                const uint32_t opc = (uint32_t)((b[k] >> 29) & kMask29);
+                const uint32_t brc = (uint32_t)(b[k] & kMask29);
+                cn += e ? opc - n0 : n0;
+                cm += e ? brc - m0 : m0;
+                stj = map_apply(map, stj);
+            } else if (q == qi) {
+                cn += (uint32_t)(a[k] & kMask31);
+                cm += (uint32_t)(b[k] & kMask31);
+            }
         }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
@@ -266,12 +315,13 @@ template <bool JSONL>
 __global__ __launch_bounds__(kTokThreads) void k_tokenize(
     const uint8_t *__restrict__ in, const UnitDesc *__restrict__ units, uint32_t n_units,
     uint32_t ntiles, unsigned long long *__restrict__ tile_st, LaunchState *__restrict__ ls,
-    UnitOut *__restrict__ uo, uint32_t *__restrict__ out_struct, int32_t *__restrict__ out_pair,
-    uint8_t *__restrict__ oc_char, uint32_t *__restrict__ oc_idx) {
+    UnitOut *__restrict__ uo, uint32_t *__restrict__ out_struct, uint8_t *__restrict__ oc_char,
+    uint32_t *__restrict__ oc_idx) {
     __shared__ uint32_t s_tile, s_unit;
     __shared__ uint32_t s_map[kWaves];
     __shared__ uint32_t s_cnt0[kWaves], s_cnt_all[kWaves];
     __shared__ uint32_t s_fix[kWaves], s_nbase[kWaves], s_mbase[kWaves];
+    __shared__ uint32_t s_stage[kWaves][kStageRows];  // one round's positions per wave
 
     const uint32_t tid = threadIdx.x;
     const uint32_t w = tid >> 6, lane = tid & 63;
@@ -468,16 +518,31 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
         uint32_t midx = mb + (excl >> 16);
         const uint32_t pos0 = (uint32_t)(U.pos_base + lane_off + 1);
         uint32_t m = sti;
-        // Structural positions, and -1 in the same pair-row cells (phase 3
-        // overwrites the openers' cells).
-        if ((uint64_t)idx0 + (uint32_t)__popc(sti) <= cap_end) {
+        // Structural positions.  The wave's positions of the round are
+        // gathered in LDS and stored as one contiguous run; a round with more
+        // than kStageRows of them, or one that reaches the end of the region,
+        // is stored lane by lane.
+        const uint32_t rn = rtot & 0xFFFFu;
+        if (rn <= (uint32_t)kStageRows && (uint64_t)nb + rn <= cap_end) {
+            uint32_t k = excl & 0xFFFFu;
+            while (m) {
+                const uint32_t j = (uint32_t)__builtin_ctz(m);
+                m &= m - 1;
+                s_stage[w][k++] = pos0 + j;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (uint32_t t = lane; t < rn; t += 64) out_struct[nb + t] = s_stage[w][t];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else if ((uint64_t)idx0 + (uint32_t)__popc(sti) <= cap_end) {
             uint32_t *p = out_struct + idx0;
-            int32_t *q = out_pair + idx0;
             while (m) {
                 const uint32_t j = (uint32_t)__builtin_ctz(m);
                 m &= m - 1;
                 *p++ = pos0 + j;
-                *q++ = -1;
             }
         } else if (m) {
             overflow = 1;
@@ -485,10 +550,7 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
             while (m) {
                 const uint32_t j = (uint32_t)__builtin_ctz(m);
                 m &= m - 1;
-                if (idx < cap_end) {
-                    out_struct[idx] = pos0 + j;
-                    out_pair[idx] = -1;
-                }
+                if (idx < cap_end) out_struct[idx] = pos0 + j;
                 idx++;
             }
         }
@@ -512,16 +574,15 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
 }
 
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                           uint32_t ntiles, uint32_t n_units, bool jsonl, uint32_t *out_struct,
-                           int32_t *out_pair) {
+                           uint32_t ntiles, uint32_t n_units, bool jsonl, uint32_t *out_struct) {
     if (jsonl)
         hipLaunchKernelGGL(k_tokenize<true>, dim3(ntiles), dim3(kTokThreads), 0, s, d_in, b.units,
-                           n_units, ntiles, b.tile_st, b.ls, b.uout, out_struct, out_pair,
-                           b.oc_char, b.oc_idx);
+                           n_units, ntiles, b.tile_st, b.ls, b.uout, out_struct, b.oc_char,
+                           b.oc_idx);
     else
         hipLaunchKernelGGL(k_tokenize<false>, dim3(ntiles), dim3(kTokThreads), 0, s, d_in, b.units,
-                           n_units, ntiles, b.tile_st, b.ls, b.uout, out_struct, out_pair,
-                           b.oc_char, b.oc_idx);
+                           n_units, ntiles, b.tile_st, b.ls, b.uout, out_struct, b.oc_char,
+                           b.oc_idx);
     return hipGetLastError();
 }
 
