@@ -24,7 +24,33 @@ sys.path.insert(0, ROOT)
 
 METRIC = "input GB/s and % HBM roofline, batched vs plain at 1 GPU; p50/p99 parse latency"
 HBM_PEAK_GBS = 8000.0
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_tokenize_config2_traffic.json")
+# HBM bytes per k_tokenize launch, measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE on this workload and scaled by the copy-kernel calibration; the
+# summary file carries the figure on a line "hbm_bytes_per_launch: <int>".
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_tokenize_plain_2GB_traffic.md")
+
+
+def profiled_traffic():
+    if not os.path.exists(PROFILE_TRAFFIC):
+        return None
+    for line in open(PROFILE_TRAFFIC):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return int(line.split(":", 1)[1].strip())
+    return None
+
+
+def calibrate_copy(nbytes: int, reps: int):
+    """Copies nbytes device-to-device reps times, the known-size kernel the
+    FETCH_SIZE / WRITE_SIZE counters are scaled by."""
+    from cujson_amd.hip import DeviceBuffer, synchronize
+    a = DeviceBuffer(nbytes)
+    b = DeviceBuffer(nbytes)
+    a.fill(1)
+    for _ in range(reps):
+        b.copy_from(a, nbytes)
+    synchronize()
+    print(json.dumps({"calibration": "device-to-device copy", "bytes": nbytes, "reps": reps}),
+          flush=True)
 
 
 def log(msg):
@@ -72,6 +98,8 @@ def main():
     ap.add_argument("--bytes", type=int, default=2_000_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-bytes", type=int, default=512 << 20)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only copy this many bytes device-to-device (counter calibration)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -79,6 +107,10 @@ def main():
                                    "(the path shards by byte range with a tiny carry exchange; "
                                    "multi-GPU is not built)"}))
         sys.exit(2)
+
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy, max(args.steps, 1))
+        return
 
     import cujson_amd as cj
     from cujson_amd import gen
@@ -136,12 +168,7 @@ def main():
     tok_bytes = n + 4 * N + 5 * M          # k_tokenize: read input, write rows it owns
     path_bytes = n + 8 * (N + 2)            # the whole path's algorithmic bytes
     achieved = tok_bytes / (tok_ms * 1e-3) / 1e9
-    traffic = None
-    if os.path.exists(PROFILE_TRAFFIC):
-        try:
-            traffic = json.load(open(PROFILE_TRAFFIC)).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+    traffic = profiled_traffic() if args.bytes == 2_000_000_000 else None
 
     out = {
         "metric": METRIC,

@@ -322,6 +322,10 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
     __shared__ uint32_t s_cnt0[kWaves], s_cnt_all[kWaves];
     __shared__ uint32_t s_fix[kWaves], s_nbase[kWaves], s_mbase[kWaves];
     __shared__ uint32_t s_stage[kWaves][kStageRows];  // one round's positions per wave
+    // Bracket characters as two bit masks per lane-round, kept in LDS until
+    // compaction: byte bit 1 ('[' / '{' have it, ']' / '}' do not) and byte
+    // bit 5 (curly), both restricted to bracket bytes.
+    __shared__ uint32_t s_b1[kWaves][kTokRounds][64], s_b5[kWaves][kTokRounds][64];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t w = tid >> 6, lane = tid & 63;
@@ -409,6 +413,8 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
         const uint32_t q = extract<7>(k);
         op_r[i] = extract<7>(u7);
         br_r[i] = extract<2>(k) & op_r[i];
+        s_b1[w][i][lane] = extract<1>(r) & br_r[i];
+        s_b5[w][i][lane] = extract<5>(r) & br_r[i];
 
         const uint32_t all_bs = (bs == 0xFFFFFFFFu);
         const uint32_t run = (uint32_t)__clz(~bs);  // trailing backslash run of the lane
@@ -555,13 +561,16 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
             }
         }
         m = oci;
+        const uint32_t b1 = s_b1[w][i][lane], b5 = s_b5[w][i][lane];
         while (m) {
             const uint32_t j = (uint32_t)__builtin_ctz(m);
             m &= m - 1;
             const uint32_t t = idx0 + (uint32_t)__popc(sti & ((1u << j) - 1u));
             if (midx < cap_end) {
                 oc_idx[midx] = t;
-                oc_char[midx] = ubase[lane_off + j];
+                // '[' 0x5B, ']' 0x5D = 0x5B ^ 6, '{' 0x7B, '}' 0x7D
+                const uint32_t open = (b1 >> j) & 1u, curly = (b5 >> j) & 1u;
+                oc_char[midx] = (uint8_t)((0x5Bu | (curly << 5)) ^ (open ? 0u : 6u));
             } else {
                 overflow = 1;
             }

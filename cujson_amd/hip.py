@@ -67,6 +67,12 @@ class DeviceBuffer:
                                D2H), "hipMemcpy D2H")
         return out
 
+    def copy_from(self, src: "DeviceBuffer", nbytes: int):
+        """Device-to-device copy (a runtime copy kernel)."""
+        assert nbytes <= self.nbytes and nbytes <= src.nbytes
+        _check(hip().hipMemcpy(ctypes.c_void_p(self.ptr), ctypes.c_void_p(src.ptr), nbytes, D2D),
+               "hipMemcpy D2D")
+
     def fill(self, byte: int):
         _check(hip().hipMemset(ctypes.c_void_p(self.ptr), byte, self.nbytes), "hipMemset")
 
