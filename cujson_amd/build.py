@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcujson_amd.so")
 ARCH = os.environ.get("CJX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["cjx_tokenize.hip", "cjx_kernels.hip", "cjx_match.hip", "cjx_api.cpp"]
+SOURCES = ["cjx_tokenize.hip", "cjx_utf8.hip", "cjx_kernels.hip", "cjx_match.hip", "cjx_api.cpp"]
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     # packed fp32 VALU ops are disabled for every HIP object of this library
@@ -34,9 +34,10 @@ def _run(cmd):
 
 
 def build_library(verbose: bool = False) -> str:
-    objs = []
+    from concurrent.futures import ThreadPoolExecutor
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
+    cmds, objs = [], []
     for src in SOURCES:
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         cmd = ["hipcc", *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-c",
@@ -44,10 +45,12 @@ def build_library(verbose: bool = False) -> str:
         if src.endswith(".cpp"):
             cmd.insert(1, "-x")
             cmd.insert(2, "hip")
-        out = _run(cmd)
-        if verbose and out:
-            print(out)
+        cmds.append(cmd)
         objs.append(obj)
+    with ThreadPoolExecutor(max_workers=min(len(cmds), 8)) as pool:
+        for out in pool.map(_run, cmds):
+            if verbose and out:
+                print(out)
     tmp = LIB + ".tmp"
     _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
@@ -65,6 +68,19 @@ def build_generator() -> str:
     return GEN_LIB
 
 
+WRAPPER_CHECK = os.path.join(HERE, "cjx_wrapper_check")
+
+
+def build_wrapper_check() -> str:
+    """The C++ program that drives the reference-shaped headers (include/cujson/)."""
+    tmp = WRAPPER_CHECK + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", tmp,
+          os.path.join(ROOT, "tests", "cpp", "wrapper_check.cpp"), "-L", HERE, "-lcujson_amd",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, WRAPPER_CHECK)
+    return WRAPPER_CHECK
+
+
 def build_oracle() -> None:
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"], check=True)
 
@@ -72,4 +88,5 @@ def build_oracle() -> None:
 if __name__ == "__main__":
     print(build_library(verbose=True))
     print(build_generator())
+    print(build_wrapper_check())
     build_oracle()

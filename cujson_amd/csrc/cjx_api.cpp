@@ -222,8 +222,11 @@ int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsign
     if (launch_reset(s, b, sh.ntiles, sh.n_units) != hipSuccess) return CJX_ERR_DEVICE;
     hipEvent_t *tev = ctx->tev[ctx->timing_count % cjx_context::kTimingRing];
     if (ctx->timing) (void)hipEventRecord(tev[0], s);
+    const bool strict = (flags & CJX_FLAG_UTF8_STRICT) != 0;
+    if (strict && launch_utf8_strict(s, b, d_in, sh.ntiles, sh.n_units) != hipSuccess)
+        return CJX_ERR_DEVICE;
     if (launch_tokenize(s, b, d_in, sh.ntiles, sh.n_units, (flags & CJX_FLAG_JSONL) != 0,
-                        d_struct) != hipSuccess)
+                        !strict, d_struct) != hipSuccess)
         return CJX_ERR_DEVICE;
     if (ctx->timing) (void)hipEventRecord(tev[1], s);
     // Concatenated results number pairs across units (JSON Lines).
@@ -550,7 +553,6 @@ int cjx_parse_standard(cjx_context *ctx, const uint8_t *data, size_t n, unsigned
     if (!out) return CJX_ERR_INPUT;
     zero_result(out);
     if (!ctx || !data || n == 0) return out->error = CJX_ERR_INPUT;  // PSJ:1597-1600
-    if (flags & CJX_FLAG_UTF8_STRICT) return out->error = CJX_ERR_INPUT;  // not built yet
     std::vector<uint64_t> offs{0}, sizes{n}, pos{0};
     int rc = parse_host_units(ctx, data, offs, sizes, pos, flags & ~CJX_FLAG_JSONL, out);
     out->error = rc;
@@ -565,7 +567,6 @@ int cjx_parse_lines(cjx_context *ctx, const uint8_t *data, size_t n, const size_
     // Argument checks of parse_json_lines.cu:1103-1150.
     if (!ctx || n == 0 || !data || n_chunks == 0 || !chunk_off || !chunk_size)
         return out->error = CJX_ERR_INPUT;
-    if (flags & CJX_FLAG_UTF8_STRICT) return out->error = CJX_ERR_INPUT;
     std::vector<uint64_t> offs(n_chunks), sizes(n_chunks), pos(n_chunks);
     uint64_t last_chunk_index = 0;
     for (size_t i = 0; i < n_chunks; i++) {
@@ -598,7 +599,6 @@ int cjx_parse_device(cjx_context *ctx, const uint8_t *d_input, size_t n, unsigne
                      int32_t *d_structural, int32_t *d_pair, size_t cap) {
     if (!ctx || !d_input || n == 0 || !d_structural || !d_pair) return CJX_ERR_INPUT;
     if (n > kMaxLaunchBytes) return CJX_ERR_INPUT;
-    if (flags & CJX_FLAG_UTF8_STRICT) return CJX_ERR_INPUT;
     int rc = set_device(ctx);
     if (rc) return rc;
     std::vector<UnitDesc> units;

@@ -120,7 +120,10 @@ struct LaunchBufs {
 
 hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uint32_t n_units);
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                           uint32_t ntiles, uint32_t n_units, bool jsonl, uint32_t *out_struct);
+                           uint32_t ntiles, uint32_t n_units, bool jsonl, bool weak_utf8,
+                           uint32_t *out_struct);
+hipError_t launch_utf8_strict(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
+                              uint32_t ntiles, uint32_t n_units);
 hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool concat);
 hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, uint32_t chunk_bound,
                         uint32_t group_bound, int32_t *pair_row);

@@ -316,7 +316,7 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
     const uint8_t *__restrict__ in, const UnitDesc *__restrict__ units, uint32_t n_units,
     uint32_t ntiles, unsigned long long *__restrict__ tile_st, LaunchState *__restrict__ ls,
     UnitOut *__restrict__ uo, uint32_t *__restrict__ out_struct, uint8_t *__restrict__ oc_char,
-    uint32_t *__restrict__ oc_idx) {
+    uint32_t *__restrict__ oc_idx, uint32_t weak_utf8) {
     __shared__ uint32_t s_tile, s_unit;
     __shared__ uint32_t s_map[kWaves];
     __shared__ uint32_t s_cnt0[kWaves], s_cnt_all[kWaves];
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
             const uint32_t high = ((r[0] | r[1] | r[2] | r[3] | r[4] | r[5] | r[6] | r[7]) &
                                    0x80808080u) |
                                   ((pk5 | pk6 | pk7) & 0x38000000u) | (pr7 & 0x80000000u);
-            if (__ballot(high != 0)) {
+            if (weak_utf8 && __ballot(high != 0)) {
                 uint32_t vA = ~0u, vB = ~0u;
                 if (lane_off + 32 > U.n) {
                     vA = vB = 0;
@@ -583,15 +583,17 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
 }
 
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                           uint32_t ntiles, uint32_t n_units, bool jsonl, uint32_t *out_struct) {
+                           uint32_t ntiles, uint32_t n_units, bool jsonl, bool weak_utf8,
+                           uint32_t *out_struct) {
+    const uint32_t weak = weak_utf8 ? 1u : 0u;
     if (jsonl)
         hipLaunchKernelGGL(k_tokenize<true>, dim3(ntiles), dim3(kTokThreads), 0, s, d_in, b.units,
                            n_units, ntiles, b.tile_st, b.ls, b.uout, out_struct, b.oc_char,
-                           b.oc_idx);
+                           b.oc_idx, weak);
     else
         hipLaunchKernelGGL(k_tokenize<false>, dim3(ntiles), dim3(kTokThreads), 0, s, d_in, b.units,
                            n_units, ntiles, b.tile_st, b.ls, b.uout, out_struct, b.oc_char,
-                           b.oc_idx);
+                           b.oc_idx, weak);
     return hipGetLastError();
 }
 

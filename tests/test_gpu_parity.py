@@ -147,6 +147,44 @@ def test_utf8_at_word_and_tile_edges(ctx):
                 check_standard(ctx, pre + s + b'bcdefgh"]', label=f"utf8 edge={edge} {s} {shift}")
 
 
+STRICT_SEQS = [b"\xc3\xa9", b"\xe2\x82\xac", b"\xf0\x9f\x98\x80", b"\xf4\x8f\xbf\xbf",
+               b"\xc3", b"\xe2\x82", b"\xf0\x9f\x98", b"\xc0\xaf", b"\xc1\xbf",
+               b"\xe0\x80\xaf", b"\xe0\xa0\x80", b"\xed\xa0\x80", b"\xed\x9f\xbf",
+               b"\xf0\x80\x80\x80", b"\xf0\x90\x80\x80", b"\xf4\x90\x80\x80", b"\xf5\x80",
+               b"\xff", b"\x80", b"\xbf\xbf", b"\xc3\xa9\xa9", b"\xe2\x82\xac\x80"]
+
+
+def test_strict_utf8_at_edges(ctx):
+    for edge in (4, 64, 2048, 16384, 65536):
+        for sq in STRICT_SEQS:
+            for shift in range(-5, 2):
+                pre = b'["' + b"a" * (edge - 2 + shift)
+                check_standard(ctx, pre + sq + b'bc"]', oracle.FLAG_UTF8_STRICT,
+                               label=f"strict edge={edge} {sq} {shift}")
+
+
+def test_strict_utf8_at_end_and_start(ctx):
+    for sq in STRICT_SEQS:
+        check_standard(ctx, sq, oracle.FLAG_UTF8_STRICT, label=f"strict alone {sq}")
+        check_standard(ctx, b'["x",' + sq, oracle.FLAG_UTF8_STRICT, label=f"strict tail {sq}")
+        check_standard(ctx, sq + b'["x"]', oracle.FLAG_UTF8_STRICT, label=f"strict head {sq}")
+
+
+def test_strict_utf8_random(ctx):
+    rng = random.Random(99)
+    pool = [bytes([b]) for b in range(0x20, 0x7F, 7)] + STRICT_SEQS
+    for it in range(60):
+        n = rng.choice([10, 300, 5000, 70000])
+        data = b"".join(rng.choice(pool) for _ in range(n))
+        check_standard(ctx, data, oracle.FLAG_UTF8_STRICT, label=f"strict soup {it}")
+    d = gen.config1()
+    check_standard(ctx, d, oracle.FLAG_UTF8_STRICT, label="strict config1")
+    for kind in range(5):
+        f = d.copy()
+        assert gen.plant_fault(f, kind, 7)
+        check_standard(ctx, f, oracle.FLAG_UTF8_STRICT, label=f"strict fault {kind}")
+
+
 def test_config2_shape_64mb(ctx):
     d = gen.config2(target_bytes=64 << 20)
     check_standard(ctx, d, label="config2 64MB")
