@@ -47,6 +47,7 @@ CJX_ERR_CAPACITY = 6
 CJX_FLAG_UTF8_STRICT = 0x1
 CJX_FLAG_DEPTH_U16 = 0x2
 CJX_FLAG_JSONL = 0x4
+CJX_FLAG_GRAPH = 0x8
 
 # Every symbol include/cujson_amd.h declares.
 ABI_SYMBOLS = (
@@ -54,6 +55,7 @@ ABI_SYMBOLS = (
     "cjx_chunk_lines", "cjx_parse_standard", "cjx_parse_lines", "cjx_free_result",
     "cjx_strerror", "cjx_parse_device", "cjx_device_wait", "cjx_context_stream",
     "cjx_version", "cjx_context_set_timing", "cjx_context_timings",
+    "cjx_parse_batch", "cjx_free_batch_result", "cjx_parse_batch_device", "cjx_batch_wait",
 )
 
 
@@ -87,6 +89,20 @@ class _Timings(ctypes.Structure):
 class _DeviceStatus(ctypes.Structure):
     _fields_ = [("n_structural", ctypes.c_int64), ("n_brackets", ctypes.c_int64),
                 ("error", ctypes.c_int32), ("capacity_exceeded", ctypes.c_int32)]
+
+
+class _DocResult(ctypes.Structure):
+    _fields_ = [("byte_base", ctypes.c_int64), ("struct_begin", ctypes.c_int64),
+                ("struct_end", ctypes.c_int64), ("error", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class _BatchResult(ctypes.Structure):
+    _fields_ = [("structural", ctypes.POINTER(ctypes.c_int32)),
+                ("pair_pos", ctypes.POINTER(ctypes.c_int32)),
+                ("n_structural", ctypes.c_int64), ("n_docs", ctypes.c_int64),
+                ("docs", ctypes.POINTER(_DocResult)), ("error", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("device_ms", ctypes.c_double)]
 
 
 _lib = None
@@ -134,6 +150,16 @@ def lib():
     L.cjx_context_timings.restype = ctypes.c_int
     L.cjx_version.argtypes = []
     L.cjx_version.restype = ctypes.c_char_p
+    L.cjx_parse_batch.argtypes = [vp, u8p, szp, szp, sz, ctypes.c_uint,
+                                  ctypes.POINTER(_BatchResult)]
+    L.cjx_parse_batch.restype = ctypes.c_int
+    L.cjx_free_batch_result.argtypes = [ctypes.POINTER(_BatchResult)]
+    L.cjx_free_batch_result.restype = None
+    L.cjx_parse_batch_device.argtypes = [vp, vp, szp, szp, sz, ctypes.c_uint, vp, vp, sz]
+    L.cjx_parse_batch_device.restype = ctypes.c_int
+    L.cjx_batch_wait.argtypes = [vp, ctypes.POINTER(_DocResult), sz,
+                                 ctypes.POINTER(_DeviceStatus)]
+    L.cjx_batch_wait.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -194,6 +220,28 @@ class Context:
         st = _DeviceStatus()
         rc = lib().cjx_device_wait(self._h, ctypes.byref(st))
         return rc, st.n_structural, st.n_brackets, bool(st.capacity_exceeded)
+
+    def parse_batch_device(self, d_input: int, doc_off, doc_size, d_structural: int,
+                           d_pair: int, cap: int, flags: int = 0) -> int:
+        """Enqueue a batch of device-resident documents (offsets 16-byte aligned)."""
+        k = len(doc_off)
+        self._batch_n = k
+        offs = (ctypes.c_size_t * k)(*doc_off)
+        sizes = (ctypes.c_size_t * k)(*doc_size)
+        return lib().cjx_parse_batch_device(self._h, ctypes.c_void_p(d_input), offs, sizes, k,
+                                            flags, ctypes.c_void_p(d_structural),
+                                            ctypes.c_void_p(d_pair), cap)
+
+    def batch_wait(self):
+        """(rc, docs, N, M, overflow): docs is an int64 array [k, 4] of
+        (byte_base, struct_begin, struct_end, error)."""
+        k = getattr(self, "_batch_n", 0)
+        arr = (_DocResult * max(k, 1))()
+        st = _DeviceStatus()
+        rc = lib().cjx_batch_wait(self._h, arr, k, ctypes.byref(st))
+        docs = np.array([(d.byte_base, d.struct_begin, d.struct_end, d.error) for d in arr[:k]],
+                        dtype=np.int64).reshape(-1, 4)
+        return rc, docs, st.n_structural, st.n_brackets, bool(st.capacity_exceeded)
 
 
 _default_ctx: Optional[Context] = None
@@ -365,3 +413,63 @@ def parse_json_lines(input: cuJSONLinesInput, flags: int = 0,
         nch = 0
     lib().cjx_parse_lines(ctx.handle, p, n, offs, sizes, nch, flags, ctypes.byref(r))
     return cuJSONResult(r)
+
+
+class BatchResult:
+    """Result of parse_batch: packed rows of all documents plus a per-document
+    table.  ``doc(i)`` rebuilds the cuJSONResult buffer parse_standard_json
+    would return for document i (2(N_i+2) cells, sentinels, -1 cells)."""
+
+    def __init__(self, r: _BatchResult, rc: int):
+        self._r = r
+        self.rc = rc
+        self.error = int(r.error)
+        self.N = int(r.n_structural)
+        self.n_docs = int(r.n_docs)
+        self.device_ms = float(r.device_ms)
+        if rc == CJX_OK and r.structural:
+            self.structural = np.ctypeslib.as_array(r.structural, shape=(max(self.N, 1),))[:self.N]
+            self.pair_pos = np.ctypeslib.as_array(r.pair_pos, shape=(max(self.N, 1),))[:self.N]
+            self.docs = np.array([(d.byte_base, d.struct_begin, d.struct_end, d.error)
+                                  for d in r.docs[:self.n_docs]], dtype=np.int64).reshape(-1, 4)
+        else:
+            self.structural = self.pair_pos = None
+            self.docs = np.zeros((0, 4), np.int64)
+
+    def doc(self, i: int):
+        """(error, buf) of document i in the reference's host layout."""
+        _, b, e, err = self.docs[i]
+        n = int(e - b)
+        buf = np.empty(2 * (n + 2), dtype=np.int32)
+        buf[0] = 0
+        buf[1:n + 1] = self.structural[b:e]
+        buf[n + 1] = n + 1
+        buf[n + 2:2 * n + 2] = self.pair_pos[b:e]
+        buf[2 * n + 2:] = -1
+        return int(err), buf
+
+    def free(self):
+        if self._r is not None:
+            lib().cjx_free_batch_result(ctypes.byref(self._r))
+            self._r = None
+            self.structural = self.pair_pos = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def parse_batch(data, doc_off, doc_size, flags: int = 0,
+                ctx: Optional[Context] = None) -> BatchResult:
+    """A batch of independent standard-JSON documents in one launch sequence
+    (documents are data[doc_off[i] : doc_off[i] + doc_size[i]])."""
+    a, p = _as_u8(data)
+    ctx = ctx or default_context()
+    k = len(doc_off)
+    offs = (ctypes.c_size_t * max(k, 1))(*doc_off)
+    sizes = (ctypes.c_size_t * max(k, 1))(*doc_size)
+    r = _BatchResult()
+    rc = lib().cjx_parse_batch(ctx.handle, p, offs, sizes, k, flags, ctypes.byref(r))
+    return BatchResult(r, rc)

@@ -86,10 +86,23 @@ struct cjx_context {
     DevBuf units, uout, ls, tile_st, layout;
     DevBuf oc_char, oc_idx, keys, sorted, tmp, spos, block_delta, hist;
     DevBuf ck, gmin, gmin1, gmin2, gagg, gd0, lows, highs;
-    PinnedBuf h_units, h_uout, h_ls;
+    PinnedBuf h_units, h_uout, h_ls, h_stage;
+    std::vector<uint64_t> batch_off;  // byte offsets of the documents of the batch in flight
     // Shape of the launch in flight (device form).
     uint32_t n_units = 0;
     bool events_recorded = false;
+    // Captured launch sequence of the last graph-mode call and what it baked in.
+    hipGraphExec_t graph = nullptr;
+    struct GraphKey {
+        LaunchBufs b;
+        const uint8_t *d_in;
+        uint32_t *d_struct;
+        int32_t *d_pair;
+        uint32_t ntiles, n_units;
+        uint64_t rows, max_rows, chunks;
+        unsigned flags;
+        bool concat;
+    } graph_key{};
 };
 
 namespace {
@@ -152,6 +165,7 @@ struct Shape {
     uint64_t max_rows;  // largest region
     uint64_t chunks;    // bound on the matching chunks of all units
     uint64_t groups;    // bound on their 64-chunk groups (per unit, rounded up)
+    bool concat;        // pair values numbered across units (JSON Lines), else per unit
 };
 
 // Fills the row totals of a shape from the units' regions.
@@ -230,17 +244,77 @@ int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsign
         return CJX_ERR_DEVICE;
     if (ctx->timing) (void)hipEventRecord(tev[1], s);
     // Concatenated results number pairs across units (JSON Lines).
-    if (launch_unit_bases(s, b, sh.n_units, sh.n_units > 1) != hipSuccess) return CJX_ERR_DEVICE;
+    if (launch_unit_bases(s, b, sh.n_units, sh.concat) != hipSuccess) return CJX_ERR_DEVICE;
     // Well-nested units are matched directly; the rest take the sort path.
     if (launch_match(s, b, sh.n_units, (uint32_t)sh.chunks, (uint32_t)sh.groups, d_pair) !=
         hipSuccess)
         return CJX_ERR_DEVICE;
-    const uint32_t G = phase3_blocks(sh.max_rows);
-    const uint32_t PG = pair_blocks(sh.max_rows);
-    for (uint32_t u = 0; u < sh.n_units; u++)
-        if (launch_phase3(s, b, u, G, PG, u16, d_pair) != hipSuccess) return CJX_ERR_DEVICE;
+    if (sh.n_units == 1) {  // one unit: the multi-block sort path
+        const uint32_t G = phase3_blocks(sh.max_rows);
+        const uint32_t PG = pair_blocks(sh.max_rows);
+        if (launch_phase3(s, b, 0, G, PG, u16, d_pair) != hipSuccess) return CJX_ERR_DEVICE;
+    } else if (launch_sort_units(s, b, sh.n_units, u16, d_pair) != hipSuccess) {
+        return CJX_ERR_DEVICE;
+    }
     if (ctx->timing) {
         (void)hipEventRecord(tev[2], s);
+        ctx->timing_count++;
+    }
+    return CJX_OK;
+}
+
+// enqueue_parse, or (CJX_FLAG_GRAPH) a replay of the same sequence captured
+// as a HIP graph.  The graph is re-captured whenever the launch shape, the
+// flags or any device pointer baked into its kernel arguments changes; the
+// inputs it reads (unit table, input bytes) are written before the launch.
+int enqueue_maybe_graph(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsigned flags,
+                        uint32_t *d_struct, int32_t *d_pair) {
+    if (!(flags & CJX_FLAG_GRAPH)) return enqueue_parse(ctx, d_in, sh, flags, d_struct, d_pair);
+    if ((flags & CJX_FLAG_DEPTH_U16) && !ctx->tmp.ensure(sh.rows * 4 + 16)) return CJX_ERR_ALLOC;
+    cjx_context::GraphKey key{};
+    key.b = bufs_of(ctx);
+    key.d_in = d_in;
+    key.d_struct = d_struct;
+    key.d_pair = d_pair;
+    key.ntiles = sh.ntiles;
+    key.n_units = sh.n_units;
+    key.rows = sh.rows;
+    key.max_rows = sh.max_rows;
+    key.chunks = sh.chunks;
+    key.flags = flags;
+    key.concat = sh.concat;
+    if (!ctx->graph || std::memcmp(&key, &ctx->graph_key, sizeof(key)) != 0) {
+        if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+        ctx->graph = nullptr;
+        const bool timing = ctx->timing;
+        ctx->timing = false;  // no event records inside the graph
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            ctx->timing = timing;
+            return CJX_ERR_DEVICE;
+        }
+        const int rc = enqueue_parse(ctx, d_in, sh, flags, d_struct, d_pair);
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+        ctx->timing = timing;
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess || !g) return CJX_ERR_DEVICE;
+        const hipError_t ei = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            ctx->graph = nullptr;
+            return CJX_ERR_DEVICE;
+        }
+        std::memcpy(&ctx->graph_key, &key, sizeof(key));
+    }
+    hipEvent_t *tev = ctx->tev[ctx->timing_count % cjx_context::kTimingRing];
+    if (ctx->timing) (void)hipEventRecord(tev[0], ctx->stream);
+    if (hipGraphLaunch(ctx->graph, ctx->stream) != hipSuccess) return CJX_ERR_DEVICE;
+    if (ctx->timing) {  // the whole graph is reported as tokenize, phase 3 as 0
+        (void)hipEventRecord(tev[1], ctx->stream);
+        (void)hipEventRecord(tev[2], ctx->stream);
         ctx->timing_count++;
     }
     return CJX_OK;
@@ -272,12 +346,15 @@ void zero_result(cjx_result *out) {
 // Parses units whose bytes are on the host: copies them in, parses, and
 // assembles the reference's host layout.  Retries once with full capacity
 // when the first capacity guess is exceeded.
-int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint64_t> &offs,
-                     const std::vector<uint64_t> &sizes, const std::vector<uint64_t> &pos_base,
-                     unsigned flags, cjx_result *out) {
+// Copies units whose bytes are on the host into the device input (through a
+// pinned staging buffer when there are many small ones) and parses them,
+// retrying once with full output regions if the first guess overflows.
+// Leaves the rows in ctx->structural / ctx->pair and the status in h_uout/h_ls.
+int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint64_t> &offs,
+                   const std::vector<uint64_t> &sizes, const std::vector<uint64_t> &pos_base,
+                   unsigned flags, bool concat, std::vector<UnitDesc> &units, float *ms) {
     int rc = set_device(ctx);
     if (rc) return rc;
-    std::vector<UnitDesc> units;
     uint64_t dev_bytes = 0;
     const uint32_t ntiles = plan_units(sizes, pos_base, units, dev_bytes);
     uint64_t total = 0;
@@ -285,12 +362,22 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
     if (ntiles == 0 || total > kMaxLaunchBytes) return CJX_ERR_INPUT;
     if (!ctx->in.ensure(dev_bytes)) return CJX_ERR_ALLOC;
     hipStream_t s = ctx->stream;
-    for (size_t i = 0; i < units.size(); i++)
-        if (hipMemcpyAsync(ctx->in.as<uint8_t>() + units[i].in_off, data + offs[i], sizes[i],
-                           hipMemcpyHostToDevice, s) != hipSuccess)
+    if (units.size() > 16) {  // one pinned staging copy and one H2D instead of one per unit
+        if (!ctx->h_stage.ensure(dev_bytes)) return CJX_ERR_ALLOC;
+        uint8_t *st = static_cast<uint8_t *>(ctx->h_stage.p);
+        if (hipStreamSynchronize(s) != hipSuccess) return CJX_ERR_DEVICE;  // staging reuse
+        for (size_t i = 0; i < units.size(); i++)
+            std::memcpy(st + units[i].in_off, data + offs[i], sizes[i]);
+        if (hipMemcpyAsync(ctx->in.p, st, dev_bytes, hipMemcpyHostToDevice, s) != hipSuccess)
             return CJX_ERR_DEVICE;
+    } else {
+        for (size_t i = 0; i < units.size(); i++)
+            if (hipMemcpyAsync(ctx->in.as<uint8_t>() + units[i].in_off, data + offs[i], sizes[i],
+                               hipMemcpyHostToDevice, s) != hipSuccess)
+                return CJX_ERR_DEVICE;
+    }
 
-    Shape sh{ntiles, (uint32_t)units.size(), 0, 0, 0, 0};
+    Shape sh{ntiles, (uint32_t)units.size(), 0, 0, 0, 0, concat};
     for (int attempt = 0; attempt < 2; attempt++) {
         set_regions(units, attempt == 1);
         shape_rows(sh, units);
@@ -311,9 +398,19 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
         if (!ls->overflow) break;
         if (attempt == 1) return CJX_ERR_CAPACITY;
     }
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    *ms = 0.f;
+    (void)hipEventElapsedTime(ms, ctx->ev0, ctx->ev1);
+    return CJX_OK;
+}
 
+int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint64_t> &offs,
+                     const std::vector<uint64_t> &sizes, const std::vector<uint64_t> &pos_base,
+                     unsigned flags, cjx_result *out) {
+    std::vector<UnitDesc> units;
+    float ms = 0.f;
+    int rc = run_host_units(ctx, data, offs, sizes, pos_base, flags, sizes.size() > 1, units, &ms);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
     const UnitOut *uo = static_cast<const UnitOut *>(ctx->h_uout.p);
     const LaunchState *ls = static_cast<const LaunchState *>(ctx->h_ls.p);
     const uint64_t N = ls->n_total;
@@ -347,7 +444,7 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
         ok = ctx->layout.ensure(cells * 4) &&
              launch_assemble(s, bufs_of(ctx), (uint32_t)units.size(),
                              ctx->structural.as<uint32_t>(), ctx->pair.as<int32_t>(),
-                             ctx->layout.as<int32_t>(), (uint32_t)N) == hipSuccess &&
+                             ctx->layout.as<int32_t>(), nullptr, (uint32_t)N, true) == hipSuccess &&
              hipMemcpyAsync(buf, ctx->layout.p, cells * 4, hipMemcpyDeviceToHost, s) == hipSuccess;
     }
     if (!ok || hipStreamSynchronize(s) != hipSuccess) {
@@ -373,6 +470,56 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
         out->result_sizes_prefix[i] = (int32_t)prefix;
     }
     return out->error;
+}
+
+// Batch of independent documents: positions and pair values are local to
+// each document; rows are packed densely on the device and copied once.
+int parse_host_batch(cjx_context *ctx, const uint8_t *data, const std::vector<uint64_t> &offs,
+                     const std::vector<uint64_t> &sizes, unsigned flags, cjx_batch_result *out) {
+    std::vector<UnitDesc> units;
+    float ms = 0.f;
+    std::vector<uint64_t> zero(sizes.size(), 0);
+    int rc = run_host_units(ctx, data, offs, sizes, zero, flags, false, units, &ms);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    const UnitOut *uo = static_cast<const UnitOut *>(ctx->h_uout.p);
+    const LaunchState *ls = static_cast<const LaunchState *>(ctx->h_ls.p);
+    const uint64_t N = ls->n_total;
+    int32_t *rows = nullptr;
+    if (hipHostMalloc((void **)&rows, (2 * N + 2) * sizeof(int32_t), hipHostMallocDefault) !=
+        hipSuccess)
+        return CJX_ERR_ALLOC;
+    bool ok = true;
+    if (N) {
+        ok = ctx->layout.ensure(2 * N * 4 + 16) &&
+             launch_assemble(s, bufs_of(ctx), (uint32_t)units.size(),
+                             ctx->structural.as<uint32_t>(), ctx->pair.as<int32_t>(),
+                             ctx->layout.as<int32_t>(), ctx->layout.as<int32_t>() + N,
+                             (uint32_t)N, false) == hipSuccess &&
+             hipMemcpyAsync(rows, ctx->layout.p, 2 * N * 4, hipMemcpyDeviceToHost, s) ==
+                 hipSuccess;
+    }
+    if (!ok || hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipHostFree(rows);
+        return ok ? CJX_ERR_DEVICE : CJX_ERR_ALLOC;
+    }
+    out->structural = rows;
+    out->pair_pos = rows + N;
+    out->n_structural = (int64_t)N;
+    out->n_docs = (int64_t)units.size();
+    out->device_ms = ms;
+    out->docs = static_cast<cjx_doc_result *>(std::calloc(units.size(), sizeof(cjx_doc_result)));
+    int first_err = CJX_OK;
+    for (size_t i = 0; i < units.size(); i++) {
+        cjx_doc_result &d = out->docs[i];
+        d.byte_base = (int64_t)offs[i];
+        d.struct_begin = uo[i].row_base;
+        d.struct_end = (int64_t)uo[i].row_base + (uo[i].n_end - uo[i].n_base);
+        d.error = unit_code(uo[i]);
+        if (d.error && !first_err) first_err = d.error;
+    }
+    out->error = first_err;
+    return CJX_OK;
 }
 
 }  // namespace
@@ -425,11 +572,13 @@ void cjx_context_destroy(cjx_context *ctx) {
     ctx->h_units.release();
     ctx->h_uout.release();
     ctx->h_ls.release();
+    ctx->h_stage.release();
     for (auto &tr : ctx->tev)
         for (hipEvent_t e : tr)
             if (e) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -606,7 +755,7 @@ int cjx_parse_device(cjx_context *ctx, const uint8_t *d_input, size_t n, unsigne
     const uint32_t ntiles = plan_units({n}, {0}, units, dev_bytes);
     units[0].out_off = 0;
     units[0].out_cap = (uint32_t)std::min<uint64_t>(cap, n);
-    Shape sh{ntiles, 1, 0, 0, 0, 0};
+    Shape sh{ntiles, 1, 0, 0, 0, 0, false};
     shape_rows(sh, units);
     if ((rc = ensure_work(ctx, sh))) return rc;
     std::memcpy(ctx->h_units.p, units.data(), sizeof(UnitDesc));
@@ -615,7 +764,7 @@ int cjx_parse_device(cjx_context *ctx, const uint8_t *d_input, size_t n, unsigne
         return CJX_ERR_DEVICE;
     ctx->n_units = 1;
     (void)hipEventRecord(ctx->ev0, ctx->stream);
-    rc = enqueue_parse(ctx, d_input, sh, flags & ~CJX_FLAG_JSONL,
+    rc = enqueue_maybe_graph(ctx, d_input, sh, flags & ~CJX_FLAG_JSONL,
                        reinterpret_cast<uint32_t *>(d_structural), d_pair);
     (void)hipEventRecord(ctx->ev1, ctx->stream);
     ctx->events_recorded = true;
@@ -637,6 +786,100 @@ int cjx_device_wait(cjx_context *ctx, cjx_device_status *st) {
         st->capacity_exceeded = ls->overflow ? 1 : 0;
     }
     return code;
+}
+
+int cjx_parse_batch(cjx_context *ctx, const uint8_t *data, const size_t *doc_off,
+                    const size_t *doc_size, size_t n_docs, unsigned flags,
+                    cjx_batch_result *out) {
+    if (!out) return CJX_ERR_INPUT;
+    std::memset(out, 0, sizeof(*out));
+    if (!ctx || !data || !doc_off || !doc_size || n_docs == 0) return out->error = CJX_ERR_INPUT;
+    std::vector<uint64_t> offs(n_docs), sizes(n_docs);
+    for (size_t i = 0; i < n_docs; i++) {
+        if (doc_size[i] == 0) return out->error = CJX_ERR_INPUT;  // PSJ:1597-1600 per document
+        offs[i] = doc_off[i];
+        sizes[i] = doc_size[i];
+    }
+    const int rc = parse_host_batch(ctx, data, offs, sizes, flags & ~CJX_FLAG_JSONL, out);
+    if (rc) out->error = rc;
+    return rc;
+}
+
+void cjx_free_batch_result(cjx_batch_result *r) {
+    if (!r) return;
+    if (r->structural) (void)hipHostFree(r->structural);
+    std::free(r->docs);
+    std::memset(r, 0, sizeof(*r));
+}
+
+int cjx_parse_batch_device(cjx_context *ctx, const uint8_t *d_input, const size_t *doc_off,
+                           const size_t *doc_size, size_t n_docs, unsigned flags,
+                           int32_t *d_structural, int32_t *d_pair, size_t cap) {
+    if (!ctx || !d_input || !doc_off || !doc_size || n_docs == 0 || !d_structural || !d_pair)
+        return CJX_ERR_INPUT;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    std::vector<UnitDesc> units(n_docs);
+    uint64_t tiles = 0, total = 0, full = 0;
+    for (size_t i = 0; i < n_docs; i++) {
+        if (doc_size[i] == 0 || (doc_off[i] & 15)) return CJX_ERR_INPUT;
+        UnitDesc &u = units[i];
+        u.in_off = doc_off[i];
+        u.n = doc_size[i];
+        u.pos_base = 0;
+        u.tile0 = (uint32_t)tiles;
+        u.ntiles = (uint32_t)((doc_size[i] + kTileBytes - 1) / kTileBytes);
+        tiles += u.ntiles;
+        total += doc_size[i];
+        full += doc_size[i];
+    }
+    if (tiles > 0xFFFFFFF0ull || total > kMaxLaunchBytes) return CJX_ERR_INPUT;
+    // Regions: each document's exact bound when the caller's rows hold them
+    // all, else the n/3 + 256 guess (an overflow reports CJX_ERR_CAPACITY).
+    const bool exact = full <= cap;
+    if (set_regions(units, exact) > cap) return CJX_ERR_CAPACITY;
+    Shape sh{(uint32_t)tiles, (uint32_t)n_docs, 0, 0, 0, 0, false};
+    shape_rows(sh, units);
+    if ((rc = ensure_work(ctx, sh))) return rc;
+    std::memcpy(ctx->h_units.p, units.data(), n_docs * sizeof(UnitDesc));
+    if (hipMemcpyAsync(ctx->units.p, ctx->h_units.p, n_docs * sizeof(UnitDesc),
+                       hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    ctx->n_units = (uint32_t)n_docs;
+    ctx->batch_off.assign(doc_off, doc_off + n_docs);
+    (void)hipEventRecord(ctx->ev0, ctx->stream);
+    rc = enqueue_maybe_graph(ctx, d_input, sh, flags & ~CJX_FLAG_JSONL,
+                       reinterpret_cast<uint32_t *>(d_structural), d_pair);
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
+    ctx->events_recorded = true;
+    return rc;
+}
+
+int cjx_batch_wait(cjx_context *ctx, cjx_doc_result *docs, size_t n_docs, cjx_device_status *st) {
+    if (!ctx || ctx->n_units == 0) return CJX_ERR_INPUT;
+    int rc = read_status(ctx, ctx->n_units);
+    if (rc) return rc;
+    const UnitOut *uo = static_cast<const UnitOut *>(ctx->h_uout.p);
+    const LaunchState *ls = static_cast<const LaunchState *>(ctx->h_ls.p);
+    int code = CJX_OK;
+    for (uint32_t i = 0; i < ctx->n_units; i++) {
+        const int c = unit_code(uo[i]);
+        if (c && !code) code = c;
+        if (docs && i < n_docs) {
+            docs[i].byte_base = i < ctx->batch_off.size() ? (int64_t)ctx->batch_off[i] : 0;
+            docs[i].struct_begin = uo[i].n_base;
+            docs[i].struct_end = uo[i].n_end;
+            docs[i].error = c;
+        }
+    }
+    if (ls->overflow) code = CJX_ERR_CAPACITY;
+    if (st) {
+        st->n_structural = ls->n_total;
+        st->n_brackets = ls->m_total;
+        st->error = code;
+        st->capacity_exceeded = ls->overflow ? 1 : 0;
+    }
+    return ls->overflow ? CJX_ERR_CAPACITY : CJX_OK;
 }
 
 }  // extern "C"

@@ -64,6 +64,8 @@ struct UnitOut {
     uint32_t nchunks;         // matching chunks (at least one when the unit has structurals)
     uint32_t grp0;            // first 64-chunk group (unit-local groups)
     uint32_t sgrp0;           // first 4096-chunk group
+    uint32_t row_base;        // structural count of the units before (packed rows)
+    uint32_t pad[3];
 };
 
 // Depth walk of a run of brackets: net change, and the minimum and maximum
@@ -127,9 +129,17 @@ hipError_t launch_utf8_strict(hipStream_t s, const LaunchBufs &b, const uint8_t 
 hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool concat);
 hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, uint32_t chunk_bound,
                         uint32_t group_bound, int32_t *pair_row);
+// Packs the units' row regions densely: structural row i of unit u goes to
+// dst_struct[row_base_u + i], its pair cell to dst_pair[row_base_u + i]; with
+// host_layout, dst_struct = dst + 1 and dst_pair = dst + N + 2 of one 2(N+2)
+// buffer whose sentinels are written too.
 hipError_t launch_assemble(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
-                           const uint32_t *d_struct, const int32_t *d_pair, int32_t *dst,
-                           uint32_t N);
+                           const uint32_t *d_struct, const int32_t *d_pair, int32_t *dst_struct,
+                           int32_t *dst_pair, uint32_t N, bool host_layout);
+// Sort-path phase 3 for every unit that is not well nested, one block per
+// unit (launches with several units).
+hipError_t launch_sort_units(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool depth_u16,
+                             int32_t *pair_row);
 hipError_t launch_phase3(hipStream_t s, const LaunchBufs &b, uint32_t u, uint32_t G,
                          uint32_t pair_grid, bool depth_u16, int32_t *pair_row);
 

@@ -68,6 +68,7 @@ __global__ __launch_bounds__(256) void k_unit_bases(UnitOut *__restrict__ uo, ui
             pre[k] += in[k] - v[k];
         }
         if (u < n_units) {
+            uo[u].row_base = pre[0];
             uo[u].pair_base = concat ? pre[0] : 0u;
             uo[u].chunk0 = pre[1];
             uo[u].nchunks = v[1];
@@ -81,44 +82,43 @@ __global__ __launch_bounds__(256) void k_unit_bases(UnitOut *__restrict__ uo, ui
     if (threadIdx.x == 0) ls->n_chunks = carry[1];
 }
 
-// Packs the per-unit output regions into the reference's host layout
-// (parse_json_lines.cu:1070-1089): dst[0] = 0, dst[1 + p_u + i] = structural
-// row i of unit u, dst[N + 1] = N + 1, dst[N + 2 + p_u + i] = its pair cell,
-// and the two trailing cells -1, where p_u is the unit's pair_base (the
-// structural count of the units before it).  Each thread moves kAsmRows
-// consecutive rows and finds their unit once by binary search.
+// Packs the per-unit output regions densely (see launch_assemble).  Each
+// thread moves kAsmRows consecutive rows and finds their unit once by binary
+// search over row_base.
 constexpr uint32_t kAsmRows = 16;
 __global__ __launch_bounds__(256) void k_assemble(const UnitOut *__restrict__ uo, uint32_t n_units,
                                                   const uint32_t *__restrict__ st,
                                                   const int32_t *__restrict__ pr,
-                                                  int32_t *__restrict__ dst, uint32_t N) {
+                                                  int32_t *__restrict__ dst_struct,
+                                                  int32_t *__restrict__ dst_pair, uint32_t N,
+                                                  int32_t *__restrict__ sentinels) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t r0 = t * kAsmRows;
-    if (t == 0) {
-        dst[0] = 0;
-        dst[N + 1] = (int32_t)(N + 1);
-        dst[2 * (uint64_t)N + 2] = -1;
-        dst[2 * (uint64_t)N + 3] = -1;
+    if (t == 0 && sentinels) {  // host layout, parse_json_lines.cu:1070-1089
+        sentinels[0] = 0;
+        sentinels[N + 1] = (int32_t)(N + 1);
+        sentinels[2 * (uint64_t)N + 2] = -1;
+        sentinels[2 * (uint64_t)N + 3] = -1;
     }
     if (r0 >= N) return;
-    uint32_t lo = 0, hi = n_units;  // last unit with pair_base <= r0
+    uint32_t lo = 0, hi = n_units;  // last unit with row_base <= r0
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (uo[mid].pair_base <= r0) lo = mid;
+        if (uo[mid].row_base <= r0) lo = mid;
         else hi = mid;
     }
     uint32_t u = lo;
-    uint32_t ub = uo[u].pair_base, ue = ub + (uo[u].n_end - uo[u].n_base);
+    uint32_t ub = uo[u].row_base, ue = ub + (uo[u].n_end - uo[u].n_base);
     const uint64_t r1 = r0 + kAsmRows < N ? r0 + kAsmRows : N;
     for (uint64_t r = r0; r < r1; r++) {
         while (r >= ue) {  // skips units without structurals
             u++;
-            ub = uo[u].pair_base;
+            ub = uo[u].row_base;
             ue = ub + (uo[u].n_end - uo[u].n_base);
         }
         const uint64_t src = uo[u].n_base + (r - ub);
-        dst[1 + r] = (int32_t)st[src];
-        dst[(uint64_t)N + 2 + r] = pr[src];
+        dst_struct[r] = (int32_t)st[src];
+        dst_pair[r] = pr[src];
     }
 }
 
@@ -376,6 +376,166 @@ __global__ __launch_bounds__(kP3Threads) void k_pairs(
 // ---------------------------------------------------------------------------
 namespace cjx {
 
+// Sort path of one unit in one block, for launches with several units (a
+// batch of documents, JSON Lines chunks): the steps of k_depth_totals /
+// k_depth_keys / k_scan_hist / k_scatter / k_pairs with the block-level
+// ranges replaced by one sequential pass per step.  Units that are well
+// nested (paired by cjx_match.hip) are skipped.
+template <bool U16>
+__global__ __launch_bounds__(kP3Threads) void k_sort_unit(
+    const uint8_t *__restrict__ oc_char, const uint32_t *__restrict__ oc_idx,
+    uint16_t *__restrict__ keys, uint32_t *__restrict__ sorted, uint32_t *__restrict__ tmp,
+    uint32_t *__restrict__ spos, UnitOut *__restrict__ uo, int32_t *__restrict__ pair_row,
+    const LaunchState *__restrict__ ls) {
+    constexpr int kW = kP3Threads / 64;
+    __shared__ uint32_t h[2][256];
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[kW][256];
+    __shared__ int32_t wsum[kW];
+    __shared__ int32_t row[kPairChunk];
+    __shared__ uint32_t s_m0, s_m1;
+    if (ls->overflow) return;
+    const uint32_t u = blockIdx.x;
+    if (uo[u].fast) return;
+    const uint32_t tid = threadIdx.x, w = tid >> 6;
+    const uint64_t lt = lanemask_lt();
+    const uint32_t kmask = U16 ? 0xFFFFu : 0xFFu;
+    const uint32_t mb = uo[u].m_base, me = uo[u].m_end, M = me - mb;
+    const uint32_t nb = uo[u].n_base, ne = uo[u].n_end;
+    const uint32_t pbase = uo[u].pair_base + 1u;
+    h[0][tid] = 0;
+    h[1][tid] = 0;
+    __syncthreads();
+
+    // Depth keys (PSJ:1345-1358, :1537, :1540) and digit histograms.
+    int32_t carry = 0;
+    for (uint32_t base = mb; base < me; base += kP3Threads) {
+        const uint32_t i = base + tid;
+        const bool valid = i < me;
+        const int32_t d = valid ? bracket_delta(oc_char[i]) : 0;
+        const int32_t incl = wave_incl_scan_i(d);
+        if (lane_id() == 63) wsum[w] = incl;
+        __syncthreads();
+        int32_t woff = 0, tot = 0;
+        for (int k = 0; k < kW; k++) {
+            if (k < (int)w) woff += wsum[k];
+            tot += wsum[k];
+        }
+        if (valid) {
+            const uint32_t depth = (uint32_t)(carry + woff + incl) & kmask;
+            const uint32_t key = (d > 0) ? ((depth - 1u) & kmask) : depth;
+            keys[i] = (uint16_t)key;
+            atomicAdd(&h[0][key & 0xFFu], 1u);
+            if (U16) atomicAdd(&h[1][key >> 8], 1u);
+        }
+        carry += tot;
+        __syncthreads();
+    }
+
+    // Stable counting sort (PSJ:1546): low key byte, then (16-bit keys) the
+    // high byte over the first pass's order.
+    constexpr int kPasses = U16 ? 2 : 1;
+    for (int pass = 0; pass < kPasses; pass++) {
+        {
+            const uint32_t v = h[pass][tid];
+            const uint32_t in = wave_incl_scan(v);
+            if (lane_id() == 63) wsum[w] = (int32_t)in;
+            __syncthreads();
+            uint32_t ex = in - v;
+            for (uint32_t k = 0; k < w; k++) ex += (uint32_t)wsum[k];
+            run[tid] = ex;
+            __syncthreads();
+        }
+        const uint32_t shift = pass ? 8u : 0u;
+        uint32_t *perm_out = (U16 && pass == 0) ? tmp : sorted;
+        const bool final_pass = pass == kPasses - 1;
+        for (uint32_t base = mb; base < me; base += kP3Threads) {
+            const uint32_t e = base + tid;
+            const bool valid = e < me;
+            uint32_t m = 0, digit = 0;
+            if (valid) {
+                m = pass ? tmp[e] : e;
+                digit = (keys[m] >> shift) & 0xFFu;
+            }
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < 8; bit++) {
+                const uint32_t bv = (digit >> bit) & 1u;
+                const uint64_t bm = __ballot(bv);
+                peers &= bv ? bm : ~bm;
+            }
+#pragma unroll
+            for (int k = 0; k < kW; k++) wcnt[k][tid] = 0;
+            __syncthreads();
+            const uint32_t rank = (uint32_t)__popcll(peers & lt);
+            if (valid && rank == 0) wcnt[w][digit] = (uint32_t)__popcll(peers);
+            __syncthreads();
+            if (valid) {
+                uint32_t pos = run[digit] + rank;
+                for (uint32_t k = 0; k < w; k++) pos += wcnt[k][digit];
+                perm_out[mb + pos] = m;
+                if (final_pass) spos[m] = pos;
+            }
+            __syncthreads();
+            uint32_t add = 0;
+#pragma unroll
+            for (int k = 0; k < kW; k++) add += wcnt[k][tid];
+            run[tid] += add;
+            __syncthreads();
+        }
+    }
+
+    // Pairing (validate_expand_MathAPI_new2, PSJ:1431-1514) into dense
+    // 4096-row tiles of the unit's pair row, as in k_pairs.
+    const uint32_t last_group = M ? 4u * ((M + 3u) / 4u - 1u) : 0u;
+    int32_t err = (tid == 0 && (M & 1u)) ? 1 : 0;
+    for (uint32_t t0 = nb; t0 < ne; t0 += kPairChunk) {
+        const uint32_t t1 = (t0 + kPairChunk < ne) ? t0 + kPairChunk : ne;
+        for (uint32_t i = tid; i < kPairChunk; i += kP3Threads) row[i] = -1;
+        if (tid < 2) {  // lower_bound of t0 (tid 0) and t1 (tid 1) in oc_idx[mb, me)
+            const uint32_t key = tid ? t1 : t0;
+            uint32_t lo = mb, hi = me;
+            while (lo < hi) {
+                const uint32_t mid = lo + ((hi - lo) >> 1);
+                if (oc_idx[mid] < key) lo = mid + 1;
+                else hi = mid;
+            }
+            if (tid) s_m1 = lo;
+            else s_m0 = lo;
+        }
+        __syncthreads();
+        for (uint32_t m = s_m0 + tid; m < s_m1; m += kP3Threads) {
+            const uint32_t sp = spos[m];
+            if (sp & 1u) continue;
+            if (sp + 1u >= M) continue;
+            if ((M & 3u) == 3u && sp == M - 3u) continue;
+            const uint32_t m2 = sorted[mb + sp + 1u];
+            const int c0 = (int)(signed char)oc_char[m];
+            const int c1 = (int)(signed char)oc_char[m2];
+            if ((c0 ^ c1) != 6) err = 1;
+            if (sp < last_group && c0 > c1) err = 1;
+            row[oc_idx[m] - t0] = (int32_t)(oc_idx[m2] - nb + pbase);
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < t1 - t0; i += kP3Threads) pair_row[t0 + i] = row[i];
+        __syncthreads();
+    }
+    err = block_reduce_sum(err, wsum);
+    if (tid == 0 && err) atomicOr(&uo[u].struct_err, 1u);
+}
+
+hipError_t launch_sort_units(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool depth_u16,
+                             int32_t *pair_row) {
+    if (n_units == 0) return hipSuccess;
+    if (depth_u16)
+        hipLaunchKernelGGL(k_sort_unit<true>, dim3(n_units), dim3(kP3Threads), 0, s, b.oc_char,
+                           b.oc_idx, b.keys, b.sorted, b.tmp, b.spos, b.uout, pair_row, b.ls);
+    else
+        hipLaunchKernelGGL(k_sort_unit<false>, dim3(n_units), dim3(kP3Threads), 0, s, b.oc_char,
+                           b.oc_idx, b.keys, b.sorted, b.tmp, b.spos, b.uout, pair_row, b.ls);
+    return hipGetLastError();
+}
+
 hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uint32_t n_units) {
     const uint32_t work = 2 * ntiles > n_units ? 2 * ntiles : n_units;
     uint32_t grid = (work + 255) / 256;
@@ -387,12 +547,18 @@ hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uin
 }
 
 hipError_t launch_assemble(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
-                           const uint32_t *d_struct, const int32_t *d_pair, int32_t *dst,
-                           uint32_t N) {
+                           const uint32_t *d_struct, const int32_t *d_pair, int32_t *dst_struct,
+                           int32_t *dst_pair, uint32_t N, bool host_layout) {
     const uint64_t threads = ((uint64_t)N + kAsmRows - 1) / kAsmRows;
     const uint32_t grid = (uint32_t)((threads + 255) / 256) + 1;
+    int32_t *sent = nullptr;
+    if (host_layout) {
+        sent = dst_struct;
+        dst_pair = dst_struct + N + 2;
+        dst_struct = dst_struct + 1;
+    }
     hipLaunchKernelGGL(k_assemble, dim3(grid), dim3(256), 0, s, b.uout, n_units, d_struct, d_pair,
-                       dst, N);
+                       dst_struct, dst_pair, N, sent);
     return hipGetLastError();
 }
 

@@ -110,3 +110,60 @@ def plant_fault(doc: np.ndarray, kind: int, seed: int) -> bool:
     """Plant one fault in place (0 overlong, 1 surrogate, 2 truncated UTF-8,
     3 unbalanced bracket, 4 unterminated string)."""
     return bool(lib().cjx_plant_fault(seed, kind, doc.ctypes.data, doc.size))
+
+
+def many_docs(n_docs=50_000, alpha=1.1, min_size=200, max_size=16 << 20, fault_frac=0.10,
+              deep_frac=0.05, max_depth=512, align=16, seed=SEED_BASE + 4):
+    """Batch of independent standard-JSON documents (config 4 shape).
+
+    Sizes are Pareto(alpha) from min_size, clipped to max_size.  A deep_frac
+    share of documents is wrapped in alternating arrays/objects of depth
+    uniform in [65, max_depth]; a fault_frac share carries exactly one fault
+    (plant_fault kinds 0-4 in turn).  Documents start at multiples of `align`
+    in the returned buffer (gaps are zero bytes).
+
+    Returns (data, offsets, sizes, kinds) with kinds[i] = fault kind or -1.
+    """
+    rng = np.random.default_rng(seed)
+    u = rng.random(n_docs)
+    sizes_t = np.minimum(min_size / np.power(1.0 - u, 1.0 / alpha), max_size).astype(np.int64)
+    deep = rng.random(n_docs) < deep_frac
+    depths = rng.integers(65, max_depth + 1, n_docs)
+    faulty = rng.random(n_docs) < fault_frac
+    seeds = rng.integers(0, 2 ** 62, n_docs)
+    L = lib()
+    docs, kinds = [], []
+    scratch = np.empty(max_size + (1 << 20), dtype=np.uint8)
+    k_fault = 0
+    for i in range(n_docs):
+        t = int(sizes_t[i])
+        if t < 8192:
+            n = L.cjx_gen_record(int(seeds[i]), t, int(seeds[i] % 6), scratch.ctypes.data,
+                                 scratch.size)
+            d = scratch[:n - 1].copy()  # drop the record's '\n'
+        else:
+            n = L.cjx_gen_standard(int(seeds[i]), 0, t, 0, 0, 0, scratch.ctypes.data, scratch.size)
+            d = scratch[:n].copy()
+        if deep[i]:
+            k = int(depths[i])
+            pre = b"".join(b"[" if j % 2 == 0 else b'{"d":' for j in range(k))
+            post = b"".join(b"]" if j % 2 == 0 else b"}" for j in reversed(range(k)))
+            d = np.concatenate([np.frombuffer(pre, np.uint8), d, np.frombuffer(post, np.uint8)])
+        kind = -1
+        if faulty[i]:
+            kind = k_fault % 5
+            k_fault += 1
+            if not plant_fault(d, kind, int(seeds[i]) ^ 0x5A5A):
+                kind = -1
+        docs.append(d)
+        kinds.append(kind)
+    sizes = np.array([d.size for d in docs], dtype=np.int64)
+    offs = np.zeros(n_docs, dtype=np.int64)
+    pos = 0
+    for i in range(n_docs):
+        offs[i] = pos
+        pos += -(-int(sizes[i]) // align) * align
+    data = np.zeros(pos + 64, dtype=np.uint8)
+    for i in range(n_docs):
+        data[offs[i]:offs[i] + sizes[i]] = docs[i]
+    return data, offs, sizes, np.array(kinds, dtype=np.int64)

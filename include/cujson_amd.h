@@ -49,6 +49,9 @@ enum {
 #define CJX_FLAG_UTF8_STRICT 0x1u /* RFC 3629 validation (off: the reference's check)  */
 #define CJX_FLAG_DEPTH_U16   0x2u /* 16-bit depth keys (off: the reference's uint8)     */
 #define CJX_FLAG_JSONL       0x4u /* '\n' is structural (set by cjx_parse_lines)        */
+#define CJX_FLAG_GRAPH       0x8u /* device-resident calls: replay the launch sequence
+                                     as a HIP graph captured on the first call of the
+                                     same shape (no counterpart in the reference)      */
 
 typedef struct cjx_context cjx_context;
 
@@ -162,6 +165,58 @@ void *cjx_context_stream(cjx_context *ctx);
 
 /* Library build identification. */
 const char *cjx_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Batches of independent standard-JSON documents.  The reference parses  */
+/* one document per parse_standard_json call (main.cu:26); a batch runs   */
+/* all of them in one launch sequence.  Each document is its own parse    */
+/* unit: positions are 1-based within the document and pair values are    */
+/* host indices of the document's own layout, so rows [struct_begin,      */
+/* struct_end) with the sentinels 0 / N_d+1 around them form exactly the  */
+/* cuJSONResult that parse_standard_json returns for that document.       */
+/* ---------------------------------------------------------------------- */
+
+typedef struct cjx_doc_result {
+    int64_t byte_base;    /* offset of the document in the batch input          */
+    int64_t struct_begin; /* first row of the document                          */
+    int64_t struct_end;   /* one past its last row                              */
+    int32_t error;        /* CJX_OK, CJX_ERR_UTF8 or CJX_ERR_STRUCTURE           */
+    int32_t reserved;
+} cjx_doc_result;
+
+typedef struct cjx_batch_result {
+    int32_t *structural;  /* pinned host: the rows of all documents, packed     */
+    int32_t *pair_pos;    /* pinned host: the pair cells of the same rows       */
+    int64_t n_structural; /* total rows                                         */
+    int64_t n_docs;
+    cjx_doc_result *docs; /* n_docs entries                                     */
+    int32_t error;        /* first document error, or the call's failure code   */
+    int32_t reserved;
+    double device_ms;
+} cjx_batch_result;
+
+/* Host documents (data + doc_off[i], doc_size[i] bytes each).  Per-document
+ * codes are in out->docs; the return value is CJX_OK unless the call as a
+ * whole failed (bad arguments, allocation, device).  Release with
+ * cjx_free_batch_result.  Replaces a loop of parse_standard_json calls
+ * (cujson/parse_standard_json.cu:1584-1687) over the documents. */
+int cjx_parse_batch(cjx_context *ctx, const uint8_t *data, const size_t *doc_off,
+                    const size_t *doc_size, size_t n_docs, unsigned flags,
+                    cjx_batch_result *out);
+void cjx_free_batch_result(cjx_batch_result *r);
+
+/* Device-resident batch (no counterpart in the reference): documents at
+ * 16-byte aligned offsets of d_input; rows stay in d_structural / d_pair
+ * (cap rows each), document i in its own region.  Returns after enqueueing;
+ * cjx_batch_wait synchronizes and fills docs[0..n_docs) (struct_begin /
+ * struct_end index the caller's row buffers).  If the documents' bytes sum
+ * to at most cap, regions are exact and never overflow; otherwise they are
+ * sized n/3 + 256 and an overflow reports CJX_ERR_CAPACITY. */
+int cjx_parse_batch_device(cjx_context *ctx, const uint8_t *d_input, const size_t *doc_off,
+                           const size_t *doc_size, size_t n_docs, unsigned flags,
+                           int32_t *d_structural, int32_t *d_pair, size_t cap);
+int cjx_batch_wait(cjx_context *ctx, cjx_doc_result *docs, size_t n_docs,
+                   cjx_device_status *st);
 
 #ifdef __cplusplus
 }
