@@ -83,7 +83,16 @@ struct cjx_context {
     uint64_t timing_count = 0;
     bool timing = false;
     DevBuf in, structural, pair;
-    DevBuf units, uout, ls, tile_st, layout;
+    DevBuf units, uout, ls, tile_st, layout, chunk_unit, big_units, unit_part;
+    uint32_t *tile_unit = nullptr;    // inside `units`, after the unit descriptors
+    uint32_t *small_units = nullptr;  // after the tile -> unit map
+    // Pinned upload slots for unit tables: a call may return before its copy
+    // has run, so the next calls use other slots; a slot is reused only after
+    // the event recorded behind its copy has completed.
+    static constexpr int kUploadSlots = 8;
+    PinnedBuf up[kUploadSlots];
+    hipEvent_t up_ev[kUploadSlots] = {};
+    uint64_t up_count = 0;
     DevBuf oc_char, oc_idx, keys, sorted, tmp, spos, block_delta, hist;
     DevBuf ck, gmin, gmin1, gmin2, gagg, gd0, lows, highs;
     PinnedBuf h_units, h_uout, h_ls, h_stage;
@@ -98,7 +107,7 @@ struct cjx_context {
         const uint8_t *d_in;
         uint32_t *d_struct;
         int32_t *d_pair;
-        uint32_t ntiles, n_units;
+        uint32_t ntiles, n_units, n_small;
         uint64_t rows, max_rows, chunks;
         unsigned flags;
         bool concat;
@@ -121,26 +130,44 @@ uint32_t pair_blocks(uint64_t max_n) {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 2048));
 }
 
+// Tokenizer work of the units: a unit of at most one wave's bytes is small
+// (one wave, no tiles); the others are cut into tiles.  Returns the tile
+// count and fills n_small.
+uint64_t assign_tiles(std::vector<UnitDesc> &units, uint32_t &n_small) {
+    uint64_t tiles = 0;
+    n_small = 0;
+    for (UnitDesc &u : units) {
+        u.tile0 = (uint32_t)tiles;
+        if (u.n <= (uint64_t)kWaveBytes) {
+            u.ntiles = 0;
+            n_small++;
+        } else {
+            u.ntiles = (uint32_t)((u.n + kTileBytes - 1) / kTileBytes);
+            tiles += u.ntiles;
+        }
+    }
+    return tiles;
+}
+
 // Units are given as (host offset, size); they are placed in the device
-// input at 16-byte aligned offsets.  Fills descriptors and returns the tile
-// count, or 0 on invalid shape.  Output regions are set by set_regions.
-uint32_t plan_units(const std::vector<uint64_t> &sizes, const std::vector<uint64_t> &pos_base,
-                    std::vector<UnitDesc> &units, uint64_t &dev_bytes) {
+// input at 16-byte aligned offsets.  Fills descriptors; returns false on an
+// invalid shape.  Output regions are set by set_regions.
+bool plan_units(const std::vector<uint64_t> &sizes, const std::vector<uint64_t> &pos_base,
+                std::vector<UnitDesc> &units, uint64_t &dev_bytes, uint32_t &ntiles,
+                uint32_t &n_small) {
     units.resize(sizes.size());
-    uint64_t off = 0, tiles = 0;
+    uint64_t off = 0;
     for (size_t i = 0; i < sizes.size(); i++) {
         UnitDesc &u = units[i];
         u.in_off = off;
         u.n = sizes[i];
         u.pos_base = pos_base[i];
-        u.tile0 = (uint32_t)tiles;
-        u.ntiles = (uint32_t)((sizes[i] + kTileBytes - 1) / kTileBytes);
-        tiles += u.ntiles;
         off += (sizes[i] + 15) & ~15ull;
     }
     dev_bytes = off + 64;
-    if (tiles > 0xFFFFFFF0ull) return 0;
-    return (uint32_t)tiles;
+    const uint64_t tiles = assign_tiles(units, n_small);
+    ntiles = (uint32_t)tiles;
+    return tiles <= 0xFFFFFFF0ull && !units.empty();
 }
 
 // Output regions: unit u owns rows [out_off, out_off + out_cap) of every
@@ -160,7 +187,7 @@ uint64_t set_regions(std::vector<UnitDesc> &units, bool full) {
 }
 
 struct Shape {
-    uint32_t ntiles, n_units;
+    uint32_t ntiles, n_units, n_small;
     uint64_t rows;      // total rows of all output regions
     uint64_t max_rows;  // largest region
     uint64_t chunks;    // bound on the matching chunks of all units
@@ -194,15 +221,50 @@ int ensure_work(cjx_context *ctx, const Shape &sh) {
               ctx->gd0.ensure(sh.groups * 4 + 16) &&
               ctx->lows.ensure(sh.chunks * kMatchKeys * 4 + 16) &&
               ctx->highs.ensure(sh.chunks * kMatchKeys * 4 + 16) &&
+              ctx->chunk_unit.ensure(sh.chunks * 4 + 16) &&
+              ctx->big_units.ensure((size_t)sh.n_units * 4 + 16) &&
+              ctx->unit_part.ensure(((size_t)sh.n_units / 256 + 2) * 32) &&
               ctx->h_units.ensure(sh.n_units * sizeof(UnitDesc)) &&
               ctx->h_uout.ensure(sh.n_units * sizeof(UnitOut)) &&
               ctx->h_ls.ensure(sizeof(LaunchState));
     return ok ? CJX_OK : CJX_ERR_ALLOC;
 }
 
+// Uploads the unit descriptors and the tile -> unit map (one copy through a
+// pinned upload slot) into ctx->units.
+int upload_units(cjx_context *ctx, const std::vector<UnitDesc> &units, uint32_t ntiles,
+                 uint32_t n_small) {
+    const size_t nb = units.size() * sizeof(UnitDesc), tb = ((size_t)ntiles + n_small) * 4;
+    if (!ctx->units.ensure(nb + tb + 16)) return CJX_ERR_ALLOC;
+    const int slot = (int)(ctx->up_count++ % cjx_context::kUploadSlots);
+    if (!ctx->up_ev[slot] && hipEventCreate(&ctx->up_ev[slot]) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    if (hipEventSynchronize(ctx->up_ev[slot]) != hipSuccess) return CJX_ERR_DEVICE;
+    if (!ctx->up[slot].ensure(nb + tb + 16)) return CJX_ERR_ALLOC;
+    uint8_t *h = static_cast<uint8_t *>(ctx->up[slot].p);
+    std::memcpy(h, units.data(), nb);
+    uint32_t *tu = reinterpret_cast<uint32_t *>(h + nb);
+    uint32_t *su = tu + ntiles;  // then the small units, in unit order
+    for (size_t u = 0; u < units.size(); u++) {
+        for (uint32_t t = 0; t < units[u].ntiles; t++) tu[units[u].tile0 + t] = (uint32_t)u;
+        if (units[u].ntiles == 0) *su++ = (uint32_t)u;
+    }
+    if (hipMemcpyAsync(ctx->units.p, h, nb + tb, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    if (hipEventRecord(ctx->up_ev[slot], ctx->stream) != hipSuccess) return CJX_ERR_DEVICE;
+    ctx->tile_unit = reinterpret_cast<uint32_t *>(ctx->units.as<uint8_t>() + nb);
+    ctx->small_units = ctx->tile_unit + ntiles;
+    return CJX_OK;
+}
+
 LaunchBufs bufs_of(cjx_context *ctx) {
     LaunchBufs b;
     b.units = ctx->units.as<UnitDesc>();
+    b.tile_unit = ctx->tile_unit;
+    b.small_units = ctx->small_units;
+    b.chunk_unit = ctx->chunk_unit.as<uint32_t>();
+    b.big_units = ctx->big_units.as<uint32_t>();
+    b.unit_part = ctx->unit_part.as<uint32_t>();
     b.uout = ctx->uout.as<UnitOut>();
     b.ls = ctx->ls.as<LaunchState>();
     b.tile_st = ctx->tile_st.as<unsigned long long>();
@@ -237,9 +299,9 @@ int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsign
     hipEvent_t *tev = ctx->tev[ctx->timing_count % cjx_context::kTimingRing];
     if (ctx->timing) (void)hipEventRecord(tev[0], s);
     const bool strict = (flags & CJX_FLAG_UTF8_STRICT) != 0;
-    if (strict && launch_utf8_strict(s, b, d_in, sh.ntiles, sh.n_units) != hipSuccess)
+    if (strict && launch_utf8_strict(s, b, d_in, sh.ntiles, sh.n_small) != hipSuccess)
         return CJX_ERR_DEVICE;
-    if (launch_tokenize(s, b, d_in, sh.ntiles, sh.n_units, (flags & CJX_FLAG_JSONL) != 0,
+    if (launch_tokenize(s, b, d_in, sh.ntiles, sh.n_small, (flags & CJX_FLAG_JSONL) != 0,
                         !strict, d_struct) != hipSuccess)
         return CJX_ERR_DEVICE;
     if (ctx->timing) (void)hipEventRecord(tev[1], s);
@@ -278,6 +340,7 @@ int enqueue_maybe_graph(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, 
     key.d_pair = d_pair;
     key.ntiles = sh.ntiles;
     key.n_units = sh.n_units;
+    key.n_small = sh.n_small;
     key.rows = sh.rows;
     key.max_rows = sh.max_rows;
     key.chunks = sh.chunks;
@@ -356,10 +419,11 @@ int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint
     int rc = set_device(ctx);
     if (rc) return rc;
     uint64_t dev_bytes = 0;
-    const uint32_t ntiles = plan_units(sizes, pos_base, units, dev_bytes);
+    uint32_t ntiles = 0, n_small = 0;
+    if (!plan_units(sizes, pos_base, units, dev_bytes, ntiles, n_small)) return CJX_ERR_INPUT;
     uint64_t total = 0;
     for (uint64_t z : sizes) total += z;
-    if (ntiles == 0 || total > kMaxLaunchBytes) return CJX_ERR_INPUT;
+    if (total == 0 || total > kMaxLaunchBytes) return CJX_ERR_INPUT;
     if (!ctx->in.ensure(dev_bytes)) return CJX_ERR_ALLOC;
     hipStream_t s = ctx->stream;
     if (units.size() > 16) {  // one pinned staging copy and one H2D instead of one per unit
@@ -377,17 +441,14 @@ int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint
                 return CJX_ERR_DEVICE;
     }
 
-    Shape sh{ntiles, (uint32_t)units.size(), 0, 0, 0, 0, concat};
+    Shape sh{ntiles, (uint32_t)units.size(), n_small, 0, 0, 0, 0, concat};
     for (int attempt = 0; attempt < 2; attempt++) {
         set_regions(units, attempt == 1);
         shape_rows(sh, units);
         if ((rc = ensure_work(ctx, sh))) return rc;
         if (!ctx->structural.ensure(sh.rows * 4 + 16) || !ctx->pair.ensure(sh.rows * 4 + 16))
             return CJX_ERR_ALLOC;
-        std::memcpy(ctx->h_units.p, units.data(), units.size() * sizeof(UnitDesc));
-        if (hipMemcpyAsync(ctx->units.p, ctx->h_units.p, units.size() * sizeof(UnitDesc),
-                           hipMemcpyHostToDevice, s) != hipSuccess)
-            return CJX_ERR_DEVICE;
+        if ((rc = upload_units(ctx, units, ntiles, n_small))) return rc;
         (void)hipEventRecord(ctx->ev0, s);
         rc = enqueue_parse(ctx, ctx->in.as<uint8_t>(), sh, flags, ctx->structural.as<uint32_t>(),
                            ctx->pair.as<int32_t>());
@@ -567,12 +628,20 @@ void cjx_context_destroy(cjx_context *ctx) {
                       &ctx->oc_char, &ctx->oc_idx,     &ctx->keys,   &ctx->sorted,
                       &ctx->tmp,     &ctx->spos,       &ctx->block_delta, &ctx->hist,
                       &ctx->ck,      &ctx->gmin,       &ctx->gmin1,  &ctx->gmin2,
-                      &ctx->gagg,    &ctx->gd0,        &ctx->lows,   &ctx->highs};
+                      &ctx->gagg,    &ctx->gd0,        &ctx->lows,   &ctx->highs,
+                      &ctx->chunk_unit, &ctx->big_units, &ctx->unit_part};
     for (DevBuf *b : bufs) b->release();
     ctx->h_units.release();
     ctx->h_uout.release();
     ctx->h_ls.release();
     ctx->h_stage.release();
+    for (int i = 0; i < cjx_context::kUploadSlots; i++) {
+        if (ctx->up_ev[i]) {
+            (void)hipEventSynchronize(ctx->up_ev[i]);
+            (void)hipEventDestroy(ctx->up_ev[i]);
+        }
+        ctx->up[i].release();
+    }
     for (auto &tr : ctx->tev)
         for (hipEvent_t e : tr)
             if (e) (void)hipEventDestroy(e);
@@ -752,16 +821,14 @@ int cjx_parse_device(cjx_context *ctx, const uint8_t *d_input, size_t n, unsigne
     if (rc) return rc;
     std::vector<UnitDesc> units;
     uint64_t dev_bytes = 0;
-    const uint32_t ntiles = plan_units({n}, {0}, units, dev_bytes);
+    uint32_t ntiles = 0, n_small = 0;
+    if (!plan_units({n}, {0}, units, dev_bytes, ntiles, n_small)) return CJX_ERR_INPUT;
     units[0].out_off = 0;
     units[0].out_cap = (uint32_t)std::min<uint64_t>(cap, n);
-    Shape sh{ntiles, 1, 0, 0, 0, 0, false};
+    Shape sh{ntiles, 1, n_small, 0, 0, 0, 0, false};
     shape_rows(sh, units);
     if ((rc = ensure_work(ctx, sh))) return rc;
-    std::memcpy(ctx->h_units.p, units.data(), sizeof(UnitDesc));
-    if (hipMemcpyAsync(ctx->units.p, ctx->h_units.p, sizeof(UnitDesc), hipMemcpyHostToDevice,
-                       ctx->stream) != hipSuccess)
-        return CJX_ERR_DEVICE;
+    if ((rc = upload_units(ctx, units, ntiles, n_small))) return rc;
     ctx->n_units = 1;
     (void)hipEventRecord(ctx->ev0, ctx->stream);
     rc = enqueue_maybe_graph(ctx, d_input, sh, flags & ~CJX_FLAG_JSONL,
@@ -820,31 +887,27 @@ int cjx_parse_batch_device(cjx_context *ctx, const uint8_t *d_input, const size_
     int rc = set_device(ctx);
     if (rc) return rc;
     std::vector<UnitDesc> units(n_docs);
-    uint64_t tiles = 0, total = 0, full = 0;
+    uint64_t total = 0, full = 0;
     for (size_t i = 0; i < n_docs; i++) {
         if (doc_size[i] == 0 || (doc_off[i] & 15)) return CJX_ERR_INPUT;
         UnitDesc &u = units[i];
         u.in_off = doc_off[i];
         u.n = doc_size[i];
         u.pos_base = 0;
-        u.tile0 = (uint32_t)tiles;
-        u.ntiles = (uint32_t)((doc_size[i] + kTileBytes - 1) / kTileBytes);
-        tiles += u.ntiles;
         total += doc_size[i];
         full += doc_size[i];
     }
+    uint32_t n_small = 0;
+    const uint64_t tiles = assign_tiles(units, n_small);
     if (tiles > 0xFFFFFFF0ull || total > kMaxLaunchBytes) return CJX_ERR_INPUT;
     // Regions: each document's exact bound when the caller's rows hold them
     // all, else the n/3 + 256 guess (an overflow reports CJX_ERR_CAPACITY).
     const bool exact = full <= cap;
     if (set_regions(units, exact) > cap) return CJX_ERR_CAPACITY;
-    Shape sh{(uint32_t)tiles, (uint32_t)n_docs, 0, 0, 0, 0, false};
+    Shape sh{(uint32_t)tiles, (uint32_t)n_docs, n_small, 0, 0, 0, 0, false};
     shape_rows(sh, units);
     if ((rc = ensure_work(ctx, sh))) return rc;
-    std::memcpy(ctx->h_units.p, units.data(), n_docs * sizeof(UnitDesc));
-    if (hipMemcpyAsync(ctx->units.p, ctx->h_units.p, n_docs * sizeof(UnitDesc),
-                       hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
-        return CJX_ERR_DEVICE;
+    if ((rc = upload_units(ctx, units, (uint32_t)tiles, n_small))) return rc;
     ctx->n_units = (uint32_t)n_docs;
     ctx->batch_off.assign(doc_off, doc_off + n_docs);
     (void)hipEventRecord(ctx->ev0, ctx->stream);

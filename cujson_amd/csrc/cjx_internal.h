@@ -36,6 +36,7 @@ constexpr int kMatchChunk = 4096;
 constexpr int kMatchThreads = 256;
 constexpr int kMatchKeys = 256;
 constexpr int kMatchMaxDepth = 256;  // deeper units take the sort path
+constexpr int kSmallBrackets = 64;   // units with at most this many brackets: k_match_small
 
 // One parse unit inside a launch: a standard document, a JSON Lines chunk or
 // one document of a batch.  Escape/string state, depth and counts restart at
@@ -86,7 +87,8 @@ struct LaunchState {
     uint32_t overflow;      // an output exceeded its capacity
     uint32_t n_total, m_total;
     uint32_t n_chunks;  // matching chunks of all units
-    uint32_t pad[3];
+    uint32_t n_big;     // units with more than 64 matching chunks (listed in big_units)
+    uint32_t pad[2];
 };
 
 // Look-back status encoding (two 64-bit words per tile, see cjx_tokenize.hip).
@@ -99,6 +101,11 @@ namespace cjx {
 // Device buffers one launch uses (owned by the context).
 struct LaunchBufs {
     UnitDesc *units;
+    uint32_t *tile_unit;   // unit of every tokenizer tile (host-built)
+    uint32_t *small_units; // units of at most kWaveBytes bytes (host-built)
+    uint32_t *chunk_unit;  // unit of every matching chunk (device-built)
+    uint32_t *big_units;   // units with more than 64 matching chunks
+    uint32_t *unit_part;   // per 256 units: 4 block sums of the unit-base scan
     UnitOut *uout;
     LaunchState *ls;
     unsigned long long *tile_st;  // 2 words per tile
@@ -121,11 +128,13 @@ struct LaunchBufs {
 };
 
 hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uint32_t n_units);
+// Units of at most kWaveBytes bytes are "small": no tiles, one wave each
+// (listed in small_units); the others are cut into kTileBytes tiles.
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                           uint32_t ntiles, uint32_t n_units, bool jsonl, bool weak_utf8,
+                           uint32_t ntiles, uint32_t n_small, bool jsonl, bool weak_utf8,
                            uint32_t *out_struct);
 hipError_t launch_utf8_strict(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                              uint32_t ntiles, uint32_t n_units);
+                              uint32_t ntiles, uint32_t n_small);
 hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool concat);
 hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, uint32_t chunk_bound,
                         uint32_t group_bound, int32_t *pair_row);

@@ -26,60 +26,131 @@ __global__ void k_reset(unsigned long long *tile_st, uint32_t ntiles, LaunchStat
     if (i0 == 0) *ls = LaunchState{};
 }
 
-// Per-unit bases, one block: pair_base offsets unit u's pair values by the
-// structural count of all earlier units when results are concatenated (the
-// JSON Lines lastStructuralIndex carry, parse_json_lines.cu:1222); chunk0,
-// grp0 and sgrp0 number the units' matching chunks and their 64- and
-// 4096-chunk groups consecutively.  A unit with structurals has at least one
-// chunk, so that its pair row is written even without brackets.
-__global__ __launch_bounds__(256) void k_unit_bases(UnitOut *__restrict__ uo, uint32_t n_units,
-                                                    uint32_t concat, LaunchState *__restrict__ ls) {
-    __shared__ uint32_t wsum[4][4];
-    __shared__ uint32_t carry[4];
-    if (threadIdx.x < 4) carry[threadIdx.x] = 0;
-    __syncthreads();
+// Per-unit bases: pair_base offsets unit u's pair values by the structural
+// count of all earlier units when results are concatenated (the JSON Lines
+// lastStructuralIndex carry, parse_json_lines.cu:1222); row_base is that
+// count always; chunk0, grp0 and sgrp0 number the units' matching chunks and
+// their 64- and 4096-chunk groups consecutively.  Units with at most
+// kSmallBrackets brackets have no chunks (k_match_small pairs them in one
+// wave).  Three launches: block sums, a scan of the block sums, and a final
+// pass that writes the fields, the chunk -> unit map of units with up to 64
+// chunks and the list of larger units (whose maps k_chunk_map writes).
+constexpr int kBaseThreads = 256;
+
+// Counters of a unit: structurals, chunks, 64- and 4096-chunk groups, brackets.
+constexpr int kUnitCounters = 5;
+__device__ __forceinline__ void unit_counts(const UnitOut &o, uint32_t v[kUnitCounters]) {
+    v[0] = o.n_end - o.n_base;
+    const uint32_t m = o.m_end - o.m_base;
+    v[1] = m > (uint32_t)kSmallBrackets ? (m + kMatchChunk - 1) / kMatchChunk : 0u;
+    v[2] = (v[1] + 63) / 64;
+    v[3] = (v[1] + 4095) / 4096;
+    v[4] = m;
+}
+
+// Block exclusive scan of the unit counters over kBaseThreads threads.
+__device__ __forceinline__ void block_scan4(const uint32_t v[kUnitCounters],
+                                            uint32_t pre[kUnitCounters],
+                                            uint32_t tot[kUnitCounters],
+                                            uint32_t (*wsum)[kBaseThreads / 64]) {
+    constexpr int kW = kBaseThreads / 64;
     const uint32_t w = threadIdx.x >> 6;
-    for (uint32_t base = 0; base < n_units; base += 256) {
-        const uint32_t u = base + threadIdx.x;
-        uint32_t v[4] = {0, 0, 0, 0};  // structurals, chunks, groups, super groups
-        if (u < n_units) {
-            v[0] = uo[u].n_end - uo[u].n_base;
-            const uint32_t m = uo[u].m_end - uo[u].m_base;
-            v[1] = m ? (m + kMatchChunk - 1) / kMatchChunk : (v[0] ? 1u : 0u);
-            v[2] = (v[1] + 63) / 64;
-            v[3] = (v[1] + 4095) / 4096;
-        }
-        uint32_t in[4];
+    uint32_t in[kUnitCounters];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            in[k] = wave_incl_scan(v[k]);
-            if (lane_id() == 63) wsum[k][w] = in[k];
-        }
-        __syncthreads();
-        uint32_t pre[4], tot[4];
+    for (int k = 0; k < kUnitCounters; k++) {
+        in[k] = wave_incl_scan(v[k]);
+        if (lane_id() == 63) wsum[k][w] = in[k];
+    }
+    __syncthreads();
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            pre[k] = carry[k];
-            tot[k] = 0;
-            for (uint32_t j = 0; j < 4; j++) {
-                if (j < w) pre[k] += wsum[k][j];
-                tot[k] += wsum[k][j];
-            }
-            pre[k] += in[k] - v[k];
+    for (int k = 0; k < kUnitCounters; k++) {
+        pre[k] = in[k] - v[k];
+        tot[k] = 0;
+        for (uint32_t j = 0; j < (uint32_t)kW; j++) {
+            if (j < w) pre[k] += wsum[k][j];
+            tot[k] += wsum[k][j];
         }
-        if (u < n_units) {
-            uo[u].row_base = pre[0];
-            uo[u].pair_base = concat ? pre[0] : 0u;
-            uo[u].chunk0 = pre[1];
-            uo[u].nchunks = v[1];
-            uo[u].grp0 = pre[2];
-            uo[u].sgrp0 = pre[3];
-        }
+    }
+}
+
+__global__ __launch_bounds__(kBaseThreads) void k_ub_partial(const UnitOut *__restrict__ uo,
+                                                             uint32_t n_units,
+                                                             uint32_t *__restrict__ part) {
+    __shared__ uint32_t wsum[kUnitCounters][kBaseThreads / 64];
+    const uint32_t u = blockIdx.x * kBaseThreads + threadIdx.x;
+    uint32_t v[kUnitCounters] = {0, 0, 0, 0, 0};
+    if (u < n_units) unit_counts(uo[u], v);
+    uint32_t pre[kUnitCounters], tot[kUnitCounters];
+    block_scan4(v, pre, tot, wsum);
+    if (threadIdx.x < kUnitCounters)
+        part[blockIdx.x * kUnitCounters + threadIdx.x] = tot[threadIdx.x];
+}
+
+// One block: exclusive prefixes of the block sums (in place) and the totals
+// (structurals, chunks and brackets of the launch).
+__global__ __launch_bounds__(kBaseThreads) void k_ub_scan(uint32_t *__restrict__ part,
+                                                          uint32_t n_blocks,
+                                                          LaunchState *__restrict__ ls) {
+    __shared__ uint32_t wsum[kUnitCounters][kBaseThreads / 64];
+    __shared__ uint32_t carry[kUnitCounters];
+    if (threadIdx.x < kUnitCounters) carry[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n_blocks; base += kBaseThreads) {
+        const uint32_t b = base + threadIdx.x;
+        uint32_t v[kUnitCounters] = {0, 0, 0, 0, 0};
+        if (b < n_blocks)
+            for (int k = 0; k < kUnitCounters; k++) v[k] = part[b * kUnitCounters + k];
+        uint32_t pre[kUnitCounters], tot[kUnitCounters];
+        block_scan4(v, pre, tot, wsum);
+        if (b < n_blocks)
+            for (int k = 0; k < kUnitCounters; k++) part[b * kUnitCounters + k] = carry[k] + pre[k];
         __syncthreads();
-        if (threadIdx.x < 4) carry[threadIdx.x] += tot[threadIdx.x];
+        if (threadIdx.x < kUnitCounters) carry[threadIdx.x] += tot[threadIdx.x];
         __syncthreads();
     }
-    if (threadIdx.x == 0) ls->n_chunks = carry[1];
+    if (threadIdx.x == 0) {
+        ls->n_total = carry[0];
+        ls->n_chunks = carry[1];
+        ls->m_total = carry[4];
+    }
+}
+
+__global__ __launch_bounds__(kBaseThreads) void k_ub_final(UnitOut *__restrict__ uo,
+                                                           uint32_t n_units, uint32_t concat,
+                                                           const uint32_t *__restrict__ part,
+                                                           LaunchState *__restrict__ ls,
+                                                           uint32_t *__restrict__ chunk_unit,
+                                                           uint32_t *__restrict__ big_units) {
+    __shared__ uint32_t wsum[kUnitCounters][kBaseThreads / 64];
+    const uint32_t u = blockIdx.x * kBaseThreads + threadIdx.x;
+    uint32_t v[kUnitCounters] = {0, 0, 0, 0, 0};
+    if (u < n_units) unit_counts(uo[u], v);
+    uint32_t pre[kUnitCounters], tot[kUnitCounters];
+    block_scan4(v, pre, tot, wsum);
+    if (u >= n_units) return;
+#pragma unroll
+    for (int k = 0; k < kUnitCounters; k++) pre[k] += part[blockIdx.x * kUnitCounters + k];
+    uo[u].row_base = pre[0];
+    uo[u].pair_base = concat ? pre[0] : 0u;
+    uo[u].chunk0 = pre[1];
+    uo[u].nchunks = v[1];
+    uo[u].grp0 = pre[2];
+    uo[u].sgrp0 = pre[3];
+    if (v[1] > 64) big_units[atomicAdd(&ls->n_big, 1u)] = u;
+    else
+        for (uint32_t j = 0; j < v[1]; j++) chunk_unit[pre[1] + j] = u;
+}
+
+// Chunk -> unit map of the units with more than 64 chunks, one block per unit.
+__global__ __launch_bounds__(256) void k_chunk_map(const UnitOut *__restrict__ uo,
+                                                   const uint32_t *__restrict__ big_units,
+                                                   uint32_t *__restrict__ chunk_unit,
+                                                   const LaunchState *__restrict__ ls) {
+    for (uint32_t bi = blockIdx.x; bi < ls->n_big; bi += gridDim.x) {
+        const uint32_t u = big_units[bi];
+        const uint32_t c0 = uo[u].chunk0, nch = uo[u].nchunks;
+        for (uint32_t j = threadIdx.x; j < nch; j += blockDim.x) chunk_unit[c0 + j] = u;
+    }
 }
 
 // Packs the per-unit output regions densely (see launch_assemble).  Each
@@ -563,8 +634,14 @@ hipError_t launch_assemble(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
 }
 
 hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool concat) {
-    hipLaunchKernelGGL(k_unit_bases, dim3(1), dim3(256), 0, s, b.uout, n_units, concat ? 1u : 0u,
-                       b.ls);
+    const uint32_t nb = (n_units + kBaseThreads - 1) / kBaseThreads;
+    hipLaunchKernelGGL(k_ub_partial, dim3(nb), dim3(kBaseThreads), 0, s, b.uout, n_units,
+                       b.unit_part);
+    hipLaunchKernelGGL(k_ub_scan, dim3(1), dim3(kBaseThreads), 0, s, b.unit_part, nb, b.ls);
+    hipLaunchKernelGGL(k_ub_final, dim3(nb), dim3(kBaseThreads), 0, s, b.uout, n_units,
+                       concat ? 1u : 0u, b.unit_part, b.ls, b.chunk_unit, b.big_units);
+    hipLaunchKernelGGL(k_chunk_map, dim3(n_units < 256u ? n_units : 256u), dim3(256), 0, s,
+                       b.uout, b.big_units, b.chunk_unit, b.ls);
     return hipGetLastError();
 }
 

@@ -46,19 +46,6 @@ __device__ __forceinline__ int32_t delta_of(uint8_t c) {
     return (c & 2u) ? 1 : -1;
 }
 
-// Last unit whose chunk0 <= c.  A unit without chunks has the same chunk0 as
-// the next unit, so the last unit with chunk0 <= c is the one that owns c.
-__device__ __forceinline__ uint32_t unit_of_chunk(const UnitOut *__restrict__ uo,
-                                                  uint32_t n_units, uint32_t c) {
-    uint32_t lo = 0, hi = n_units - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (uo[mid].chunk0 <= c) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 __device__ __forceinline__ DepthAgg shfl_agg_up(const DepthAgg &a, int off) {
     DepthAgg o;
     o.d = __shfl_up(a.d, off, 64);
@@ -100,7 +87,7 @@ __device__ __forceinline__ DepthAgg wave_scan_agg(DepthAgg a) {
 // K1: depth walk of every chunk.
 __global__ __launch_bounds__(kMatchThreads) void k_chunk_stats(const uint8_t *__restrict__ oc_char,
                                                                const UnitOut *__restrict__ uo,
-                                                               uint32_t n_units,
+                                                               const uint32_t *__restrict__ chunk_unit,
                                                                ChunkStat *__restrict__ ck,
                                                                const LaunchState *__restrict__ ls) {
     constexpr int kPer = kMatchChunk / kMatchThreads;
@@ -109,7 +96,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_chunk_stats(const uint8_t *__
     if (ls->overflow) return;
     const uint32_t n_chunks = ls->n_chunks;
     for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
-        const uint32_t u = unit_of_chunk(uo, n_units, c);
+        const uint32_t u = chunk_unit[c];
         const uint32_t first = uo[u].m_base + (c - uo[u].chunk0) * kMatchChunk;
         const uint32_t cnt =
             uo[u].m_end > first ? min((uint32_t)kMatchChunk, uo[u].m_end - first) : 0u;
@@ -154,18 +141,17 @@ constexpr int kScanThreads = 1024;
 __global__ __launch_bounds__(kScanThreads) void k_chunk_scan(
     UnitOut *__restrict__ uo, ChunkStat *__restrict__ ck, int32_t *__restrict__ gmin,
     int32_t *__restrict__ gmin1, int32_t *__restrict__ gmin2, DepthAgg *__restrict__ gagg,
-    int32_t *__restrict__ gd0, const LaunchState *__restrict__ ls) {
+    int32_t *__restrict__ gd0, const uint32_t *__restrict__ big_units,
+    const LaunchState *__restrict__ ls) {
     constexpr int kW = kScanThreads / 64;
     __shared__ DepthAgg part[kW];
     if (ls->overflow) return;
-    const uint32_t u = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = lane_id();
+    for (uint32_t bi = blockIdx.x; bi < ls->n_big; bi += gridDim.x) {
+    __syncthreads();  // part[] of the previous unit no longer read
+    const uint32_t u = big_units[bi], tid = threadIdx.x, w = tid >> 6, lane = lane_id();
     const uint32_t c0 = uo[u].chunk0, nch = uo[u].nchunks;
     const uint32_t g0 = uo[u].grp0, h0 = uo[u].sgrp0;
     const uint32_t G = (nch + 63) / 64, H = (nch + 4095) / 4096;
-    if (nch == 0) {
-        if (tid == 0) uo[u].fast = 1;
-        return;
-    }
     // (a) group walks, two groups per wave step.
     for (uint32_t j = w; j < G; j += 2 * kW) {
         const uint32_t j2 = j + kW;
@@ -250,6 +236,45 @@ __global__ __launch_bounds__(kScanThreads) void k_chunk_scan(
             gmin[c0 + x2] = d0 + s2.mn;
         }
     }
+    }
+}
+
+// K2 for units of at most 64 chunks, one wave per unit: the ordered scan of
+// the unit's chunk walks, chunk start depths and minima, the (single) group
+// minima, and the well-nested test.
+__global__ __launch_bounds__(256) void k_chunk_scan_small(UnitOut *__restrict__ uo,
+                                                          uint32_t n_units,
+                                                          ChunkStat *__restrict__ ck,
+                                                          int32_t *__restrict__ gmin,
+                                                          int32_t *__restrict__ gmin1,
+                                                          int32_t *__restrict__ gmin2,
+                                                          const LaunchState *__restrict__ ls) {
+    if (ls->overflow) return;
+    const uint32_t lane = lane_id();
+    const uint32_t u = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (u >= n_units) return;
+    const uint32_t nch = uo[u].nchunks;
+    if (nch > 64) return;  // k_chunk_scan
+    if (nch == 0) return;  // k_match_small decides these units
+    const uint32_t c0 = uo[u].chunk0;
+    ChunkStat st{0, 0, 0, 0};
+    if (lane < nch) st = ck[c0 + lane];
+    const DepthAgg a{st.d, st.mn, st.mx};
+    const DepthAgg incl = wave_scan_agg(a);
+    const DepthAgg t = {__shfl(incl.d, 63, 64), __shfl(incl.mn, 63, 64), __shfl(incl.mx, 63, 64)};
+    const int32_t d0 = incl.d - st.d;
+    int32_t m = INT32_MAX;
+    if (lane < nch) {
+        ck[c0 + lane].d0 = d0;
+        gmin[c0 + lane] = d0 + st.mn;
+        m = d0 + st.mn;
+    }
+    for (int off = 32; off >= 1; off >>= 1) m = min(m, __shfl_xor(m, off, 64));
+    if (lane == 0) {
+        gmin1[uo[u].grp0] = m;
+        gmin2[uo[u].sgrp0] = m;
+        uo[u].fast = (t.d == 0 && t.mn >= 0 && t.mx <= kMatchMaxDepth) ? 1u : 0u;
+    }
 }
 
 // K3: matching inside one chunk.  Thread t owns the 16 brackets
@@ -267,8 +292,8 @@ constexpr int kRowTile = 4096;
 constexpr uint16_t kNoPart = 0xFFFF;
 __global__ __launch_bounds__(kMatchThreads) void k_match_local(
     const uint8_t *__restrict__ oc_char, const uint32_t *__restrict__ oc_idx,
-    UnitOut *__restrict__ uo, uint32_t n_units, const ChunkStat *__restrict__ ck,
-    uint32_t *__restrict__ lows, uint32_t *__restrict__ highs, int32_t *__restrict__ pair_row,
+    UnitOut *__restrict__ uo, const uint32_t *__restrict__ chunk_unit,
+    const ChunkStat *__restrict__ ck, uint32_t *__restrict__ lows, uint32_t *__restrict__ highs, int32_t *__restrict__ pair_row,
     const LaunchState *__restrict__ ls) {
     constexpr int kW = kMatchThreads / 64;
     constexpr int kGroups = kMatchThreads / 16;
@@ -292,7 +317,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_local(
     const uint32_t tid = threadIdx.x, w = tid >> 6, lane = lane_id();
     const uint32_t n_chunks = ls->n_chunks;
     for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
-        const uint32_t u = unit_of_chunk(uo, n_units, c);
+        const uint32_t u = chunk_unit[c];
         if (!uo[u].fast) continue;
         const uint32_t xc = c - uo[u].chunk0;
         const uint32_t first = uo[u].m_base + xc * kMatchChunk;
@@ -440,6 +465,65 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_local(
     }
 }
 
+// Units with at most kSmallBrackets brackets, one wave per unit, lane l
+// holding bracket l: the depth walk by a wave scan, the well-nested test, and
+// the matching by ballots (an opener's closer is the next bracket of its
+// key).  The unit's pair row is written densely through a 64-cell LDS window.
+// A unit that is not well nested keeps fast = 0 and goes to the sort path.
+__global__ __launch_bounds__(256) void k_match_small(const uint8_t *__restrict__ oc_char,
+                                                     const uint32_t *__restrict__ oc_idx,
+                                                     UnitOut *__restrict__ uo, uint32_t n_units,
+                                                     int32_t *__restrict__ pair_row,
+                                                     const LaunchState *__restrict__ ls) {
+    __shared__ int32_t s_win[4][64];
+    if (ls->overflow) return;
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t u = blockIdx.x * 4 + w;
+    if (u >= n_units) return;
+    const uint32_t mb = uo[u].m_base, M = uo[u].m_end - mb;
+    if (M > (uint32_t)kSmallBrackets) return;  // the chunk kernels
+    const uint32_t nb = uo[u].n_base, ne = uo[u].n_end, pbase = uo[u].pair_base + 1u;
+    const bool has = lane < M;
+    const uint32_t c = has ? oc_char[mb + lane] : 0u;
+    const uint32_t row = has ? oc_idx[mb + lane] : 0u;
+    const int32_t d = has ? delta_of((uint8_t)c) : 0;
+    const int32_t incl = (int32_t)wave_incl_scan((uint32_t)d);  // depth after the bracket
+    const bool bad = has && (incl < 0 || incl > kMatchMaxDepth);
+    if (__ballot(bad) || __shfl(incl, 63, 64) != 0) return;  // not well nested
+    const uint32_t key = (uint32_t)(d > 0 ? incl - 1 : incl);  // exact depth, in [0, 255]
+    uint64_t peers = __ballot(has);
+#pragma unroll
+    for (int bit = 0; bit < 8; bit++) {
+        const uint32_t bv = (key >> bit) & 1u;
+        const uint64_t bm = __ballot(bv);
+        peers &= bv ? bm : ~bm;
+    }
+    const uint64_t after = lane == 63 ? 0ull : peers & ~((2ull << lane) - 1ull);
+    const uint32_t partner = after ? (uint32_t)__ffsll((long long)after) - 1u : lane;
+    const uint32_t c2 = (uint32_t)__shfl((int)c, (int)partner, 64);
+    const uint32_t row2 = (uint32_t)__shfl((int)row, (int)partner, 64);
+    const bool opener = has && d > 0;
+    const bool err = opener && (c ^ c2) != 6u;
+    for (uint32_t base = nb; base < ne; base += 64) {
+        s_win[w][lane] = -1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (opener && row >= base && row < base + 64)
+            s_win[w][row - base] = (int32_t)(row2 - nb + pbase);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (base + lane < ne) pair_row[base + lane] = s_win[w][lane];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    const uint64_t any_err = __ballot(err);
+    if (lane == 0) {
+        if (any_err) atomicOr(&uo[u].struct_err, 1u);
+        uo[u].fast = 1;
+    }
+}
+
 // First chunk (unit-local index) x >= x0 whose walk reaches depth d or below,
 // or nch.  The whole wave runs one search: each step tests 64 entries of one
 // level (chunks, 64-chunk groups, 4096-chunk groups) with a ballot.
@@ -492,8 +576,8 @@ __device__ __forceinline__ uint32_t wave_next_reaching(const int32_t *__restrict
 constexpr uint32_t kNearChunks = 8;
 __global__ __launch_bounds__(64) void k_match_cross(
     const uint8_t *__restrict__ oc_char, const uint32_t *__restrict__ oc_idx,
-    UnitOut *__restrict__ uo, uint32_t n_units, const ChunkStat *__restrict__ ck,
-    const int32_t *__restrict__ gmin, const int32_t *__restrict__ gmin1,
+    UnitOut *__restrict__ uo, const uint32_t *__restrict__ chunk_unit,
+    const ChunkStat *__restrict__ ck, const int32_t *__restrict__ gmin, const int32_t *__restrict__ gmin1,
     const int32_t *__restrict__ gmin2, const uint32_t *__restrict__ lows,
     const uint32_t *__restrict__ highs, int32_t *__restrict__ pair_row,
     const LaunchState *__restrict__ ls) {
@@ -501,7 +585,7 @@ __global__ __launch_bounds__(64) void k_match_cross(
     const uint32_t n_chunks = ls->n_chunks;
     const uint32_t lane = lane_id();
     for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
-        const uint32_t u = unit_of_chunk(uo, n_units, c);
+        const uint32_t u = chunk_unit[c];
         if (!uo[u].fast) continue;
         const uint32_t c0 = uo[u].chunk0, nch = uo[u].nchunks;
         const uint32_t g0 = uo[u].grp0, h0 = uo[u].sgrp0;
@@ -555,19 +639,24 @@ __global__ __launch_bounds__(64) void k_match_cross(
 hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, uint32_t chunk_bound,
                         uint32_t group_bound, int32_t *pair_row) {
     (void)group_bound;
-    if (chunk_bound == 0) return hipSuccess;
     // The chunk count is only known on the device: grid-stride over a grid
     // that fills the chip (256 CUs) without launching one block per bound.
-    const uint32_t g = chunk_bound < 2048u ? chunk_bound : 2048u;
-    const uint32_t gc = chunk_bound < 8192u ? chunk_bound : 8192u;
+    const uint32_t g = chunk_bound < 2048u ? (chunk_bound ? chunk_bound : 1u) : 2048u;
+    const uint32_t gc = chunk_bound < 8192u ? (chunk_bound ? chunk_bound : 1u) : 8192u;
+    const uint32_t gb = n_units < 256u ? n_units : 256u;
+    hipLaunchKernelGGL(k_match_small, dim3((n_units + 3) / 4), dim3(256), 0, s, b.oc_char,
+                       b.oc_idx, b.uout, n_units, pair_row, b.ls);
     hipLaunchKernelGGL(k_chunk_stats, dim3(g), dim3(kMatchThreads), 0, s, b.oc_char, b.uout,
-                       n_units, b.ck, b.ls);
-    hipLaunchKernelGGL(k_chunk_scan, dim3(n_units), dim3(kScanThreads), 0, s, b.uout, b.ck,
-                       b.gmin, b.gmin1, b.gmin2, b.gagg, b.gd0, b.ls);
+                       b.chunk_unit, b.ck, b.ls);
+    hipLaunchKernelGGL(k_chunk_scan_small, dim3((n_units + 3) / 4), dim3(256), 0, s, b.uout,
+                       n_units, b.ck, b.gmin, b.gmin1, b.gmin2, b.ls);
+    hipLaunchKernelGGL(k_chunk_scan, dim3(gb), dim3(kScanThreads), 0, s, b.uout, b.ck, b.gmin,
+                       b.gmin1, b.gmin2, b.gagg, b.gd0, b.big_units, b.ls);
     hipLaunchKernelGGL(k_match_local, dim3(g), dim3(kMatchThreads), 0, s, b.oc_char, b.oc_idx,
-                       b.uout, n_units, b.ck, b.lows, b.highs, pair_row, b.ls);
+                       b.uout, b.chunk_unit, b.ck, b.lows, b.highs, pair_row, b.ls);
     hipLaunchKernelGGL(k_match_cross, dim3(gc), dim3(64), 0, s, b.oc_char, b.oc_idx, b.uout,
-                       n_units, b.ck, b.gmin, b.gmin1, b.gmin2, b.lows, b.highs, pair_row, b.ls);
+                       b.chunk_unit, b.ck, b.gmin, b.gmin1, b.gmin2, b.lows, b.highs, pair_row,
+                       b.ls);
     return hipGetLastError();
 }
 

@@ -294,27 +294,20 @@ __device__ __forceinline__ void lookback_tile(const unsigned long long *st, uint
     }
 }
 
-__device__ __forceinline__ uint32_t find_unit(const UnitDesc *__restrict__ units, uint32_t n_units,
-                                              uint32_t t) {
-    uint32_t lo = 0, hi = n_units - 1;  // last unit with tile0 <= t
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (units[mid].tile0 <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     return wave_read(wave_incl_scan(x), 63);
 }
 
-// One workgroup per tile; tiles are taken from a ticket counter in launch
-// order so that every look-back waits only on running workgroups.
-template <bool JSONL>
+// Large units (SMALL = false): one workgroup per 64 KiB tile; tiles are taken
+// from a ticket counter in launch order so that every look-back waits only on
+// running workgroups.  Small units (SMALL = true, at most one wave's 16 KiB):
+// one wave per unit, which starts in state (0, 0) and owns the unit's whole
+// output region, so there is no look-back and no workgroup barrier.
+template <bool JSONL, bool SMALL>
 __global__ __launch_bounds__(kTokThreads) void k_tokenize(
-    const uint8_t *__restrict__ in, const UnitDesc *__restrict__ units, uint32_t n_units,
-    uint32_t ntiles, unsigned long long *__restrict__ tile_st, LaunchState *__restrict__ ls,
+    const uint8_t *__restrict__ in, const UnitDesc *__restrict__ units,
+    const uint32_t *__restrict__ tile_unit, const uint32_t *__restrict__ small_units,
+    uint32_t n_small, unsigned long long *__restrict__ tile_st, LaunchState *__restrict__ ls,
     UnitOut *__restrict__ uo, uint32_t *__restrict__ out_struct, uint8_t *__restrict__ oc_char,
     uint32_t *__restrict__ oc_idx, uint32_t weak_utf8) {
     __shared__ uint32_t s_tile, s_unit;
@@ -331,18 +324,26 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
     const uint32_t w = tid >> 6, lane = tid & 63;
     const uint64_t lt = lanemask_lt();
 
-    if (tid == 0) {
-        const uint32_t t = atomicAdd(&ls->tile_counter, 1u);
-        s_tile = t;
-        s_unit = find_unit(units, n_units, t);
+    uint32_t tile = 0, u;
+    if constexpr (SMALL) {
+        const uint32_t idx = blockIdx.x * kWaves + w;
+        if (idx >= n_small) return;  // the whole wave; SMALL code has no workgroup barrier
+        u = small_units[idx];
+    } else {
+        if (tid == 0) {
+            const uint32_t t = atomicAdd(&ls->tile_counter, 1u);
+            s_tile = t;
+            s_unit = tile_unit[t];
+        }
+        __syncthreads();
+        tile = s_tile;
+        u = s_unit;
     }
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint32_t u = s_unit;
     const UnitDesc U = units[u];
-    const uint32_t tile_in_unit = tile - U.tile0;
+    const uint32_t tile_in_unit = SMALL ? 0u : tile - U.tile0;
     const uint8_t *ubase = in + U.in_off;
-    const uint64_t wave_off = (uint64_t)tile_in_unit * kTileBytes + (uint64_t)w * kWaveBytes;
+    const uint64_t wave_off =
+        SMALL ? 0ull : (uint64_t)tile_in_unit * kTileBytes + (uint64_t)w * kWaveBytes;
 
     // Rounds stream through a double buffer: round i+2 is loaded as soon as
     // round i has been transposed.
@@ -371,6 +372,12 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
 
 #pragma unroll
     for (int i = 0; i < kTokRounds; i++) {
+        // Rounds past the end of the unit hold no bytes: no masks, carries
+        // unchanged (nothing after the unit's end depends on them).
+        if (wave_off + (uint64_t)i * kRoundBytes >= U.n) {
+            instr_r[i] = op_r[i] = br_r[i] = 0;
+            continue;
+        }
         const uint64_t lane_off = wave_off + (uint64_t)i * kRoundBytes + (uint64_t)lane * kLaneBytes;
         uint32_t r[8], k[8];
         transpose4(x[i & 1][0], x[i & 1][2], x[i & 1][4], x[i & 1][6], r[0], r[1], r[2], r[3]);
@@ -445,74 +452,93 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
 
     const uint64_t bad_lanes = __ballot(utf8_bad);
     const uint32_t wcnt0 = wave_sum(lane_cnt0), wall = wave_sum(lane_all);
-    if (lane == 0) {
-        if (bad_lanes) atomicOr(&uo[u].utf8_err, 1u);
-        s_map[w] = (seen ? 0u : 1u) | (c_run << 1) | (dflag << 2) | (s_run << 3);
-        s_cnt0[w] = wcnt0;
-        s_cnt_all[w] = wall;
-    }
-    __syncthreads();
-
-    // ---- one look-back: entry state and output offsets -----------------------
-    if (w == 0) {
-        // The tile entered at (c, s) = (0, 0): wave string fixes and counts.
-        uint32_t T = kMapIdentity, state = 0, n0 = 0, m0 = 0, opc = 0, brc = 0;
-        uint32_t fix0[kWaves];
-#pragma unroll
-        for (int kk = 0; kk < kWaves; kk++) {
-            const uint32_t m = s_map[kk];
-            fix0[kk] = ((state >> 1) & 1u) ^ ((state & 1u) & ((m >> 2) & 1u));
-            state = map_apply(m, state);
-            T = map_compose(T, m);
-            const uint32_t c0 = s_cnt0[kk], ca = s_cnt_all[kk];
-            n0 += fix0[kk] ? (ca & 0xFFFFu) - (c0 & 0xFFFFu) : (c0 & 0xFFFFu);
-            m0 += fix0[kk] ? (ca >> 16) - (c0 >> 16) : (c0 >> 16);
-            opc += ca & 0xFFFFu;
-            brc += ca >> 16;
-        }
-        uint32_t entry = 0, n_excl = 0, m_excl = 0;
-        if (tile_in_unit != 0) {
-            if (lane == 0) publish_agg(tile_st, tile, T, n0, m0, opc, brc);
-            lookback_tile(tile_st, tile, U.tile0, entry, n_excl, m_excl);
-        }
-        const uint32_t e = ((entry >> 1) ^ (entry & (T >> 2))) & 1u;
-        const uint32_t nt = e ? opc - n0 : n0, mt = e ? brc - m0 : m0;
+    uint32_t fix, nb, mb;
+    if constexpr (SMALL) {
+        // The unit is this wave: entered in state (0, 0), so no fix.
+        const uint32_t nu = wcnt0 & 0xFFFFu, mu = wcnt0 >> 16;
         if (lane == 0) {
-            publish_inc(tile_st, tile, map_apply(T, entry), n_excl + nt, m_excl + mt);
-            uint32_t nb = U.out_off + n_excl, mb = U.out_off + m_excl;
+            if (bad_lanes) atomicOr(&uo[u].utf8_err, 1u);
+            uo[u].n_base = U.out_off;
+            uo[u].m_base = U.out_off;
+            uo[u].n_end = U.out_off + nu;
+            uo[u].m_end = U.out_off + mu;
+            if (nu > U.out_cap) atomicOr(&ls->overflow, 1u);
+        }
+        fix = 0;
+        nb = mb = U.out_off;
+        (void)wall;
+        (void)dflag;
+        (void)seen;
+    } else {
+        if (lane == 0) {
+            if (bad_lanes) atomicOr(&uo[u].utf8_err, 1u);
+            s_map[w] = (seen ? 0u : 1u) | (c_run << 1) | (dflag << 2) | (s_run << 3);
+            s_cnt0[w] = wcnt0;
+            s_cnt_all[w] = wall;
+        }
+        __syncthreads();
+
+        // ---- one look-back: entry state and output offsets -----------------------
+        if (w == 0) {
+            // The tile entered at (c, s) = (0, 0): wave string fixes and counts.
+            uint32_t T = kMapIdentity, state = 0, n0 = 0, m0 = 0, opc = 0, brc = 0;
+            uint32_t fix0[kWaves];
 #pragma unroll
             for (int kk = 0; kk < kWaves; kk++) {
-                const uint32_t f = fix0[kk] ^ e;
+                const uint32_t m = s_map[kk];
+                fix0[kk] = ((state >> 1) & 1u) ^ ((state & 1u) & ((m >> 2) & 1u));
+                state = map_apply(m, state);
+                T = map_compose(T, m);
                 const uint32_t c0 = s_cnt0[kk], ca = s_cnt_all[kk];
-                s_fix[kk] = f;
-                s_nbase[kk] = nb;
-                s_mbase[kk] = mb;
-                nb += f ? (ca & 0xFFFFu) - (c0 & 0xFFFFu) : (c0 & 0xFFFFu);
-                mb += f ? (ca >> 16) - (c0 >> 16) : (c0 >> 16);
+                n0 += fix0[kk] ? (ca & 0xFFFFu) - (c0 & 0xFFFFu) : (c0 & 0xFFFFu);
+                m0 += fix0[kk] ? (ca >> 16) - (c0 >> 16) : (c0 >> 16);
+                opc += ca & 0xFFFFu;
+                brc += ca >> 16;
             }
-            if (tile_in_unit == 0) {
-                uo[u].n_base = U.out_off;
-                uo[u].m_base = U.out_off;
+            uint32_t entry = 0, n_excl = 0, m_excl = 0;
+            if (tile_in_unit != 0) {
+                if (lane == 0) publish_agg(tile_st, tile, T, n0, m0, opc, brc);
+                lookback_tile(tile_st, tile, U.tile0, entry, n_excl, m_excl);
             }
-            if (tile_in_unit == U.ntiles - 1) {
-                const uint32_t nu = n_excl + nt, mu = m_excl + mt;
-                uo[u].n_end = U.out_off + nu;
-                uo[u].m_end = U.out_off + mu;
-                if (nu > U.out_cap) atomicOr(&ls->overflow, 1u);
-                atomicAdd(&ls->n_total, nu);
-                atomicAdd(&ls->m_total, mu);
+            const uint32_t e = ((entry >> 1) ^ (entry & (T >> 2))) & 1u;
+            const uint32_t nt = e ? opc - n0 : n0, mt = e ? brc - m0 : m0;
+            if (lane == 0) {
+                publish_inc(tile_st, tile, map_apply(T, entry), n_excl + nt, m_excl + mt);
+                uint32_t nb = U.out_off + n_excl, mb = U.out_off + m_excl;
+#pragma unroll
+                for (int kk = 0; kk < kWaves; kk++) {
+                    const uint32_t f = fix0[kk] ^ e;
+                    const uint32_t c0 = s_cnt0[kk], ca = s_cnt_all[kk];
+                    s_fix[kk] = f;
+                    s_nbase[kk] = nb;
+                    s_mbase[kk] = mb;
+                    nb += f ? (ca & 0xFFFFu) - (c0 & 0xFFFFu) : (c0 & 0xFFFFu);
+                    mb += f ? (ca >> 16) - (c0 >> 16) : (c0 >> 16);
+                }
+                if (tile_in_unit == 0) {
+                    uo[u].n_base = U.out_off;
+                    uo[u].m_base = U.out_off;
+                }
+                if (tile_in_unit == U.ntiles - 1) {
+                    const uint32_t nu = n_excl + nt, mu = m_excl + mt;
+                    uo[u].n_end = U.out_off + nu;
+                    uo[u].m_end = U.out_off + mu;
+                    if (nu > U.out_cap) atomicOr(&ls->overflow, 1u);
+                }
             }
         }
+        __syncthreads();
+        fix = s_fix[w] ? ~0u : 0u;
+        nb = s_nbase[w];
+        mb = s_mbase[w];
     }
-    __syncthreads();
 
     // ---- select structurals and compact them ---------------------------------
-    const uint32_t fix = s_fix[w] ? ~0u : 0u;
-    uint32_t nb = s_nbase[w], mb = s_mbase[w];
     const uint32_t cap_end = U.out_off + U.out_cap;
     uint32_t overflow = 0;
 #pragma unroll
     for (int i = 0; i < kTokRounds; i++) {
+        if (wave_off + (uint64_t)i * kRoundBytes >= U.n) break;  // no bytes, no rows
         const uint64_t lane_off = wave_off + (uint64_t)i * kRoundBytes + (uint64_t)lane * kLaneBytes;
         const uint32_t sti = op_r[i] & ~(instr_r[i] ^ fix);
         const uint32_t oci = br_r[i] & sti;
@@ -583,17 +609,29 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
 }
 
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                           uint32_t ntiles, uint32_t n_units, bool jsonl, bool weak_utf8,
+                           uint32_t ntiles, uint32_t n_small, bool jsonl, bool weak_utf8,
                            uint32_t *out_struct) {
     const uint32_t weak = weak_utf8 ? 1u : 0u;
-    if (jsonl)
-        hipLaunchKernelGGL(k_tokenize<true>, dim3(ntiles), dim3(kTokThreads), 0, s, d_in, b.units,
-                           n_units, ntiles, b.tile_st, b.ls, b.uout, out_struct, b.oc_char,
-                           b.oc_idx, weak);
-    else
-        hipLaunchKernelGGL(k_tokenize<false>, dim3(ntiles), dim3(kTokThreads), 0, s, d_in, b.units,
-                           n_units, ntiles, b.tile_st, b.ls, b.uout, out_struct, b.oc_char,
-                           b.oc_idx, weak);
+    const uint32_t gs = (n_small + kWaves - 1) / kWaves;
+    if (jsonl) {
+        if (ntiles)
+            hipLaunchKernelGGL((k_tokenize<true, false>), dim3(ntiles), dim3(kTokThreads), 0, s,
+                               d_in, b.units, b.tile_unit, b.small_units, n_small, b.tile_st, b.ls,
+                               b.uout, out_struct, b.oc_char, b.oc_idx, weak);
+        if (n_small)
+            hipLaunchKernelGGL((k_tokenize<true, true>), dim3(gs), dim3(kTokThreads), 0, s, d_in,
+                               b.units, b.tile_unit, b.small_units, n_small, b.tile_st, b.ls,
+                               b.uout, out_struct, b.oc_char, b.oc_idx, weak);
+    } else {
+        if (ntiles)
+            hipLaunchKernelGGL((k_tokenize<false, false>), dim3(ntiles), dim3(kTokThreads), 0, s,
+                               d_in, b.units, b.tile_unit, b.small_units, n_small, b.tile_st, b.ls,
+                               b.uout, out_struct, b.oc_char, b.oc_idx, weak);
+        if (n_small)
+            hipLaunchKernelGGL((k_tokenize<false, true>), dim3(gs), dim3(kTokThreads), 0, s, d_in,
+                               b.units, b.tile_unit, b.small_units, n_small, b.tile_st, b.ls,
+                               b.uout, out_struct, b.oc_char, b.oc_idx, weak);
+    }
     return hipGetLastError();
 }
 

@@ -48,24 +48,21 @@ __device__ __forceinline__ uint32_t word_at(const uint8_t *base, uint64_t o, uin
 // One block per tokenizer tile; thread t checks bytes [256 t, 256 t + 256) of
 // the tile, four at a time, skipping words that hold no byte >= 80 and follow
 // a word without one.
+// Blocks past the tiles take one small unit each (at most 16 KiB).
 __global__ __launch_bounds__(kU8Threads) void k_utf8_strict(const uint8_t *__restrict__ in,
                                                             const UnitDesc *__restrict__ units,
-                                                            uint32_t n_units,
-                                                            UnitOut *__restrict__ uo,
-                                                            const LaunchState *__restrict__ ls) {
-    (void)ls;
-    const uint32_t tile = blockIdx.x;
-    uint32_t lo = 0, hi = n_units - 1;  // last unit with tile0 <= tile
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (units[mid].tile0 <= tile) lo = mid;
-        else hi = mid - 1;
-    }
-    const uint32_t u = lo;
+                                                            const uint32_t *__restrict__ tile_unit,
+                                                            const uint32_t *__restrict__ small_units,
+                                                            uint32_t ntiles,
+                                                            UnitOut *__restrict__ uo) {
+    const uint32_t blk = blockIdx.x;
+    const bool small = blk >= ntiles;
+    const uint32_t u = small ? small_units[blk - ntiles] : tile_unit[blk];
     const UnitDesc U = units[u];
     const uint8_t *base = in + U.in_off;
     const uint64_t n = U.n;
-    const uint64_t s = (uint64_t)(tile - U.tile0) * kTileBytes + threadIdx.x * kU8ThreadBytes;
+    const uint64_t s = (small ? 0ull : (uint64_t)(blk - U.tile0) * kTileBytes) +
+                       threadIdx.x * kU8ThreadBytes;
     if (s >= n) return;
     uint32_t prev = s >= 4 ? word_at(base, s - 4, n) : 0u;
     uint32_t bad = 0;
@@ -94,10 +91,10 @@ __global__ __launch_bounds__(kU8Threads) void k_utf8_strict(const uint8_t *__res
 }
 
 hipError_t launch_utf8_strict(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                              uint32_t ntiles, uint32_t n_units) {
-    if (ntiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_utf8_strict, dim3(ntiles), dim3(kU8Threads), 0, s, d_in, b.units,
-                       n_units, b.uout, b.ls);
+                              uint32_t ntiles, uint32_t n_small) {
+    if (ntiles + n_small == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_utf8_strict, dim3(ntiles + n_small), dim3(kU8Threads), 0, s, d_in,
+                       b.units, b.tile_unit, b.small_units, ntiles, b.uout);
     return hipGetLastError();
 }
 
