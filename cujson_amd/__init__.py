@@ -224,23 +224,26 @@ class Context:
     def parse_batch_device(self, d_input: int, doc_off, doc_size, d_structural: int,
                            d_pair: int, cap: int, flags: int = 0) -> int:
         """Enqueue a batch of device-resident documents (offsets 16-byte aligned)."""
-        k = len(doc_off)
+        offs = np.ascontiguousarray(doc_off, dtype=np.uint64)
+        sizes = np.ascontiguousarray(doc_size, dtype=np.uint64)
+        k = offs.size
         self._batch_n = k
-        offs = (ctypes.c_size_t * k)(*doc_off)
-        sizes = (ctypes.c_size_t * k)(*doc_size)
-        return lib().cjx_parse_batch_device(self._h, ctypes.c_void_p(d_input), offs, sizes, k,
-                                            flags, ctypes.c_void_p(d_structural),
+        szp = ctypes.POINTER(ctypes.c_size_t)
+        return lib().cjx_parse_batch_device(self._h, ctypes.c_void_p(d_input),
+                                            offs.ctypes.data_as(szp), sizes.ctypes.data_as(szp),
+                                            k, flags, ctypes.c_void_p(d_structural),
                                             ctypes.c_void_p(d_pair), cap)
 
     def batch_wait(self):
         """(rc, docs, N, M, overflow): docs is an int64 array [k, 4] of
         (byte_base, struct_begin, struct_end, error)."""
         k = getattr(self, "_batch_n", 0)
-        arr = (_DocResult * max(k, 1))()
+        arr = np.zeros((max(k, 1), 4), dtype=np.int64)  # the cjx_doc_result layout
         st = _DeviceStatus()
-        rc = lib().cjx_batch_wait(self._h, arr, k, ctypes.byref(st))
-        docs = np.array([(d.byte_base, d.struct_begin, d.struct_end, d.error) for d in arr[:k]],
-                        dtype=np.int64).reshape(-1, 4)
+        rc = lib().cjx_batch_wait(self._h, arr.ctypes.data_as(ctypes.POINTER(_DocResult)), k,
+                                  ctypes.byref(st))
+        docs = arr[:k].copy()
+        docs[:, 3] = docs[:, 3] & 0xFFFFFFFF  # error (int32) then the reserved field
         return rc, docs, st.n_structural, st.n_brackets, bool(st.capacity_exceeded)
 
 
@@ -430,8 +433,10 @@ class BatchResult:
         if rc == CJX_OK and r.structural:
             self.structural = np.ctypeslib.as_array(r.structural, shape=(max(self.N, 1),))[:self.N]
             self.pair_pos = np.ctypeslib.as_array(r.pair_pos, shape=(max(self.N, 1),))[:self.N]
-            self.docs = np.array([(d.byte_base, d.struct_begin, d.struct_end, d.error)
-                                  for d in r.docs[:self.n_docs]], dtype=np.int64).reshape(-1, 4)
+            raw = np.ctypeslib.as_array(ctypes.cast(r.docs, ctypes.POINTER(ctypes.c_int64)),
+                                        shape=(self.n_docs * 4,)).reshape(-1, 4).copy()
+            raw[:, 3] = raw[:, 3] & 0xFFFFFFFF  # error (int32) then the reserved field
+            self.docs = raw
         else:
             self.structural = self.pair_pos = None
             self.docs = np.zeros((0, 4), np.int64)
@@ -467,9 +472,10 @@ def parse_batch(data, doc_off, doc_size, flags: int = 0,
     (documents are data[doc_off[i] : doc_off[i] + doc_size[i]])."""
     a, p = _as_u8(data)
     ctx = ctx or default_context()
-    k = len(doc_off)
-    offs = (ctypes.c_size_t * max(k, 1))(*doc_off)
-    sizes = (ctypes.c_size_t * max(k, 1))(*doc_size)
+    offs = np.ascontiguousarray(doc_off, dtype=np.uint64)
+    sizes = np.ascontiguousarray(doc_size, dtype=np.uint64)
+    szp = ctypes.POINTER(ctypes.c_size_t)
     r = _BatchResult()
-    rc = lib().cjx_parse_batch(ctx.handle, p, offs, sizes, k, flags, ctypes.byref(r))
+    rc = lib().cjx_parse_batch(ctx.handle, p, offs.ctypes.data_as(szp), sizes.ctypes.data_as(szp),
+                               offs.size, flags, ctypes.byref(r))
     return BatchResult(r, rc)
