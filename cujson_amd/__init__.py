@@ -48,6 +48,7 @@ CJX_FLAG_UTF8_STRICT = 0x1
 CJX_FLAG_DEPTH_U16 = 0x2
 CJX_FLAG_JSONL = 0x4
 CJX_FLAG_GRAPH = 0x8
+CJX_FLAG_RECORDS = 0x10
 
 # Every symbol include/cujson_amd.h declares.
 ABI_SYMBOLS = (
@@ -74,7 +75,15 @@ class _Result(ctypes.Structure):
         ("error", ctypes.c_int32),
         ("error_chunk", ctypes.c_int32),
         ("device_ms", ctypes.c_double),
+        ("n_records", ctypes.c_int64),
+        ("records", ctypes.c_void_p),
     ]
+
+
+# cjx_record: one JSON Lines record (CJX_FLAG_RECORDS)
+RECORD_DTYPE = np.dtype([("byte_begin", np.int64), ("byte_end", np.int64),
+                         ("struct_first", np.int32), ("struct_last", np.int32),
+                         ("chunk", np.int32), ("error", np.int32)])
 
 
 class _Input(ctypes.Structure):
@@ -313,6 +322,12 @@ class cuJSONResult:
             self.resultSizes = []
             self.resultSizesPrefix = []
         self.N = N
+        nrec = int(r.n_records)
+        if r.records and nrec:
+            raw = (ctypes.c_char * (nrec * RECORD_DTYPE.itemsize)).from_address(r.records)
+            self.records = np.frombuffer(raw, dtype=RECORD_DTYPE).copy()
+        else:
+            self.records = np.zeros(0, dtype=RECORD_DTYPE)
 
     def free(self):
         if self._r is not None:

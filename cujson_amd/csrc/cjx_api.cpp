@@ -19,6 +19,9 @@
 
 using namespace cjx;
 
+static_assert(sizeof(RecordOut) == sizeof(cjx_record), "record layout");
+static_assert(sizeof(cjx_doc_result) == 32, "document result layout");
+
 namespace {
 
 constexpr uint64_t kMaxLaunchBytes = 0x7FFFFFFFull;  // positions and indices stay in int32
@@ -83,7 +86,7 @@ struct cjx_context {
     uint64_t timing_count = 0;
     bool timing = false;
     DevBuf in, structural, pair;
-    DevBuf units, uout, ls, tile_st, layout, chunk_unit, big_units, unit_part;
+    DevBuf units, uout, ls, tile_st, layout, chunk_unit, big_units, unit_part, rec_count, recs;
     uint32_t *tile_unit = nullptr;    // inside `units`, after the unit descriptors
     uint32_t *small_units = nullptr;  // after the tile -> unit map
     // Pinned upload slots for unit tables: a call may return before its copy
@@ -464,6 +467,38 @@ int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint
     return CJX_OK;
 }
 
+// The per-record table of the JSON Lines units just parsed (their rows are
+// still in ctx->structural): count and number on the device, read the count,
+// fill the table on the device, copy it out.
+int collect_records(cjx_context *ctx, uint32_t n_units, cjx_result *out) {
+    hipStream_t s = ctx->stream;
+    LaunchBufs b = bufs_of(ctx);
+    if (!ctx->rec_count.ensure((size_t)n_units * 4 + 16)) return CJX_ERR_ALLOC;
+    if (launch_records(s, b, ctx->in.as<uint8_t>(), n_units, ctx->structural.as<uint32_t>(),
+                       ctx->rec_count.as<uint32_t>(), nullptr, false) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    int rc = read_status(ctx, n_units);
+    if (rc) return rc;
+    const uint64_t R = static_cast<const LaunchState *>(ctx->h_ls.p)->n_records;
+    out->n_records = (int64_t)R;
+    if (R == 0) return CJX_OK;
+    if (!ctx->recs.ensure(R * sizeof(RecordOut) + 16)) return CJX_ERR_ALLOC;
+    cjx_record *h = nullptr;
+    if (hipHostMalloc((void **)&h, R * sizeof(cjx_record), hipHostMallocDefault) != hipSuccess)
+        return CJX_ERR_ALLOC;
+    if (launch_records(s, b, ctx->in.as<uint8_t>(), n_units, ctx->structural.as<uint32_t>(),
+                       ctx->rec_count.as<uint32_t>(), ctx->recs.as<RecordOut>(), true) !=
+            hipSuccess ||
+        hipMemcpyAsync(h, ctx->recs.p, R * sizeof(cjx_record), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipHostFree(h);
+        return CJX_ERR_DEVICE;
+    }
+    out->records = h;
+    return CJX_OK;
+}
+
 int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint64_t> &offs,
                      const std::vector<uint64_t> &sizes, const std::vector<uint64_t> &pos_base,
                      unsigned flags, cjx_result *out) {
@@ -517,6 +552,14 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
     buf[2 * N + 2] = -1;
     buf[2 * N + 3] = -1;
     out->buf = buf;
+    if ((flags & CJX_FLAG_RECORDS) && (flags & CJX_FLAG_JSONL)) {
+        const int rr = collect_records(ctx, (uint32_t)units.size(), out);
+        if (rr) {
+            out->buf = nullptr;
+            (void)hipHostFree(buf);
+            return rr;
+        }
+    }
     out->structural = buf;
     out->pair_pos = buf + N + 1;
     out->total_result_size = (int32_t)(N + 2);
@@ -629,7 +672,8 @@ void cjx_context_destroy(cjx_context *ctx) {
                       &ctx->tmp,     &ctx->spos,       &ctx->block_delta, &ctx->hist,
                       &ctx->ck,      &ctx->gmin,       &ctx->gmin1,  &ctx->gmin2,
                       &ctx->gagg,    &ctx->gd0,        &ctx->lows,   &ctx->highs,
-                      &ctx->chunk_unit, &ctx->big_units, &ctx->unit_part};
+                      &ctx->chunk_unit, &ctx->big_units, &ctx->unit_part, &ctx->rec_count,
+                      &ctx->recs};
     for (DevBuf *b : bufs) b->release();
     ctx->h_units.release();
     ctx->h_uout.release();
@@ -807,6 +851,7 @@ int cjx_parse_lines(cjx_context *ctx, const uint8_t *data, size_t n, const size_
 void cjx_free_result(cjx_result *r) {
     if (!r) return;
     if (r->buf) (void)hipHostFree(r->buf);
+    if (r->records) (void)hipHostFree(r->records);
     std::free(r->result_sizes);
     std::free(r->result_sizes_prefix);
     std::memset(r, 0, sizeof(*r));

@@ -88,7 +88,15 @@ struct LaunchState {
     uint32_t n_total, m_total;
     uint32_t n_chunks;  // matching chunks of all units
     uint32_t n_big;     // units with more than 64 matching chunks (listed in big_units)
-    uint32_t pad[2];
+    uint32_t n_records; // JSON Lines records (cjx_records.hip)
+    uint32_t pad;
+};
+
+// One JSON Lines record (cjx_records.hip); same layout as cjx_record.
+struct RecordOut {
+    int64_t byte_begin, byte_end;
+    int32_t struct_first, struct_last;
+    int32_t chunk, error;
 };
 
 // Look-back status encoding (two 64-bit words per tile, see cjx_tokenize.hip).
@@ -149,6 +157,12 @@ hipError_t launch_assemble(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
 // unit (launches with several units).
 hipError_t launch_sort_units(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool depth_u16,
                              int32_t *pair_row);
+// Per-record table of JSON Lines units: write = false counts the records of
+// each unit and numbers them (rec_count becomes the bases, ls->n_records the
+// total); write = true fills rec.
+hipError_t launch_records(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
+                          uint32_t n_units, const uint32_t *d_struct, uint32_t *rec_count,
+                          RecordOut *rec, bool write);
 hipError_t launch_phase3(hipStream_t s, const LaunchBufs &b, uint32_t u, uint32_t G,
                          uint32_t pair_grid, bool depth_u16, int32_t *pair_row);
 

@@ -49,11 +49,26 @@ enum {
 #define CJX_FLAG_UTF8_STRICT 0x1u /* RFC 3629 validation (off: the reference's check)  */
 #define CJX_FLAG_DEPTH_U16   0x2u /* 16-bit depth keys (off: the reference's uint8)     */
 #define CJX_FLAG_JSONL       0x4u /* '\n' is structural (set by cjx_parse_lines)        */
+#define CJX_FLAG_RECORDS     0x10u /* cjx_parse_lines: also return the per-record table   */
 #define CJX_FLAG_GRAPH       0x8u /* device-resident calls: replay the launch sequence
                                      as a HIP graph captured on the first call of the
                                      same shape (no counterpart in the reference)      */
 
 typedef struct cjx_context cjx_context;
+
+/* One JSON Lines record (CJX_FLAG_RECORDS).  The reference has no record
+ * table; a record is implicit as the structural entries up to a '\n' entry
+ * (parse_json_lines.cu:453-456).  Here a record is the run of structural
+ * entries of one chunk that ends with a '\n' entry, plus an unterminated one
+ * when entries follow the chunk's last '\n'. */
+typedef struct cjx_record {
+    int64_t byte_begin;   /* offset of its first byte in the input            */
+    int64_t byte_end;     /* one past its '\n' (or the chunk end)             */
+    int32_t struct_first; /* first structural index (host layout, 1-based)    */
+    int32_t struct_last;  /* last one (its '\n' entry when terminated)        */
+    int32_t chunk;        /* the chunk it belongs to                          */
+    int32_t error;        /* that chunk's status code                         */
+} cjx_record;
 
 /* Host-side result; field meanings follow cuJSONResult (cujson/cujson_types.h:22-33). */
 typedef struct cjx_result {
@@ -70,6 +85,8 @@ typedef struct cjx_result {
     int32_t error;                /* status of the parse                            */
     int32_t error_chunk;          /* first failing chunk, -1 if none                */
     double device_ms;             /* device time of the phases (hipEvent)           */
+    int64_t n_records;            /* JSON Lines with CJX_FLAG_RECORDS: record count */
+    cjx_record *records;          /* and the records (pinned host memory)           */
 } cjx_result;
 
 /* Pinned host input (cuJSONInput, cujson/cujson_types.h:9-12). */

@@ -1,0 +1,85 @@
+"""Per-record table of JSON Lines parses (CJX_FLAG_RECORDS) against records
+derived from the oracle's output: a record ends at each '\\n' structural
+entry of a chunk, plus an unterminated one when entries follow the chunk's
+last '\\n'."""
+import numpy as np
+import pytest
+
+import cujson_amd as cj
+import oracle
+from cujson_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cj.Context(0)
+    yield c
+    c.close()
+
+
+def expected_records(data, chunks, o):
+    recs = []
+    st = o.structural
+    row = 1
+    for ci, ((off, size), cnt) in enumerate(zip(chunks, o.result_sizes)):
+        code = oracle.parse_unit(bytes(data[off:off + size]), oracle.FLAG_JSONL).code
+        first, prev_val = row, None
+        for t in range(row, row + cnt):
+            val = int(st[t])
+            if data[val - 1] == 0x0A:
+                recs.append((prev_val if prev_val is not None else off, val, first, t, ci, code))
+                first, prev_val = t + 1, val
+        if first <= row + cnt - 1:
+            recs.append((prev_val if prev_val is not None else off, off + size, first,
+                         row + cnt - 1, ci, code))
+        row += cnt
+    return np.array(recs, dtype=np.int64).reshape(-1, 6)
+
+
+def check_records(ctx, data, chunks, label):
+    data = bytes(data)
+    inp = cj.cuJSONLinesInput(np.frombuffer(data, np.uint8), len(chunks), len(data),
+                              [c[0] for c in chunks], [c[1] for c in chunks])
+    g = cj.parse_json_lines(inp, cj.CJX_FLAG_RECORDS, ctx)
+    o = oracle.parse_lines(data, chunks)
+    assert g.error == o.code, label
+    if o.code != oracle.OK:  # the reference aborts the file: only the codes compare
+        codes = [oracle.parse_unit(bytes(data[a:a + n]), oracle.FLAG_JSONL).code
+                 for a, n in chunks]
+        assert all(codes[r["chunk"]] == r["error"] for r in g.records), label
+        return
+    assert np.array_equal(g.buf, o.buf), label
+    want = expected_records(data, chunks, o)
+    got = np.stack([g.records[f].astype(np.int64) for f in
+                    ("byte_begin", "byte_end", "struct_first", "struct_last", "chunk", "error")],
+                   axis=1) if g.records.size else np.zeros((0, 6), np.int64)
+    assert got.shape == want.shape, (label, got.shape, want.shape)
+    if not np.array_equal(got, want):
+        bad = np.flatnonzero((got != want).any(axis=1))
+        raise AssertionError(f"{label}: {bad.size} records differ, first {bad[:4]}: "
+                             f"{got[bad[:2]]} vs {want[bad[:2]]}")
+
+
+@pytest.mark.parametrize("cc", [1, 3, 51])
+def test_records_twitter(ctx, twitter_small, cc):
+    check_records(ctx, twitter_small, oracle.chunks_by_count(twitter_small, cc), f"cc={cc}")
+
+
+def test_records_generated_lines(ctx):
+    d = bytes(gen.lines(5000))
+    for cc in (1, 7, 64):
+        check_records(ctx, d, oracle.chunks_by_count(d, cc), f"lines cc={cc}")
+    check_records(ctx, d, oracle.chunks_by_bytes(d, 4096), "lines 4 KiB chunks")
+
+
+def test_records_edge_cases(ctx):
+    cases = [b'{"a":1}\n{"b":[2]}', b'{"a":1}\n\n{"b":2}\n', b'[1,2]\n', b'1\n2\n3',
+             b'{"x":"a\nb"}\n{"y":1}\n', b'\n\n\n', b'{"a":[}\n{"b":1}\n']
+    for d in cases:
+        for cc in (1, 2):
+            chunks = [c for c in oracle.chunks_by_count(d, cc)]
+            if any(s == 0 for _, s in chunks):
+                continue
+            check_records(ctx, d, chunks, f"{d!r} cc={cc}")
