@@ -57,6 +57,7 @@ ABI_SYMBOLS = (
     "cjx_strerror", "cjx_parse_device", "cjx_device_wait", "cjx_context_stream",
     "cjx_version", "cjx_context_set_timing", "cjx_context_timings",
     "cjx_parse_batch", "cjx_free_batch_result", "cjx_parse_batch_device", "cjx_batch_wait",
+    "cjx_parse_lines_device",
 )
 
 
@@ -97,7 +98,8 @@ class _Timings(ctypes.Structure):
 
 class _DeviceStatus(ctypes.Structure):
     _fields_ = [("n_structural", ctypes.c_int64), ("n_brackets", ctypes.c_int64),
-                ("error", ctypes.c_int32), ("capacity_exceeded", ctypes.c_int32)]
+                ("error", ctypes.c_int32), ("capacity_exceeded", ctypes.c_int32),
+                ("n_records", ctypes.c_int64)]
 
 
 class _DocResult(ctypes.Structure):
@@ -169,6 +171,8 @@ def lib():
     L.cjx_batch_wait.argtypes = [vp, ctypes.POINTER(_DocResult), sz,
                                  ctypes.POINTER(_DeviceStatus)]
     L.cjx_batch_wait.restype = ctypes.c_int
+    L.cjx_parse_lines_device.argtypes = [vp, vp, szp, szp, sz, ctypes.c_uint, vp, vp, sz, vp, sz]
+    L.cjx_parse_lines_device.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -243,6 +247,23 @@ class Context:
                                             k, flags, ctypes.c_void_p(d_structural),
                                             ctypes.c_void_p(d_pair), cap)
 
+    def parse_lines_device(self, d_input: int, chunk_off, chunk_size, d_structural: int,
+                           d_pair: int, cap: int, d_records: int = 0, max_records: int = 0,
+                           flags: int = 0) -> int:
+        """Enqueue device-resident JSON Lines chunks (offsets 16-byte aligned);
+        with d_records the per-record table goes there (CJX_FLAG_RECORDS is set)."""
+        offs = np.ascontiguousarray(chunk_off, dtype=np.uint64)
+        sizes = np.ascontiguousarray(chunk_size, dtype=np.uint64)
+        self._batch_n = offs.size
+        szp = ctypes.POINTER(ctypes.c_size_t)
+        if d_records:
+            flags |= CJX_FLAG_RECORDS
+        return lib().cjx_parse_lines_device(self._h, ctypes.c_void_p(d_input),
+                                            offs.ctypes.data_as(szp), sizes.ctypes.data_as(szp),
+                                            offs.size, flags, ctypes.c_void_p(d_structural),
+                                            ctypes.c_void_p(d_pair), cap,
+                                            ctypes.c_void_p(d_records), max_records)
+
     def batch_wait(self):
         """(rc, docs, N, M, overflow): docs is an int64 array [k, 4] of
         (byte_base, struct_begin, struct_end, error)."""
@@ -253,6 +274,7 @@ class Context:
                                   ctypes.byref(st))
         docs = arr[:k].copy()
         docs[:, 3] = docs[:, 3] & 0xFFFFFFFF  # error (int32) then the reserved field
+        self.last_records = int(st.n_records)
         return rc, docs, st.n_structural, st.n_brackets, bool(st.capacity_exceeded)
 
 

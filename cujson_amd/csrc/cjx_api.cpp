@@ -475,7 +475,7 @@ int collect_records(cjx_context *ctx, uint32_t n_units, cjx_result *out) {
     LaunchBufs b = bufs_of(ctx);
     if (!ctx->rec_count.ensure((size_t)n_units * 4 + 16)) return CJX_ERR_ALLOC;
     if (launch_records(s, b, ctx->in.as<uint8_t>(), n_units, ctx->structural.as<uint32_t>(),
-                       ctx->rec_count.as<uint32_t>(), nullptr, false) != hipSuccess)
+                       ctx->rec_count.as<uint32_t>(), nullptr, 0, false) != hipSuccess)
         return CJX_ERR_DEVICE;
     int rc = read_status(ctx, n_units);
     if (rc) return rc;
@@ -487,7 +487,7 @@ int collect_records(cjx_context *ctx, uint32_t n_units, cjx_result *out) {
     if (hipHostMalloc((void **)&h, R * sizeof(cjx_record), hipHostMallocDefault) != hipSuccess)
         return CJX_ERR_ALLOC;
     if (launch_records(s, b, ctx->in.as<uint8_t>(), n_units, ctx->structural.as<uint32_t>(),
-                       ctx->rec_count.as<uint32_t>(), ctx->recs.as<RecordOut>(), true) !=
+                       ctx->rec_count.as<uint32_t>(), ctx->recs.as<RecordOut>(), R, true) !=
             hipSuccess ||
         hipMemcpyAsync(h, ctx->recs.p, R * sizeof(cjx_record), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
@@ -625,6 +625,11 @@ int parse_host_batch(cjx_context *ctx, const uint8_t *data, const std::vector<ui
     out->error = first_err;
     return CJX_OK;
 }
+
+int parse_units_device(cjx_context *ctx, const uint8_t *d_input, const size_t *doc_off,
+                       const size_t *doc_size, size_t n_docs, unsigned flags,
+                       int32_t *d_structural, int32_t *d_pair, size_t cap,
+                       cjx_record *d_records, size_t max_records);
 
 }  // namespace
 
@@ -927,8 +932,35 @@ void cjx_free_batch_result(cjx_batch_result *r) {
 int cjx_parse_batch_device(cjx_context *ctx, const uint8_t *d_input, const size_t *doc_off,
                            const size_t *doc_size, size_t n_docs, unsigned flags,
                            int32_t *d_structural, int32_t *d_pair, size_t cap) {
+    return parse_units_device(ctx, d_input, doc_off, doc_size, n_docs,
+                              flags & ~(CJX_FLAG_JSONL | CJX_FLAG_RECORDS), d_structural, d_pair,
+                              cap, nullptr, 0);
+}
+
+int cjx_parse_lines_device(cjx_context *ctx, const uint8_t *d_input, const size_t *chunk_off,
+                           const size_t *chunk_size, size_t n_chunks, unsigned flags,
+                           int32_t *d_structural, int32_t *d_pair, size_t cap,
+                           cjx_record *d_records, size_t max_records) {
+    return parse_units_device(ctx, d_input, chunk_off, chunk_size, n_chunks,
+                              flags | CJX_FLAG_JSONL, d_structural, d_pair, cap, d_records,
+                              max_records);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Units at 16-byte aligned offsets of a device buffer.  Independent
+// documents (no JSONL flag): positions and pair values local to each unit.
+// JSON Lines chunks: positions count from the first chunk (lastChunkIndex)
+// and pair values are numbered across chunks, as in the host layout.
+int parse_units_device(cjx_context *ctx, const uint8_t *d_input, const size_t *doc_off,
+                       const size_t *doc_size, size_t n_docs, unsigned flags,
+                       int32_t *d_structural, int32_t *d_pair, size_t cap,
+                       cjx_record *d_records, size_t max_records) {
     if (!ctx || !d_input || !doc_off || !doc_size || n_docs == 0 || !d_structural || !d_pair)
         return CJX_ERR_INPUT;
+    const bool jsonl = (flags & CJX_FLAG_JSONL) != 0;
     int rc = set_device(ctx);
     if (rc) return rc;
     std::vector<UnitDesc> units(n_docs);
@@ -938,7 +970,7 @@ int cjx_parse_batch_device(cjx_context *ctx, const uint8_t *d_input, const size_
         UnitDesc &u = units[i];
         u.in_off = doc_off[i];
         u.n = doc_size[i];
-        u.pos_base = 0;
+        u.pos_base = jsonl ? total : 0;
         total += doc_size[i];
         full += doc_size[i];
     }
@@ -949,19 +981,36 @@ int cjx_parse_batch_device(cjx_context *ctx, const uint8_t *d_input, const size_
     // all, else the n/3 + 256 guess (an overflow reports CJX_ERR_CAPACITY).
     const bool exact = full <= cap;
     if (set_regions(units, exact) > cap) return CJX_ERR_CAPACITY;
-    Shape sh{(uint32_t)tiles, (uint32_t)n_docs, n_small, 0, 0, 0, 0, false};
+    Shape sh{(uint32_t)tiles, (uint32_t)n_docs, n_small, 0, 0, 0, 0, jsonl};
     shape_rows(sh, units);
     if ((rc = ensure_work(ctx, sh))) return rc;
     if ((rc = upload_units(ctx, units, (uint32_t)tiles, n_small))) return rc;
     ctx->n_units = (uint32_t)n_docs;
     ctx->batch_off.assign(doc_off, doc_off + n_docs);
     (void)hipEventRecord(ctx->ev0, ctx->stream);
-    rc = enqueue_maybe_graph(ctx, d_input, sh, flags & ~CJX_FLAG_JSONL,
-                       reinterpret_cast<uint32_t *>(d_structural), d_pair);
+    rc = enqueue_maybe_graph(ctx, d_input, sh, flags & ~CJX_FLAG_RECORDS,
+                             reinterpret_cast<uint32_t *>(d_structural), d_pair);
+    if (!rc && jsonl && (flags & CJX_FLAG_RECORDS) && d_records) {
+        LaunchBufs b = bufs_of(ctx);
+        if (!ctx->rec_count.ensure(n_docs * 4 + 16)) return CJX_ERR_ALLOC;
+        b = bufs_of(ctx);
+        if (launch_records(ctx->stream, b, d_input, (uint32_t)n_docs,
+                           reinterpret_cast<uint32_t *>(d_structural),
+                           ctx->rec_count.as<uint32_t>(), nullptr, 0, false) != hipSuccess ||
+            launch_records(ctx->stream, b, d_input, (uint32_t)n_docs,
+                           reinterpret_cast<uint32_t *>(d_structural),
+                           ctx->rec_count.as<uint32_t>(), reinterpret_cast<RecordOut *>(d_records),
+                           max_records, true) != hipSuccess)
+            return CJX_ERR_DEVICE;
+    }
     (void)hipEventRecord(ctx->ev1, ctx->stream);
     ctx->events_recorded = true;
     return rc;
 }
+
+}  // namespace
+
+extern "C" {
 
 int cjx_batch_wait(cjx_context *ctx, cjx_doc_result *docs, size_t n_docs, cjx_device_status *st) {
     if (!ctx || ctx->n_units == 0) return CJX_ERR_INPUT;
@@ -985,9 +1034,10 @@ int cjx_batch_wait(cjx_context *ctx, cjx_doc_result *docs, size_t n_docs, cjx_de
         st->n_structural = ls->n_total;
         st->n_brackets = ls->m_total;
         st->error = code;
-        st->capacity_exceeded = ls->overflow ? 1 : 0;
+        st->capacity_exceeded = (ls->overflow || ls->rec_overflow) ? 1 : 0;
+        st->n_records = ls->n_records;
     }
-    return ls->overflow ? CJX_ERR_CAPACITY : CJX_OK;
+    return (ls->overflow || ls->rec_overflow) ? CJX_ERR_CAPACITY : CJX_OK;
 }
 
 }  // extern "C"

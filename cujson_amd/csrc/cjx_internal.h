@@ -88,8 +88,8 @@ struct LaunchState {
     uint32_t n_total, m_total;
     uint32_t n_chunks;  // matching chunks of all units
     uint32_t n_big;     // units with more than 64 matching chunks (listed in big_units)
-    uint32_t n_records; // JSON Lines records (cjx_records.hip)
-    uint32_t pad;
+    uint32_t n_records;     // JSON Lines records (cjx_records.hip)
+    uint32_t rec_overflow;  // more records than the caller's table holds
 };
 
 // One JSON Lines record (cjx_records.hip); same layout as cjx_record.
@@ -162,7 +162,7 @@ hipError_t launch_sort_units(hipStream_t s, const LaunchBufs &b, uint32_t n_unit
 // total); write = true fills rec.
 hipError_t launch_records(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
                           uint32_t n_units, const uint32_t *d_struct, uint32_t *rec_count,
-                          RecordOut *rec, bool write);
+                          RecordOut *rec, uint64_t max_records, bool write);
 hipError_t launch_phase3(hipStream_t s, const LaunchBufs &b, uint32_t u, uint32_t G,
                          uint32_t pair_grid, bool depth_u16, int32_t *pair_row);
 

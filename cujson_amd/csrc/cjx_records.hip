@@ -94,7 +94,8 @@ __global__ __launch_bounds__(kRecThreads) void k_rec_write(const uint8_t *__rest
                                                            const uint32_t *__restrict__ st,
                                                            const uint32_t *__restrict__ rec_base,
                                                            RecordOut *__restrict__ rec,
-                                                           const LaunchState *__restrict__ ls) {
+                                                           uint64_t max_records,
+                                                           LaunchState *__restrict__ ls) {
     __shared__ uint32_t wcnt[kRecThreads / 64];
     __shared__ int32_t wmax[kRecThreads / 64];
     __shared__ uint32_t s_count;
@@ -146,7 +147,8 @@ __global__ __launch_bounds__(kRecThreads) void k_rec_write(const uint8_t *__rest
             r.struct_last = (int32_t)(t + hb);
             r.chunk = (int32_t)u;
             r.error = code;
-            rec[k + before] = r;
+            if (k + before < max_records) rec[k + before] = r;
+            else atomicOr(&ls->rec_overflow, 1u);
         }
         __syncthreads();
         if (threadIdx.x == kRecThreads - 1) {
@@ -169,13 +171,14 @@ __global__ __launch_bounds__(kRecThreads) void k_rec_write(const uint8_t *__rest
         r.struct_last = (int32_t)((ne - 1) + hb);
         r.chunk = (int32_t)u;
         r.error = code;
-        rec[k + s_count] = r;
+        if (k + s_count < max_records) rec[k + s_count] = r;
+        else atomicOr(&ls->rec_overflow, 1u);
     }
 }
 
 hipError_t launch_records(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
                           uint32_t n_units, const uint32_t *d_struct, uint32_t *rec_count,
-                          RecordOut *rec, bool write) {
+                          RecordOut *rec, uint64_t max_records, bool write) {
     if (!write) {
         hipLaunchKernelGGL(k_rec_count, dim3(n_units), dim3(kRecThreads), 0, s, d_in, b.units,
                            b.uout, d_struct, rec_count, b.ls);
@@ -183,7 +186,7 @@ hipError_t launch_records(hipStream_t s, const LaunchBufs &b, const uint8_t *d_i
                            b.ls);
     } else {
         hipLaunchKernelGGL(k_rec_write, dim3(n_units), dim3(kRecThreads), 0, s, d_in, b.units,
-                           b.uout, d_struct, rec_count, rec, b.ls);
+                           b.uout, d_struct, rec_count, rec, max_records, b.ls);
     }
     return hipGetLastError();
 }

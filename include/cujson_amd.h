@@ -147,6 +147,7 @@ typedef struct cjx_device_status {
     int64_t n_brackets;
     int32_t error;
     int32_t capacity_exceeded;
+    int64_t n_records; /* JSON Lines with CJX_FLAG_RECORDS */
 } cjx_device_status;
 
 /* Parse a device-resident document.
@@ -234,6 +235,20 @@ int cjx_parse_batch_device(cjx_context *ctx, const uint8_t *d_input, const size_
                            int32_t *d_structural, int32_t *d_pair, size_t cap);
 int cjx_batch_wait(cjx_context *ctx, cjx_doc_result *docs, size_t n_docs,
                    cjx_device_status *st);
+
+/* Device-resident JSON Lines (no counterpart in the reference, whose
+ * parse_json_lines copies each chunk in, parse_json_lines.cu:1160-1230):
+ * line-aligned chunks placed at 16-byte aligned offsets of d_input, parsed
+ * in one launch sequence with the reference's carries (positions count from
+ * the first chunk as if the chunks were contiguous, pair values are numbered
+ * across chunks).  Rows stay in per-chunk regions (cjx_batch_wait gives each
+ * chunk's range; row_base ordering follows chunk order).  With
+ * CJX_FLAG_RECORDS and d_records, the per-record table (host-layout
+ * structural indices) is written to d_records[0..max_records). */
+int cjx_parse_lines_device(cjx_context *ctx, const uint8_t *d_input, const size_t *chunk_off,
+                           const size_t *chunk_size, size_t n_chunks, unsigned flags,
+                           int32_t *d_structural, int32_t *d_pair, size_t cap,
+                           cjx_record *d_records, size_t max_records);
 
 #ifdef __cplusplus
 }

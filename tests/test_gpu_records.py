@@ -83,3 +83,52 @@ def test_records_edge_cases(ctx):
             if any(s == 0 for _, s in chunks):
                 continue
             check_records(ctx, d, chunks, f"{d!r} cc={cc}")
+
+
+def test_device_lines_match_host_lines(ctx):
+    """cjx_parse_lines_device on chunks placed at 16-byte aligned offsets:
+    per-chunk rows, pair values (numbered across chunks) and records equal the
+    host path's."""
+    from cujson_amd.hip import DeviceBuffer
+    d = bytes(gen.lines(3000))
+    chunks = oracle.chunks_by_bytes(d, 8192)
+    o = oracle.parse_lines(d, chunks)
+    assert o.code == 0
+    offs, pos = [], 0
+    for _, size in chunks:
+        offs.append(pos)
+        pos += -(-size // 16) * 16
+    buf = np.zeros(pos + 64, np.uint8)
+    for (a, size), off in zip(chunks, offs):
+        buf[off:off + size] = np.frombuffer(d[a:a + size], np.uint8)
+    dev = DeviceBuffer(buf.size)
+    dev.upload(buf)
+    cap = len(d) + 64
+    st, pr = DeviceBuffer(cap * 4), DeviceBuffer(cap * 4)
+    max_rec = 10000
+    rec = DeviceBuffer(max_rec * cj.RECORD_DTYPE.itemsize)
+    assert ctx.parse_lines_device(dev.ptr, offs, [s for _, s in chunks], st.ptr, pr.ptr, cap,
+                                  rec.ptr, max_rec) == 0
+    rc, docs, N, M, over = ctx.batch_wait()
+    assert rc == 0 and not over and N == o.N
+    srow, prow = st.download(np.int32, cap), pr.download(np.int32, cap)
+    packed_s = np.concatenate([srow[b:e] for _, b, e, _ in docs])
+    packed_p = np.concatenate([prow[b:e] for _, b, e, _ in docs])
+    assert np.array_equal(packed_s, o.structural[1:N + 1])
+    assert np.array_equal(packed_p, o.pair_pos[1:N + 1])
+    R = ctx.last_records
+    got = np.frombuffer(rec.download(np.uint8, R * cj.RECORD_DTYPE.itemsize).tobytes(),
+                        dtype=cj.RECORD_DTYPE)
+    want = expected_records(d, chunks, o)
+    got6 = np.stack([got[f].astype(np.int64) for f in
+                     ("byte_begin", "byte_end", "struct_first", "struct_last", "chunk", "error")],
+                    axis=1)
+    assert np.array_equal(got6, want)
+    # a table that is too small is reported, never overrun
+    small = DeviceBuffer(16 * cj.RECORD_DTYPE.itemsize + 64)
+    small.fill(0x5A)
+    assert ctx.parse_lines_device(dev.ptr, offs, [s for _, s in chunks], st.ptr, pr.ptr, cap,
+                                  small.ptr, 16) == 0
+    rc, docs, N, M, over = ctx.batch_wait()
+    assert rc == cj.CJX_ERR_CAPACITY and over
+    assert np.all(small.download(np.uint8, 64, 16 * cj.RECORD_DTYPE.itemsize) == 0x5A)
