@@ -136,6 +136,101 @@ def many_docs_bench(cj, gen, n_docs: int, reps: int):
     }
 
 
+def lines_bench(cj, gen, n_records: int, reps: int):
+    """8M JSON Lines records (log-normal sizes, median 400 B, sigma 1.2,
+    clipped to 64 KiB, depth 1-6), device-resident, in launches of at most
+    2^31 - 1 bytes (positions are int32 in the host layout).  Batched: every
+    chunk of a launch in one launch sequence with the per-record table, the
+    chunk size picked by the autotuner.  Plain: one launch per chunk, the
+    reference's sequential chunk loop (parse_json_lines.cu:1160-1230)."""
+    from cujson_amd import autotune
+    from cujson_amd.hip import DeviceBuffer, Event, synchronize
+    data = gen.lines_parallel(n_records)
+    total = int(data.size)
+    ctx = cj.Context(0)
+    sample = data[: min(total, 256 << 20)]
+    cut = int(np.flatnonzero(sample == 10)[-1]) + 1
+    best, table = autotune.tune_lines_chunk(ctx, sample[:cut], signature="config3",
+                                            use_cache=False)
+    offs, sizes = cj.chunk_lines(data, 1, best)
+    offs, sizes = np.array(offs, np.int64), np.array(sizes, np.int64)
+    limit = (1 << 31) - 1
+    parts, start, acc = [], 0, 0
+    for i, sz in enumerate(sizes):
+        if acc + sz > limit:
+            parts.append((start, i))
+            start, acc = i, 0
+        acc += int(sz)
+    parts.append((start, len(sizes)))
+    launches = []
+    for a, b in parts:
+        buf, aoffs = autotune.aligned_layout(data, offs[a:b], sizes[a:b])
+        dev = DeviceBuffer(buf.size)
+        dev.upload(buf)
+        nbytes = int(sizes[a:b].sum())
+        cap = nbytes // 3 + 256 * (b - a) + 64
+        max_rec = int(np.count_nonzero(data[offs[a]:offs[a] + nbytes] == 10)) + (b - a)
+        launches.append(dict(dev=dev, offs=aoffs, sizes=sizes[a:b], cap=cap,
+                             st=DeviceBuffer(cap * 4), pr=DeviceBuffer(cap * 4), max_rec=max_rec,
+                             rec=DeviceBuffer(max_rec * cj.RECORD_DTYPE.itemsize)))
+        del buf
+    stream = ctx.stream
+
+    def batched():
+        for L in launches:
+            rc = ctx.parse_lines_device(L["dev"].ptr, L["offs"], L["sizes"], L["st"].ptr,
+                                        L["pr"].ptr, L["cap"], L["rec"].ptr, L["max_rec"])
+            if rc:
+                raise RuntimeError(f"parse_lines_device: {cj.strerror(rc)}")
+
+    batched()
+    synchronize()
+    nrec = 0
+    for L in launches:  # each launch's status (the last launch is the one a wait reads)
+        ctx.parse_lines_device(L["dev"].ptr, L["offs"], L["sizes"], L["st"].ptr, L["pr"].ptr,
+                               L["cap"], L["rec"].ptr, L["max_rec"])
+        rc, docs, N, M, over = ctx.batch_wait()
+        if rc or over:
+            raise RuntimeError(f"lines launch failed rc={rc} overflow={over}")
+        nrec += ctx.last_records
+    ctx.set_timing(True)
+    for _ in range(reps):
+        batched()
+    tim = ctx.timings(reps * len(launches))
+    ctx.set_timing(False)
+    synchronize()
+    per = np.array([t[2] for t in tim]).reshape(reps, len(launches)).sum(axis=1)
+    batched_ms = float(np.median(per))
+    # plain: one launch per chunk
+    e0, e1 = Event(), Event()
+    synchronize()
+    e0.record(stream)
+    n_chunks = 0
+    for L in launches:
+        for off, sz in zip(L["offs"], L["sizes"]):
+            rc = ctx.parse_lines_device(L["dev"].ptr + int(off), [0], [int(sz)], L["st"].ptr,
+                                        L["pr"].ptr, L["cap"])
+            if rc:
+                raise RuntimeError(f"plain chunk: {cj.strerror(rc)}")
+            n_chunks += 1
+    e1.record(stream)
+    e1.synchronize()
+    plain_ms = e0.elapsed_ms(e1)
+    ctx.close()
+    return {
+        "workload": f"{n_records} JSON Lines records, log-normal sizes (median 400 B, sigma 1.2, "
+                    f"clipped to 64 KiB), depth 1-6; device-resident; launches of < 2^31 bytes",
+        "records": int(nrec), "bytes": total, "launches": len(launches), "chunks": n_chunks,
+        "chunk_bytes": int(best), "autotune_ms": {str(k): round(v, 3) for k, v in table.items()},
+        "batched_ms": round(batched_ms, 3),
+        "batched_gbs": round(total / (batched_ms * 1e-3) / 1e9, 2),
+        "batched_form": "all chunks of a launch in one launch sequence, with the per-record table",
+        "plain_ms": round(plain_ms, 3),
+        "plain_gbs": round(total / (plain_ms * 1e-3) / 1e9, 2),
+        "batched_vs_plain": round(plain_ms / batched_ms, 2),
+    }
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -184,6 +279,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 50,000-document batched-vs-plain measurement")
     ap.add_argument("--docs", type=int, default=50_000)
+    ap.add_argument("--no-lines", action="store_true",
+                    help="skip the JSON Lines batched-vs-plain measurement")
+    ap.add_argument("--records", type=int, default=8_000_000)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only copy this many bytes device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -303,6 +401,9 @@ def main():
     if not args.no_batched:
         log(f"batched vs plain on {args.docs} documents")
         out["batched"] = many_docs_bench(cj, gen, args.docs, reps=5)
+    if not args.no_lines:
+        log(f"JSON Lines batched vs plain on {args.records} records")
+        out["lines"] = lines_bench(cj, gen, args.records, reps=3)
     if not args.no_cpu_baseline:
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample_bytes)

@@ -167,3 +167,26 @@ def many_docs(n_docs=50_000, alpha=1.1, min_size=200, max_size=16 << 20, fault_f
     for i in range(n_docs):
         data[offs[i]:offs[i] + sizes[i]] = docs[i]
     return data, offs, sizes, np.array(kinds, dtype=np.int64)
+
+
+def lines_parallel(n_records, median=400.0, sigma=1.2, max_size=64 * 1024,
+                   seed=SEED_BASE + 3, blocks=64, threads=None) -> np.ndarray:
+    """lines() for large record counts: `blocks` independent record streams
+    (seeds seed + k) generated on several threads and concatenated."""
+    from concurrent.futures import ThreadPoolExecutor
+    per = [n_records // blocks + (1 if k < n_records % blocks else 0) for k in range(blocks)]
+    L = lib()
+
+    def work(k):
+        cap = int(per[k] * max(median * 8, 2048)) + (1 << 20)
+        out = np.empty(cap, dtype=np.uint8)
+        n = L.cjx_gen_lines(seed + 1000 + k, per[k], median, sigma, 200, max_size,
+                            out.ctypes.data, cap)
+        if n == 0 and per[k]:
+            raise RuntimeError("generator buffer too small")
+        return out[:n]
+
+    threads = threads or min(16, os.cpu_count() or 4)
+    with ThreadPoolExecutor(threads) as ex:
+        parts = list(ex.map(work, range(blocks)))
+    return np.concatenate(parts)
