@@ -231,6 +231,27 @@ def lines_bench(cj, gen, n_records: int, reps: int):
     }
 
 
+def serving_bench(seconds: float, rate: float, plain_seconds: float) -> dict:
+    """Open-loop serving run (libcjx_loadgen.so, in this process): Poisson
+    arrivals at `rate` req/s for `seconds` s, sizes 80% 1-16 KiB, 19%
+    64 KiB-4 MiB, 1% 64-256 MiB, payloads in pinned host memory, 64 GB
+    admission budget over 4 slots; then the plain one-document-per-call path
+    on the same request sequence for `plain_seconds` s."""
+    import ctypes
+    L = ctypes.CDLL(os.path.join(ROOT, "cujson_amd", "libcjx_loadgen.so"))
+    L.cjx_loadgen_run.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
+    L.cjx_loadgen_run.restype = ctypes.c_int
+    buf = ctypes.create_string_buffer(16384)
+    a = f"rate={rate} seconds={seconds} plain_seconds={plain_seconds} budget={64e9:.0f} slots=4"
+    rc = L.cjx_loadgen_run(a.encode(), buf, len(buf))
+    out = json.loads(buf.value.decode() or "{}")
+    out["rc"] = int(rc)
+    out["payloads"] = "pinned host memory (every request crosses the host link both ways)"
+    if out.get("plain_GBps"):
+        out["sustained_vs_plain"] = round(out["sustained_GBps"] / out["plain_GBps"], 2)
+    return out
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -282,6 +303,11 @@ def main():
     ap.add_argument("--no-lines", action="store_true",
                     help="skip the JSON Lines batched-vs-plain measurement")
     ap.add_argument("--records", type=int, default=8_000_000)
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the open-loop serving run")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-rate", type=float, default=20000.0)
+    ap.add_argument("--serve-plain-seconds", type=float, default=10.0)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only copy this many bytes device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -404,6 +430,10 @@ def main():
     if not args.no_lines:
         log(f"JSON Lines batched vs plain on {args.records} records")
         out["lines"] = lines_bench(cj, gen, args.records, reps=3)
+    if not args.no_serving:
+        log(f"serving: {args.serve_rate:.0f} req/s for {args.serve_seconds:.0f} s")
+        out["serving"] = serving_bench(args.serve_seconds, args.serve_rate,
+                                       args.serve_plain_seconds)
     if not args.no_cpu_baseline:
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample_bytes)

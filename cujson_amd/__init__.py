@@ -57,7 +57,8 @@ ABI_SYMBOLS = (
     "cjx_strerror", "cjx_parse_device", "cjx_device_wait", "cjx_context_stream",
     "cjx_version", "cjx_context_set_timing", "cjx_context_timings",
     "cjx_parse_batch", "cjx_free_batch_result", "cjx_parse_batch_device", "cjx_batch_wait",
-    "cjx_parse_lines_device",
+    "cjx_parse_lines_device", "cjx_server_create", "cjx_server_submit", "cjx_server_wait",
+    "cjx_server_release", "cjx_server_get_stats", "cjx_server_destroy",
 )
 
 
@@ -114,6 +115,19 @@ class _BatchResult(ctypes.Structure):
                 ("n_structural", ctypes.c_int64), ("n_docs", ctypes.c_int64),
                 ("docs", ctypes.POINTER(_DocResult)), ("error", ctypes.c_int32),
                 ("reserved", ctypes.c_int32), ("device_ms", ctypes.c_double)]
+
+
+class _Completion(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("buf", ctypes.POINTER(ctypes.c_int32)),
+                ("owner", ctypes.c_void_p), ("n_structural", ctypes.c_int64),
+                ("error", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("batches", "documents", "bytes",
+                                                "max_inflight_bytes", "relaunches", "pending",
+                                                "region_bytes")]
 
 
 _lib = None
@@ -173,6 +187,19 @@ def lib():
     L.cjx_batch_wait.restype = ctypes.c_int
     L.cjx_parse_lines_device.argtypes = [vp, vp, szp, szp, sz, ctypes.c_uint, vp, vp, sz, vp, sz]
     L.cjx_parse_lines_device.restype = ctypes.c_int
+    L.cjx_server_create.argtypes = [ctypes.c_int, sz, ctypes.c_int, sz, ctypes.POINTER(vp)]
+    L.cjx_server_create.restype = ctypes.c_int
+    L.cjx_server_submit.argtypes = [vp, vp, sz, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint64)]
+    L.cjx_server_submit.restype = ctypes.c_int
+    L.cjx_server_wait.argtypes = [vp, ctypes.POINTER(_Completion), sz, ctypes.POINTER(sz),
+                                  ctypes.c_int]
+    L.cjx_server_wait.restype = ctypes.c_int
+    L.cjx_server_release.argtypes = [vp, ctypes.POINTER(_Completion)]
+    L.cjx_server_release.restype = None
+    L.cjx_server_get_stats.argtypes = [vp, ctypes.POINTER(_ServerStats)]
+    L.cjx_server_get_stats.restype = ctypes.c_int
+    L.cjx_server_destroy.argtypes = [vp]
+    L.cjx_server_destroy.restype = None
     _lib = L
     return L
 
@@ -516,3 +543,73 @@ def parse_batch(data, doc_off, doc_size, flags: int = 0,
     rc = lib().cjx_parse_batch(ctx.handle, p, offs.ctypes.data_as(szp), sizes.ctypes.data_as(szp),
                                offs.size, flags, ctypes.byref(r))
     return BatchResult(r, rc)
+
+
+class Server:
+    """Request server with continuous batching (cjx_server_* in the C ABI).
+
+    submit() queues one standard-JSON document and returns its ticket;
+    completions() returns finished requests as (ticket, error, buf,
+    latency_ms, queue_ms) with buf the reference host layout (a numpy copy).
+    The submitted bytes are kept referenced until their completion."""
+
+    def __init__(self, device: int = 0, budget: int = 8 << 30, slots: int = 4,
+                 max_batch: int = 4096):
+        h = ctypes.c_void_p()
+        rc = lib().cjx_server_create(device, budget, slots, max_batch, ctypes.byref(h))
+        if rc:
+            raise RuntimeError(f"cjx_server_create: {strerror(rc)}")
+        self.handle = h
+        self._live = {}
+
+    def submit(self, data, flags: int = 0) -> int:
+        a = np.ascontiguousarray(np.frombuffer(data, dtype=np.uint8)
+                                 if isinstance(data, (bytes, bytearray)) else data,
+                                 dtype=np.uint8)
+        t = ctypes.c_uint64()
+        rc = lib().cjx_server_submit(self.handle, a.ctypes.data, a.size, flags, ctypes.byref(t))
+        if rc:
+            raise RuntimeError(f"cjx_server_submit: {strerror(rc)}")
+        self._live[t.value] = a
+        return t.value
+
+    def completions(self, max_n: int = 1024, timeout_ms: int = 100):
+        arr = (_Completion * max_n)()
+        n = ctypes.c_size_t()
+        L = lib()
+        L.cjx_server_wait(self.handle, arr, max_n, ctypes.byref(n), timeout_ms)
+        out = []
+        for i in range(n.value):
+            c = arr[i]
+            buf = None
+            if c.buf:
+                cells = 2 * (int(c.n_structural) + 2)
+                buf = np.ctypeslib.as_array(c.buf, shape=(cells,)).copy()
+            out.append((int(c.ticket), int(c.error), buf, float(c.latency_ms),
+                        float(c.queue_ms)))
+            L.cjx_server_release(self.handle, ctypes.byref(c))
+            self._live.pop(int(c.ticket), None)
+        return out
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        lib().cjx_server_get_stats(self.handle, ctypes.byref(st))
+        return {k: int(getattr(st, k)) for k, _ in _ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().cjx_server_destroy(self.handle)
+            self.handle = None
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -14,7 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcujson_amd.so")
 ARCH = os.environ.get("CJX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["cjx_tokenize.hip", "cjx_utf8.hip", "cjx_kernels.hip", "cjx_match.hip", "cjx_records.hip", "cjx_api.cpp"]
+SOURCES = ["cjx_tokenize.hip", "cjx_utf8.hip", "cjx_kernels.hip", "cjx_match.hip", "cjx_records.hip", "cjx_api.cpp",
+           "cjx_server.cpp"]
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     # packed fp32 VALU ops are disabled for every HIP object of this library
@@ -81,6 +82,19 @@ def build_wrapper_check() -> str:
     return WRAPPER_CHECK
 
 
+LOADGEN_LIB = os.path.join(HERE, "libcjx_loadgen.so")
+
+
+def build_loadgen() -> str:
+    """The serving load driver (bench tool) over the library and the generator."""
+    tmp = LOADGEN_LIB + ".tmp"
+    _run(["hipcc", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
+          "-o", tmp, os.path.join(CSRC, "cjx_loadgen.cpp"), "-L", HERE, "-lcujson_amd",
+          "-lcjx_gen", "-Wl,-rpath,$ORIGIN", "-lpthread"])
+    os.replace(tmp, LOADGEN_LIB)
+    return LOADGEN_LIB
+
+
 def build_oracle() -> None:
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"], check=True)
 
@@ -89,4 +103,5 @@ if __name__ == "__main__":
     print(build_library(verbose=True))
     print(build_generator())
     print(build_wrapper_check())
+    print(build_loadgen())
     build_oracle()

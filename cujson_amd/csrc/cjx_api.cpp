@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/cujson_amd.h"
+#include "cjx_host.h"
 #include "cjx_internal.h"
 
 using namespace cjx;
@@ -22,102 +23,7 @@ using namespace cjx;
 static_assert(sizeof(RecordOut) == sizeof(cjx_record), "record layout");
 static_assert(sizeof(cjx_doc_result) == 32, "document result layout");
 
-namespace {
-
-constexpr uint64_t kMaxLaunchBytes = 0x7FFFFFFFull;  // positions and indices stay in int32
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    // Grows (never shrinks); contents are not preserved.
-    bool ensure(size_t bytes) {
-        if (bytes <= cap) return true;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        size_t want = bytes + bytes / 8 + 4096;
-        if (hipMalloc(&p, want) != hipSuccess) {
-            p = nullptr;
-            return false;
-        }
-        cap = want;
-        return true;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T> T *as() const { return static_cast<T *>(p); }
-};
-
-struct PinnedBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    bool ensure(size_t bytes) {
-        if (bytes <= cap) return true;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
-            p = nullptr;
-            return false;
-        }
-        cap = bytes;
-        return true;
-    }
-    void release() {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-}  // namespace
-
-struct cjx_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // Timing ring: event triples (tokenizer start, tokenizer end, phase-3 end)
-    // for the last kTimingRing parses while timing is enabled.
-    static constexpr int kTimingRing = 256;
-    hipEvent_t tev[kTimingRing][3] = {};
-    uint64_t timing_count = 0;
-    bool timing = false;
-    DevBuf in, structural, pair;
-    DevBuf units, uout, ls, tile_st, layout, chunk_unit, big_units, unit_part, rec_count, recs;
-    uint32_t *tile_unit = nullptr;    // inside `units`, after the unit descriptors
-    uint32_t *small_units = nullptr;  // after the tile -> unit map
-    // Pinned upload slots for unit tables: a call may return before its copy
-    // has run, so the next calls use other slots; a slot is reused only after
-    // the event recorded behind its copy has completed.
-    static constexpr int kUploadSlots = 8;
-    PinnedBuf up[kUploadSlots];
-    hipEvent_t up_ev[kUploadSlots] = {};
-    uint64_t up_count = 0;
-    DevBuf oc_char, oc_idx, keys, sorted, tmp, spos, block_delta, hist;
-    DevBuf ck, gmin, gmin1, gmin2, gagg, gd0, lows, highs;
-    PinnedBuf h_units, h_uout, h_ls, h_stage;
-    std::vector<uint64_t> batch_off;  // byte offsets of the documents of the batch in flight
-    // Shape of the launch in flight (device form).
-    uint32_t n_units = 0;
-    bool events_recorded = false;
-    // Captured launch sequence of the last graph-mode call and what it baked in.
-    hipGraphExec_t graph = nullptr;
-    struct GraphKey {
-        LaunchBufs b;
-        const uint8_t *d_in;
-        uint32_t *d_struct;
-        int32_t *d_pair;
-        uint32_t ntiles, n_units, n_small;
-        uint64_t rows, max_rows, chunks;
-        unsigned flags;
-        bool concat;
-    } graph_key{};
-};
-
-namespace {
+namespace cjx_host {
 
 int set_device(cjx_context *ctx) {
     return hipSetDevice(ctx->device) == hipSuccess ? CJX_OK : CJX_ERR_DEVICE;
@@ -189,14 +95,6 @@ uint64_t set_regions(std::vector<UnitDesc> &units, bool full) {
     return off;
 }
 
-struct Shape {
-    uint32_t ntiles, n_units, n_small;
-    uint64_t rows;      // total rows of all output regions
-    uint64_t max_rows;  // largest region
-    uint64_t chunks;    // bound on the matching chunks of all units
-    uint64_t groups;    // bound on their 64-chunk groups (per unit, rounded up)
-    bool concat;        // pair values numbered across units (JSON Lines), else per unit
-};
 
 // Fills the row totals of a shape from the units' regions.
 void shape_rows(Shape &sh, const std::vector<UnitDesc> &units) {
@@ -631,7 +529,9 @@ int parse_units_device(cjx_context *ctx, const uint8_t *d_input, const size_t *d
                        int32_t *d_structural, int32_t *d_pair, size_t cap,
                        cjx_record *d_records, size_t max_records);
 
-}  // namespace
+}  // namespace cjx_host
+
+using namespace cjx_host;
 
 extern "C" {
 
@@ -948,7 +848,7 @@ int cjx_parse_lines_device(cjx_context *ctx, const uint8_t *d_input, const size_
 
 }  // extern "C"
 
-namespace {
+namespace cjx_host {
 
 // Units at 16-byte aligned offsets of a device buffer.  Independent
 // documents (no JSONL flag): positions and pair values local to each unit.
@@ -1008,7 +908,7 @@ int parse_units_device(cjx_context *ctx, const uint8_t *d_input, const size_t *d
     return rc;
 }
 
-}  // namespace
+}  // namespace cjx_host
 
 extern "C" {
 

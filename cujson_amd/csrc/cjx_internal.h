@@ -153,6 +153,11 @@ hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, ui
 hipError_t launch_assemble(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
                            const uint32_t *d_struct, const int32_t *d_pair, int32_t *dst_struct,
                            int32_t *dst_pair, uint32_t N, bool host_layout);
+// Per-document host layouts of a batch: document u's 2(n_u + 2) cells start
+// at cell 2 row_base(u) + 4u of dst (2N + 4 n_units cells in all).
+hipError_t launch_assemble_docs(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
+                                const uint32_t *d_struct, const int32_t *d_pair, int32_t *dst,
+                                uint32_t N);
 // Sort-path phase 3 for every unit that is not well nested, one block per
 // unit (launches with several units).
 hipError_t launch_sort_units(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool depth_u16,

@@ -193,6 +193,50 @@ __global__ __launch_bounds__(256) void k_assemble(const UnitOut *__restrict__ uo
     }
 }
 
+// Per-document host layout: document u (rows [row_base, row_base + n_u) of
+// the packed order) becomes the 2(n_u + 2) cells
+//   [0, s_1..s_n, n + 1, p_1..p_n, -1, -1]   (parse_standard_json.cu:1659-1680)
+// starting at cell 2 row_base + 4u of dst, so every document of a batch is a
+// complete reference-layout buffer inside one copy.  Threads below n_units
+// also write their unit's four sentinel cells.
+__global__ __launch_bounds__(256) void k_assemble_docs(const UnitOut *__restrict__ uo,
+                                                       uint32_t n_units,
+                                                       const uint32_t *__restrict__ st,
+                                                       const int32_t *__restrict__ pr,
+                                                       int32_t *__restrict__ dst, uint32_t N) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_units) {
+        const uint64_t n = uo[t].n_end - uo[t].n_base;
+        int32_t *d = dst + 2 * (uint64_t)uo[t].row_base + 4 * t;
+        d[0] = 0;
+        d[n + 1] = (int32_t)(n + 1);
+        d[2 * n + 2] = -1;
+        d[2 * n + 3] = -1;
+    }
+    const uint64_t r0 = t * kAsmRows;
+    if (r0 >= N) return;
+    uint32_t lo = 0, hi = n_units;  // last unit with row_base <= r0
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (uo[mid].row_base <= r0) lo = mid;
+        else hi = mid;
+    }
+    uint32_t u = lo;
+    uint32_t ub = uo[u].row_base, un = uo[u].n_end - uo[u].n_base;
+    const uint64_t r1 = r0 + kAsmRows < N ? r0 + kAsmRows : N;
+    for (uint64_t r = r0; r < r1; r++) {
+        while (r >= (uint64_t)ub + un) {  // skips units without structurals
+            u++;
+            ub = uo[u].row_base;
+            un = uo[u].n_end - uo[u].n_base;
+        }
+        const uint64_t j = r - ub, src = uo[u].n_base + j;
+        int32_t *d = dst + 2 * (uint64_t)ub + 4 * (uint64_t)u;
+        d[1 + j] = (int32_t)st[src];
+        d[un + 2 + j] = pr[src];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Phase 3 helpers.  All phase-3 kernels split the unit's bracket range
 // [m_base, m_end) into gridDim.x contiguous block ranges the same way.
@@ -630,6 +674,17 @@ hipError_t launch_assemble(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
     }
     hipLaunchKernelGGL(k_assemble, dim3(grid), dim3(256), 0, s, b.uout, n_units, d_struct, d_pair,
                        dst_struct, dst_pair, N, sent);
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble_docs(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
+                                const uint32_t *d_struct, const int32_t *d_pair, int32_t *dst,
+                                uint32_t N) {
+    const uint64_t rows_threads = ((uint64_t)N + kAsmRows - 1) / kAsmRows;
+    const uint64_t threads = rows_threads > n_units ? rows_threads : n_units;
+    const uint32_t grid = (uint32_t)((threads + 255) / 256) + 1;
+    hipLaunchKernelGGL(k_assemble_docs, dim3(grid), dim3(256), 0, s, b.uout, n_units, d_struct,
+                       d_pair, dst, N);
     return hipGetLastError();
 }
 

@@ -250,6 +250,56 @@ int cjx_parse_lines_device(cjx_context *ctx, const uint8_t *d_input, const size_
                            int32_t *d_structural, int32_t *d_pair, size_t cap,
                            cjx_record *d_records, size_t max_records);
 
+/* ------------------------------------------------------------------------
+ * Request server (no counterpart in the reference, whose parse calls are
+ * synchronous one-document calls: parse_standard_json.cu:1590-1700).  For
+ * serving callers with many concurrent requests: submit from any thread, a
+ * worker thread batches what is pending onto n_slots streams (continuous
+ * batching: a batch starts as soon as a slot is free), and completions come
+ * back per request in the reference's host layout.
+ *
+ * device_budget bounds the estimated device bytes of the batches in flight
+ * (each slot's batch stays within device_budget / n_slots; a larger single
+ * request runs alone); slot buffers stay resident between batches.  The
+ * document bytes passed to cjx_server_submit must stay valid until its
+ * completion is returned.  flags: CJX_FLAG_UTF8_STRICT, CJX_FLAG_DEPTH_U16
+ * (requests with different flags are never batched together).
+ * ---------------------------------------------------------------------- */
+typedef struct cjx_server cjx_server;
+
+typedef struct cjx_completion {
+    uint64_t ticket;      /* from cjx_server_submit                              */
+    int32_t *buf;         /* 2*(n_structural+2) int32, the cjx_result layout;
+                             valid until cjx_server_release                      */
+    void *owner;          /* pinned region holding buf (server-internal)         */
+    int64_t n_structural; /* N                                                   */
+    int32_t error;        /* CJX_OK, CJX_ERR_UTF8, CJX_ERR_STRUCTURE, ...        */
+    int32_t reserved;
+    double latency_ms;    /* submit -> completion ready                          */
+    double queue_ms;      /* submit -> batch launched                            */
+} cjx_completion;
+
+typedef struct cjx_server_stats {
+    uint64_t batches, documents, bytes; /* launched batches, completed docs, admitted bytes */
+    uint64_t max_inflight_bytes;        /* peak estimated device bytes in flight          */
+    uint64_t relaunches;                /* batches re-run with exact output regions       */
+    uint64_t pending;                   /* requests waiting for a slot                    */
+    uint64_t region_bytes;              /* pinned result regions held                     */
+} cjx_server_stats;
+
+int cjx_server_create(int device, size_t device_budget, int n_slots, size_t max_batch,
+                      cjx_server **out);
+int cjx_server_submit(cjx_server *srv, const uint8_t *data, size_t n, unsigned flags,
+                      uint64_t *ticket);
+/* Up to cap completions (any order); waits up to timeout_ms when none is ready. */
+int cjx_server_wait(cjx_server *srv, cjx_completion *out, size_t cap, size_t *n_out,
+                    int timeout_ms);
+void cjx_server_release(cjx_server *srv, cjx_completion *c);
+int cjx_server_get_stats(cjx_server *srv, cjx_server_stats *st);
+/* Finishes every submitted request, then frees the server (completions not
+ * yet released become invalid). */
+void cjx_server_destroy(cjx_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,20 @@ def test_entry_points_reject_bad_arguments_without_device():
                                     ctypes.byref(r)) == cj.CJX_ERR_INPUT
 
 
+def test_server_rejects_bad_arguments_without_device():
+    h = ctypes.c_void_p()
+    L = cj.lib()
+    assert L.cjx_server_create(0, 1 << 30, 0, 64, ctypes.byref(h)) == cj.CJX_ERR_INPUT
+    assert L.cjx_server_create(0, 0, 2, 64, ctypes.byref(h)) == cj.CJX_ERR_INPUT
+    assert L.cjx_server_create(0, 1 << 30, 2, 0, ctypes.byref(h)) == cj.CJX_ERR_INPUT
+    assert not h.value
+    t = ctypes.c_uint64()
+    assert L.cjx_server_submit(None, None, 0, 0, ctypes.byref(t)) == cj.CJX_ERR_INPUT
+    n = ctypes.c_size_t()
+    assert L.cjx_server_wait(None, None, 0, ctypes.byref(n), 0) == cj.CJX_ERR_INPUT
+    L.cjx_server_destroy(None)
+
+
 def test_context_without_gpu_fails_loudly():
     try:
         import torch
@@ -73,6 +87,8 @@ def test_context_without_gpu_fails_loudly():
     assert cj.lib().cjx_context_create(0, ctypes.byref(h)) == cj.CJX_ERR_DEVICE
     with pytest.raises(RuntimeError):
         cj.Context(0)
+    with pytest.raises(RuntimeError):
+        cj.Server(0, 1 << 30, 2, 64)
 
 
 def test_byte_class_tables():
