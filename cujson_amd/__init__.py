@@ -58,7 +58,7 @@ ABI_SYMBOLS = (
     "cjx_version", "cjx_context_set_timing", "cjx_context_timings",
     "cjx_parse_batch", "cjx_free_batch_result", "cjx_parse_batch_device", "cjx_batch_wait",
     "cjx_parse_lines_device", "cjx_server_create", "cjx_server_submit", "cjx_server_wait",
-    "cjx_server_release", "cjx_server_get_stats", "cjx_server_destroy",
+    "cjx_server_release", "cjx_server_reserve", "cjx_server_get_stats", "cjx_server_destroy",
 )
 
 
@@ -189,6 +189,8 @@ def lib():
     L.cjx_parse_lines_device.restype = ctypes.c_int
     L.cjx_server_create.argtypes = [ctypes.c_int, sz, ctypes.c_int, sz, ctypes.POINTER(vp)]
     L.cjx_server_create.restype = ctypes.c_int
+    L.cjx_server_reserve.argtypes = [vp, sz]
+    L.cjx_server_reserve.restype = ctypes.c_int
     L.cjx_server_submit.argtypes = [vp, vp, sz, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint64)]
     L.cjx_server_submit.restype = ctypes.c_int
     L.cjx_server_wait.argtypes = [vp, ctypes.POINTER(_Completion), sz, ctypes.POINTER(sz),
@@ -561,6 +563,12 @@ class Server:
             raise RuntimeError(f"cjx_server_create: {strerror(rc)}")
         self.handle = h
         self._live = {}
+
+    def reserve(self, max_request_bytes: int) -> None:
+        """Pre-sizes every slot (only before the first submit)."""
+        rc = lib().cjx_server_reserve(self.handle, max_request_bytes)
+        if rc:
+            raise RuntimeError(f"cjx_server_reserve: {strerror(rc)}")
 
     def submit(self, data, flags: int = 0) -> int:
         a = np.ascontiguousarray(np.frombuffer(data, dtype=np.uint8)

@@ -42,12 +42,14 @@ struct DevBuf {
 struct PinnedBuf {
     void *p = nullptr;
     size_t cap = 0;
-    bool ensure(size_t bytes) {
+    // flags: hipHostMallocCoherent for memory that kernels read or write
+    // directly (visible to the host without a cache write-back)
+    bool ensure(size_t bytes, unsigned flags = hipHostMallocDefault) {
         if (bytes <= cap) return true;
         if (p) (void)hipHostFree(p);
         p = nullptr;
         cap = 0;
-        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc(&p, bytes, flags) != hipSuccess) {
             p = nullptr;
             return false;
         }

@@ -158,6 +158,10 @@ hipError_t launch_assemble(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
 hipError_t launch_assemble_docs(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
                                 const uint32_t *d_struct, const int32_t *d_pair, int32_t *dst,
                                 uint32_t N);
+// Kernel copy of two byte ranges (src/dst 16-byte aligned; either side may be
+// pinned host memory); n1 = 0 for one range.
+hipError_t launch_copy(hipStream_t s, const void *src0, void *dst0, uint64_t n0, const void *src1,
+                       void *dst1, uint64_t n1);
 // Sort-path phase 3 for every unit that is not well nested, one block per
 // unit (launches with several units).
 hipError_t launch_sort_units(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool depth_u16,

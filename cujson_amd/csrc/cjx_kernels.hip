@@ -677,6 +677,43 @@ hipError_t launch_assemble(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
     return hipGetLastError();
 }
 
+// Copies up to two byte ranges (16-byte aligned ends) with 16-byte loads and
+// stores; either side may be pinned host memory, so small transfers run on
+// the compute queue instead of waiting behind large DMA copies.
+struct CopySeg {
+    const uint8_t *src;
+    uint8_t *dst;
+    uint64_t n;
+};
+
+__global__ __launch_bounds__(256) void k_copy_segs(CopySeg a, CopySeg b) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const CopySeg g = k ? b : a;
+        const uint64_t n16 = g.n >> 4;
+        const uint4 *s = reinterpret_cast<const uint4 *>(g.src);
+        uint4 *d = reinterpret_cast<uint4 *>(g.dst);
+        for (uint64_t i = t0; i < n16; i += stride) d[i] = s[i];
+        for (uint64_t i = (n16 << 4) + t0; i < g.n; i += stride) g.dst[i] = g.src[i];
+    }
+}
+
+hipError_t launch_copy(hipStream_t s, const void *src0, void *dst0, uint64_t n0, const void *src1,
+                       void *dst1, uint64_t n1) {
+    const uint64_t words = ((n0 > n1 ? n0 : n1) + 15) / 16;
+    // 64 blocks already run a host-link copy at full rate (and leave the CUs
+    // to the parse kernels of other streams)
+    uint64_t grid = (words + 255) / 256;
+    if (grid < 1) grid = 1;
+    if (grid > 64) grid = 64;
+    CopySeg a{static_cast<const uint8_t *>(src0), static_cast<uint8_t *>(dst0), n0};
+    CopySeg b{static_cast<const uint8_t *>(src1), static_cast<uint8_t *>(dst1), n1};
+    hipLaunchKernelGGL(k_copy_segs, dim3((uint32_t)grid), dim3(256), 0, s, a, b);
+    return hipGetLastError();
+}
+
 hipError_t launch_assemble_docs(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
                                 const uint32_t *d_struct, const int32_t *d_pair, int32_t *dst,
                                 uint32_t N) {

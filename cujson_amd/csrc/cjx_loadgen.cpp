@@ -16,7 +16,9 @@
 // (cjx_parse_standard): error code and structural count always, a checksum
 // of the whole buffer for small documents and of sampled cells for the rest.
 // The plain path is then timed on the same request sequence, one document
-// per call, for comparison.
+// per call, for comparison.  Before serving, the host-link ceiling of this
+// traffic (inputs in, results out, several streams, no parse) is measured
+// with the copy engines: it bounds what any server can sustain here.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -52,7 +54,7 @@ double ms_since(clk::time_point t0) {
 struct Config {
     double rate = 20000, seconds = 60, plain_seconds = 10;
     size_t budget = 64ull << 30, max_batch = 4096;
-    int slots = 4, pool_small = 4096, pool_mid = 128, pool_large = 8;
+    int slots = 8, pool_small = 4096, pool_mid = 128, pool_large = 8;
     uint64_t seed = 20240605;
     double p_small = 0.80, p_mid = 0.19;  // the rest is large
     size_t small_lo = 1 << 10, small_hi = 16 << 10, mid_lo = 64 << 10, mid_hi = 4 << 20,
@@ -187,6 +189,45 @@ bool make_pool(const Config &c, std::vector<Doc> &pool, std::vector<int> *by_cla
     return true;
 }
 
+// Host-link ceiling of the serving traffic: 4 streams, each repeatedly
+// copying a 128 MiB input in and `out_ratio` of it out through the copy
+// engines (no parse), as the server does for one request.  Returns input
+// GB/s, or 0 on failure.
+double link_ceiling(const uint8_t *src, size_t src_bytes, double out_ratio) {
+    const int S = 4, reps = 6;
+    const size_t in_b = std::min<size_t>(src_bytes, 128u << 20) & ~(size_t)15;
+    const size_t out_b = (size_t)((double)in_b * out_ratio) & ~(size_t)15;
+    std::vector<void *> din(S, nullptr), dout(S, nullptr), hout(S, nullptr);
+    std::vector<hipStream_t> st(S, nullptr);
+    double gbs = 0;
+    bool ok = in_b > 0 && out_b > 0;
+    for (int i = 0; i < S && ok; i++)
+        ok = hipMalloc(&din[i], in_b) == hipSuccess && hipMalloc(&dout[i], out_b) == hipSuccess &&
+             hipHostMalloc(&hout[i], out_b, hipHostMallocCoherent) == hipSuccess &&
+             hipStreamCreate(&st[i]) == hipSuccess;
+    if (ok) {
+        auto issue = [&](int i) {
+            (void)hipMemcpyAsync(din[i], src, in_b, hipMemcpyHostToDevice, st[i]);
+            (void)hipMemcpyAsync(hout[i], dout[i], out_b, hipMemcpyDeviceToHost, st[i]);
+        };
+        for (int i = 0; i < S; i++) issue(i);
+        ok = hipDeviceSynchronize() == hipSuccess;
+        const auto t0 = clk::now();
+        for (int r = 0; r < reps; r++)
+            for (int i = 0; i < S; i++) issue(i);
+        ok = ok && hipDeviceSynchronize() == hipSuccess;
+        const double ms = ms_since(t0);
+        if (ok && ms > 0) gbs = (double)reps * S * (double)in_b / (ms * 1e6);
+    }
+    for (int i = 0; i < S; i++) {
+        if (st[i]) (void)hipStreamDestroy(st[i]);
+        if (din[i]) (void)hipFree(din[i]);
+        if (dout[i]) (void)hipFree(dout[i]);
+        if (hout[i]) (void)hipHostFree(hout[i]);
+    }
+    return gbs;
+}
+
 }  // namespace
 
 extern "C" {
@@ -245,6 +286,18 @@ int cjx_loadgen_run(const char *args, char *out, size_t cap) {
         doc_of.push_back(v[(size_t)(u01(rng) * (double)v.size()) % v.size()]);
     }
     const size_t n_req = t_arr.size();
+    // result bytes per input byte over the schedule, and the link ceiling
+    double in_sum = 0, out_sum = 0;
+    for (size_t i = 0; i < n_req; i++) {
+        const Doc &d = pool[(size_t)doc_of[i]];
+        in_sum += (double)d.n;
+        out_sum += 8.0 * (double)(d.n_structural + 2);
+    }
+    const double out_ratio = in_sum > 0 ? out_sum / in_sum : 0.0;
+    const Doc *biggest = &pool[0];
+    for (const Doc &d : pool)
+        if (d.n > biggest->n) biggest = &d;
+    const double ceiling = link_ceiling(biggest->p, biggest->n, out_ratio);
 
     cjx_server *srv = nullptr;
     if ((rc = cjx_server_create(0, c.budget, c.slots, c.max_batch, &srv))) {
@@ -253,7 +306,15 @@ int cjx_loadgen_run(const char *args, char *out, size_t cap) {
         std::snprintf(out, cap, "{\"error\": \"server: %d\"}", rc);
         return 1;
     }
-    // Warm-up: every pool document once (grows the slot buffers and regions).
+    // Resident buffers for the largest request, then a warm-up with every
+    // pool document once.
+    if ((rc = cjx_server_reserve(srv, c.large_hi))) {
+        cjx_server_destroy(srv);
+        cjx_context_destroy(ctx);
+        (void)hipHostFree(pinned);
+        std::snprintf(out, cap, "{\"error\": \"reserve: %d\"}", rc);
+        return 1;
+    }
     {
         for (const Doc &d : pool) cjx_server_submit(srv, d.p, d.n, 0, nullptr);
         size_t got = 0;
@@ -388,7 +449,8 @@ int cjx_loadgen_run(const char *args, char *out, size_t cap) {
         "\"relaunches\": %llu, \"max_inflight_GB\": %.3f, \"region_GB\": %.3f, "
         "\"plain_requests\": %zu, \"plain_GBps\": %.3f, \"plain_req_s\": %.1f, "
         "\"plain_p50_ms\": %.4f, \"plain_p99_ms\": %.4f, \"pool_docs\": %zu, "
-        "\"pool_generate_s\": %.2f, \"budget_GB\": %.1f, \"slots\": %d, \"max_batch\": %zu}",
+        "\"pool_generate_s\": %.2f, \"budget_GB\": %.1f, \"slots\": %d, \"max_batch\": %zu, "
+        "\"result_bytes_per_input_byte\": %.4f, \"link_ceiling_GBps\": %.3f}",
         n_req, n_done, mismatches.load(), submit_failed ? "true" : "false", c.rate, c.seconds,
         offered_bytes / (c.seconds * 1e9), span_ms > 0 ? bytes / (span_ms * 1e6) : 0.0,
         span_ms > 0 ? (double)n_done / (span_ms / 1000.0) : 0.0, pct(all, 0.5), pct(all, 0.99),
@@ -402,7 +464,8 @@ int cjx_loadgen_run(const char *args, char *out, size_t cap) {
         (unsigned long long)(st1.relaunches - st0.relaunches), st1.max_inflight_bytes / 1e9,
         st1.region_bytes / 1e9, plain_n, plain_ms > 0 ? plain_bytes / (plain_ms * 1e6) : 0.0,
         plain_ms > 0 ? (double)plain_n / (plain_ms / 1000.0) : 0.0, pct(plain_lat, 0.5),
-        pct(plain_lat, 0.99), pool.size(), gen_ms / 1000.0, c.budget / 1e9, c.slots, c.max_batch);
+        pct(plain_lat, 0.99), pool.size(), gen_ms / 1000.0, c.budget / 1e9, c.slots, c.max_batch,
+        out_ratio, ceiling);
     return (mismatches.load() || n_done != n_req || submit_failed) ? 2 : 0;
 }
 

@@ -260,7 +260,11 @@ int cjx_parse_lines_device(cjx_context *ctx, const uint8_t *d_input, const size_
  *
  * device_budget bounds the estimated device bytes of the batches in flight
  * (each slot's batch stays within device_budget / n_slots; a larger single
- * request runs alone); slot buffers stay resident between batches.  The
+ * request runs alone); slot buffers stay resident between batches.  A batch
+ * holds at most max_batch requests and 64 MiB of input.  n_slots / 2 of the
+ * slots are express slots that take only requests below 4 MiB, in batches
+ * of at most 16 MiB, so small requests never queue behind the transfers of
+ * very large ones.  The
  * document bytes passed to cjx_server_submit must stay valid until its
  * completion is returned.  flags: CJX_FLAG_UTF8_STRICT, CJX_FLAG_DEPTH_U16
  * (requests with different flags are never batched together).
@@ -289,6 +293,11 @@ typedef struct cjx_server_stats {
 
 int cjx_server_create(int device, size_t device_budget, int n_slots, size_t max_batch,
                       cjx_server **out);
+/* Before the first request: grows every slot's device buffers and the
+ * pinned result pool for the largest batches and for single requests of up
+ * to max_request_bytes, so that serving allocates nothing (device frees
+ * synchronize the whole device).  CJX_ERR_INPUT once requests were submitted. */
+int cjx_server_reserve(cjx_server *srv, size_t max_request_bytes);
 int cjx_server_submit(cjx_server *srv, const uint8_t *data, size_t n, unsigned flags,
                       uint64_t *ticket);
 /* Up to cap completions (any order); waits up to timeout_ms when none is ready. */

@@ -27,12 +27,14 @@ def mix():
     return docs, flags
 
 
-def run_server(docs, flags, threads=1, **kw):
+def run_server(docs, flags, threads=1, reserve=0, **kw):
     """Submits every document (from `threads` threads, interleaved) and
     returns {index: (error, buf)} plus the server statistics."""
     out, tickets = {}, {}
     lock = threading.Lock()
     with cj.Server(**kw) as srv:
+        if reserve:
+            srv.reserve(reserve)
         def submit(part):
             for i in part:
                 t = srv.submit(docs[i], flags[i])
@@ -105,3 +107,30 @@ def test_server_matches_plain_parse(mix):
                 assert np.array_equal(out[i][1], r.buf), i
     finally:
         ctx.close()
+
+
+def test_server_large_batches_take_the_copy_engine_path():
+    """Batches above the kernel-copy limit (16 MiB of input or of results) go
+    through the copy engines: large inputs copied from the caller's memory,
+    small ones staged, results laid out on the device and copied out."""
+    docs = [gen.standard_doc(target_bytes=t, seed=100 + i)
+            for i, t in enumerate((3 << 20, 22 << 20, 20 << 10, 6 << 20, 9 << 10, 7 << 20))]
+    docs.append(docs[0].copy())
+    assert gen.plant_fault(docs[-1], 3, 7)
+    docs += [np.frombuffer(b'{"k":[1,2,{"z":null}]}', np.uint8).copy()] * 3
+    flags = [0] * len(docs)
+    out, st = run_server(docs, flags, budget=16 << 30, slots=2, max_batch=4096)
+    check_oracle(docs, flags, out, "large batch")
+    assert st["batches"] < len(docs)
+
+
+def test_server_reserved_buffers_serve_the_mix(mix):
+    docs, flags = mix
+    out, st = run_server(docs, flags, threads=2, reserve=max(d.size for d in docs),
+                         budget=4 << 30, slots=4, max_batch=256)
+    check_oracle(docs, flags, out, "reserved")
+    with cj.Server(budget=1 << 30, slots=2, max_batch=8) as srv:
+        srv.submit(docs[0])
+        with pytest.raises(RuntimeError):
+            srv.reserve(1 << 20)  # only before the first request
+        assert srv.completions(timeout_ms=2000)
