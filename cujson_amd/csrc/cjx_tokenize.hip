@@ -19,19 +19,23 @@
 // Classification is three v_perm table lookups ANDed together, indexed by
 // byte bits 0-2, 3-5 and 6-7.  Every class needed is a product set in those
 // three coordinates, so the AND is exact:
-//   bit 0 ':' (and '\n' for JSON Lines)   bit 4 E    (>= 0xE0)
-//   bit 1 ','                              bit 5 F    (>= 0xF0)
-//   bit 2 '[' ']' '{' '}'                  bit 6 '\\'
-//   bit 3 CONT (0x80-0xBF)                 bit 7 '"'
+//   bit 0 ':' (and '\n' for JSON Lines)   bit 4 '\\'
+//   bit 1 CONT (0x80-0xBF)                 bit 5 E    (>= 0xE0)
+//   bit 2 '[' ']' '{' '}'                  bit 6 F    (>= 0xF0)
+//   bit 3 ','                              bit 7 '"'
+// Once the UTF-8 rule has used CONT and E, bits 1 and 5 are replaced by the
+// raw byte's bits 1 (set for '[' and '{', clear for ']' and '}') and 5 (set
+// for '{' and '}'), and one 8x8 bit-matrix transpose per byte slot turns the
+// eight class words into the eight 32-bit class masks at once.
 #include "cjx_device.h"
 
 namespace cjx {
 
 // Table entries, byte i = value for index i.
-constexpr uint64_t kLutBits02 = 0x38383C7A3CB93838ull;        // by bits 0-2
-constexpr uint64_t kLutBits35Std = 0x3D381A984C080808ull;     // by bits 3-5
-constexpr uint64_t kLutBits35Lines = 0x3D381A984C080908ull;   // '\n' joins ':'
-constexpr uint64_t kLutBits67 = 0x0000000030084483ull;        // by bits 6-7
+constexpr uint64_t kLutBits02 = 0x6262667A66E36262ull;        // by bits 0-2
+constexpr uint64_t kLutBits35Std = 0x67622AA216020202ull;     // by bits 3-5
+constexpr uint64_t kLutBits35Lines = 0x67622AA216020302ull;   // '\n' joins ':'
+constexpr uint64_t kLutBits67 = 0x0000000060021489ull;        // by bits 6-7
 
 
 __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) {
@@ -59,16 +63,30 @@ __device__ __forceinline__ void transpose4(uint32_t a, uint32_t b, uint32_t c, u
     o3 = perm(ce_hi, ab_hi, 0x07060302u);
 }
 
-// Linear 32-bit mask (bit i <-> byte i of the lane) of class bit K.
-template <int K>
-__device__ __forceinline__ uint32_t extract(const uint32_t (&t)[8]) {
-    uint32_t acc = 0;
+// 8x8 bit-matrix transpose in every byte slot: afterwards t[K] holds, in
+// byte b, bit d = bit K of byte b of the input t[d].  With transposed input
+// words (byte b of word d = input byte 8b+d) t[K] is the linear 32-bit mask
+// (bit i <-> byte i of the lane) of class bit K.  Three butterfly stages
+// swap the off-diagonal 4x4, 2x2 and 1x1 blocks.
+__host__ __device__ __forceinline__ void transpose8x8(uint32_t (&t)[8]) {
 #pragma unroll
-    for (int d = 0; d < 8; d++) {
-        const uint32_t v = (d >= K) ? (t[d] << (d - K)) : (t[d] >> (K - d));
-        acc |= v & (0x01010101u << d);
+    for (int d = 0; d < 4; d++) {
+        const uint32_t a = t[d], b = t[d + 4];
+        t[d] = (a & 0x0F0F0F0Fu) | ((b << 4) & 0xF0F0F0F0u);
+        t[d + 4] = ((a >> 4) & 0x0F0F0F0Fu) | (b & 0xF0F0F0F0u);
     }
-    return acc;
+#pragma unroll
+    for (int d = 0; d < 8; d += (d & 1) ? 3 : 1) {  // 0, 1, 4, 5
+        const uint32_t a = t[d], b = t[d + 2];
+        t[d] = (a & 0x33333333u) | ((b << 2) & 0xCCCCCCCCu);
+        t[d + 2] = ((a >> 2) & 0x33333333u) | (b & 0xCCCCCCCCu);
+    }
+#pragma unroll
+    for (int d = 0; d < 8; d += 2) {
+        const uint32_t a = t[d], b = t[d + 1];
+        t[d] = (a & 0x55555555u) | ((b << 1) & 0xAAAAAAAAu);
+        t[d + 1] = ((a >> 1) & 0x55555555u) | (b & 0xAAAAAAAAu);
+    }
 }
 
 // Unescaped quotes given the parity `ov` of the backslash run ending just
@@ -127,6 +145,8 @@ __device__ __forceinline__ void load_lane(const uint8_t *base, uint64_t o, uint6
 //      [p-1 and p are continuations] != [p-2 >= E0 or p-3 >= F0]
 //  * a word without one errs iff the previous word ends in a byte >= C0,
 //    or its third byte is >= E0, or its second >= F0 (:219-223, :239-242)
+// The class bits are CONT = bit 1, E = bit 5, F = bit 6; the per-byte error
+// terms are formed in bit 1 and the ASCII-word terms in bit 7.
 // pk5/pk6/pk7/pr7 carry the previous lane's transposed words 5, 6, 7 (classes)
 // and 7 (raw).  vA/vB: 0xFF in slot s if word 2s / 2s+1 lies inside the unit.
 __device__ __forceinline__ uint32_t utf8_lane_error(const uint32_t (&r)[8], const uint32_t (&k)[8],
@@ -134,21 +154,21 @@ __device__ __forceinline__ uint32_t utf8_lane_error(const uint32_t (&r)[8], cons
                                                     uint32_t pr7, uint32_t vA, uint32_t vB) {
     const uint32_t ks5 = alignbyte3(k[5], pk5), ks6 = alignbyte3(k[6], pk6);
     const uint32_t ks7 = alignbyte3(k[7], pk7), rs7 = alignbyte3(r[7], pr7);
-    const uint32_t e0 = (ks7 & k[0]) ^ ((ks6 >> 1) | (ks5 >> 2));
-    const uint32_t e1 = (k[0] & k[1]) ^ ((ks7 >> 1) | (ks6 >> 2));
-    const uint32_t e2 = (k[1] & k[2]) ^ ((k[0] >> 1) | (ks7 >> 2));
-    const uint32_t e3 = (k[2] & k[3]) ^ ((k[1] >> 1) | (k[0] >> 2));
-    const uint32_t e4 = (k[3] & k[4]) ^ ((k[2] >> 1) | (k[1] >> 2));
-    const uint32_t e5 = (k[4] & k[5]) ^ ((k[3] >> 1) | (k[2] >> 2));
-    const uint32_t e6 = (k[5] & k[6]) ^ ((k[4] >> 1) | (k[3] >> 2));
-    const uint32_t e7 = (k[6] & k[7]) ^ ((k[5] >> 1) | (k[4] >> 2));
-    const uint32_t errA = (e0 | e1 | e2 | e3) & 0x08080808u;
-    const uint32_t errB = (e4 | e5 | e6 | e7) & 0x08080808u;
+    const uint32_t e0 = (ks7 & k[0]) ^ ((ks6 >> 4) | (ks5 >> 5));
+    const uint32_t e1 = (k[0] & k[1]) ^ ((ks7 >> 4) | (ks6 >> 5));
+    const uint32_t e2 = (k[1] & k[2]) ^ ((k[0] >> 4) | (ks7 >> 5));
+    const uint32_t e3 = (k[2] & k[3]) ^ ((k[1] >> 4) | (k[0] >> 5));
+    const uint32_t e4 = (k[3] & k[4]) ^ ((k[2] >> 4) | (k[1] >> 5));
+    const uint32_t e5 = (k[4] & k[5]) ^ ((k[3] >> 4) | (k[2] >> 5));
+    const uint32_t e6 = (k[5] & k[6]) ^ ((k[4] >> 4) | (k[3] >> 5));
+    const uint32_t e7 = (k[6] & k[7]) ^ ((k[5] >> 4) | (k[4] >> 5));
+    const uint32_t errA = (e0 | e1 | e2 | e3) & 0x02020202u;
+    const uint32_t errB = (e4 | e5 | e6 | e7) & 0x02020202u;
     const uint32_t gA = (r[0] | r[1] | r[2] | r[3]) & 0x80808080u;
     const uint32_t gB = (r[4] | r[5] | r[6] | r[7]) & 0x80808080u;
-    const uint32_t gen = ((errA & (gA >> 4)) & vA) | ((errB & (gB >> 4)) & vB);
-    const uint32_t ascA = ((rs7 & ~(ks7 << 4)) | (ks6 << 3) | (ks5 << 2)) & ~gA & vA;
-    const uint32_t ascB = ((r[3] & ~(k[3] << 4)) | (k[2] << 3) | (k[1] << 2)) & ~gB & vB;
+    const uint32_t gen = ((errA & (gA >> 6)) & vA) | ((errB & (gB >> 6)) & vB);
+    const uint32_t ascA = ((rs7 & ~(ks7 << 6)) | (ks6 << 2) | (ks5 << 1)) & ~gA & vA;
+    const uint32_t ascB = ((r[3] & ~(k[3] << 6)) | (k[2] << 2) | (k[1] << 1)) & ~gB & vB;
     return gen | ((ascA | ascB) & 0x80808080u);
 }
 
@@ -393,7 +413,7 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
             const uint32_t pk7 = wave_shr1(k[7], c_k7), pr7 = wave_shr1(r[7], c_r7);
             const uint32_t high = ((r[0] | r[1] | r[2] | r[3] | r[4] | r[5] | r[6] | r[7]) &
                                    0x80808080u) |
-                                  ((pk5 | pk6 | pk7) & 0x38000000u) | (pr7 & 0x80000000u);
+                                  ((pk5 | pk6 | pk7) & 0x62000000u) | (pr7 & 0x80000000u);
             if (weak_utf8 && __ballot(high != 0)) {
                 uint32_t vA = ~0u, vB = ~0u;
                 if (lane_off + 32 > U.n) {
@@ -413,15 +433,17 @@ __global__ __launch_bounds__(kTokThreads) void k_tokenize(
         }
 
         // ---- phase 2: masks, escapes, strings ------------------------------
-        uint32_t u7[8];
+        // Class bits 1 and 5 (CONT, E) give way to raw bits 1 and 5, then one
+        // transpose yields every mask.
 #pragma unroll
-        for (int d = 0; d < 8; d++) u7[d] = (k[d] & 0x07070707u) + 0x7F7F7F7Fu;
-        const uint32_t bs = extract<6>(k);
-        const uint32_t q = extract<7>(k);
-        op_r[i] = extract<7>(u7);
-        br_r[i] = extract<2>(k) & op_r[i];
-        s_b1[w][i][lane] = extract<1>(r) & br_r[i];
-        s_b5[w][i][lane] = extract<5>(r) & br_r[i];
+        for (int d = 0; d < 8; d++) k[d] = (k[d] & ~0x22222222u) | (r[d] & 0x22222222u);
+        transpose8x8(k);
+        const uint32_t bs = k[4];
+        const uint32_t q = k[7];
+        br_r[i] = k[2];
+        op_r[i] = k[0] | k[2] | k[3];
+        s_b1[w][i][lane] = k[1] & k[2];
+        s_b5[w][i][lane] = k[5] & k[2];
 
         const uint32_t all_bs = (bs == 0xFFFFFFFFu);
         const uint32_t run = (uint32_t)__clz(~bs);  // trailing backslash run of the lane
@@ -652,6 +674,14 @@ static uint32_t host_perm(uint32_t hi, uint32_t lo, uint32_t sel) {
         out |= v << (8 * i);
     }
     return out;
+}
+
+// Host run of the mask transpose, for the self-test that runs without a GPU.
+extern "C" void cjx_debug_transpose8x8(uint32_t *t) {
+    uint32_t v[8];
+    for (int d = 0; d < 8; d++) v[d] = t[d];
+    cjx::transpose8x8(v);
+    for (int d = 0; d < 8; d++) t[d] = v[d];
 }
 
 extern "C" uint32_t cjx_debug_byte_class(uint32_t byte, int jsonl) {

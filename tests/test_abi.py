@@ -91,6 +91,26 @@ def test_context_without_gpu_fails_loudly():
         cj.Server(0, 1 << 30, 2, 64)
 
 
+def test_mask_transpose():
+    """The tokenizer's 8x8 bit transpose (host run of the same function):
+    word K, byte b, bit d must be bit K of byte b of input word d."""
+    f = cj.lib().cjx_debug_transpose8x8
+    f.argtypes = [ctypes.POINTER(ctypes.c_uint32)]
+    rng = np.random.default_rng(5)
+    cases = [rng.integers(0, 2 ** 32, 8, dtype=np.uint64) for _ in range(200)]
+    cases += [np.array([1 << (8 * (d % 4) + d) for d in range(8)], np.uint64),
+              np.full(8, 0xFFFFFFFF, np.uint64), np.zeros(8, np.uint64)]
+    for c in cases:
+        t = (ctypes.c_uint32 * 8)(*[int(x) for x in c])
+        f(t)
+        for K in range(8):
+            want = 0
+            for b in range(4):
+                for d in range(8):
+                    want |= ((int(c[d]) >> (8 * b + K)) & 1) << (8 * b + d)
+            assert t[K] == want, (K, hex(t[K]), hex(want))
+
+
 def test_byte_class_tables():
     """The tokenizer's three-lookup byte classifier (host model of the same
     tables and of v_perm) against the class definitions, for all 256 bytes."""
@@ -102,17 +122,17 @@ def test_byte_class_tables():
             want = 0
             if b == 0x3A or (jsonl and b == 0x0A):
                 want |= 0x01
-            if b == 0x2C:
+            if 0x80 <= b <= 0xBF:
                 want |= 0x02
             if b in (0x5B, 0x5D, 0x7B, 0x7D):
                 want |= 0x04
-            if 0x80 <= b <= 0xBF:
+            if b == 0x2C:
                 want |= 0x08
-            if b >= 0xE0:
-                want |= 0x10
-            if b >= 0xF0:
-                want |= 0x20
             if b == 0x5C:
+                want |= 0x10
+            if b >= 0xE0:
+                want |= 0x20
+            if b >= 0xF0:
                 want |= 0x40
             if b == 0x22:
                 want |= 0x80
