@@ -16,14 +16,20 @@ template <int CTRL, int ROW_MASK = 0xF>
 __device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROW_MASK, 0xF, false);
 }
+// Row shift whose lanes without a source read 0 (bound_ctrl), so no old
+// value is needed and the shift can fold into the consuming add.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_zero(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
+}
 
 // Inclusive prefix sum over the 64 lanes of a wave (row shifts, then row
 // broadcasts 15 and 31).
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
-    x += dpp<kDppRowShr1>(0u, x);
-    x += dpp<kDppRowShr2>(0u, x);
-    x += dpp<kDppRowShr4>(0u, x);
-    x += dpp<kDppRowShr8>(0u, x);
+    x += dpp_zero<kDppRowShr1>(x);
+    x += dpp_zero<kDppRowShr2>(x);
+    x += dpp_zero<kDppRowShr4>(x);
+    x += dpp_zero<kDppRowShr8>(x);
     x += dpp<kDppRowBcast15, 0xA>(0u, x);
     x += dpp<kDppRowBcast31, 0xC>(0u, x);
     return x;

@@ -23,7 +23,10 @@ constexpr int kTokRounds = CJX_TOK_ROUNDS;
 constexpr int kRoundBytes = 64 * kLaneBytes;              // 2 KiB per wave round
 constexpr int kWaveBytes = kRoundBytes * kTokRounds;
 constexpr int kTileBytes = kWaveBytes * kWaves;
-constexpr int kStageRows = 1024;  // LDS rows per wave for coalesced structural stores
+#ifndef CJX_STAGE_ROWS
+#define CJX_STAGE_ROWS 512
+#endif
+constexpr int kStageRows = CJX_STAGE_ROWS;  // LDS rows per wave for coalesced structural stores
 
 // Phase-3 geometry.
 constexpr int kP3Threads = 256;
