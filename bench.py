@@ -293,6 +293,33 @@ def cpu_baseline(sample_bytes: int):
     out["oracle_port_gbs"] = round(small.size / (t1 - t0) / 1e9, 4)
     if out["value"] is None:
         out["value"] = out["oracle_port_gbs"]
+    out["pison"] = pison_baseline()
+    return out
+
+
+def pison_baseline(sample_bytes: int = 256 << 20):
+    """Pison (the reference's vendored CPU parser), leveled-bitmap construction
+    as its harness times it, 8 levels, on one thread and on every host core
+    this process may use (at most 16).  Pison's leveled bitmaps stop at 22
+    levels and deeper input corrupts its memory, so the sample is the same
+    generator without the depth-64 branch."""
+    import ctypes
+    from cujson_amd import gen
+    path = os.path.join(ROOT, "oracle", "_ref", "libpison_ref.so")
+    if not os.path.exists(path):
+        return {"value": None, "note": "Pison build absent"}
+    L = ctypes.CDLL(path)
+    L.pison_construct_ms.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int]
+    L.pison_construct_ms.restype = ctypes.c_double
+    doc = gen.standard_doc(target_bytes=sample_bytes, seed=gen.SEED_BASE + 103).tobytes()
+    cores = min(16, len(os.sched_getaffinity(0)))
+    out = {"unit": "GB/s", "levels": 8, "cores_all": cores,
+           "sample": f"{len(doc) / 1e6:.0f} MB document from the same generator without the "
+                     f"deep branch, BitmapConstructor::construct + getIterator, best of 3"}
+    for name, th in (("one_core", 1), ("all_cores", cores)):
+        ms = L.pison_construct_ms(doc, len(doc), th, 8, 3)
+        out[name] = round(len(doc) / (ms * 1e-3) / 1e9, 3) if ms > 0 else None
     return out
 
 
