@@ -12,6 +12,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/cujson_amd.h"
@@ -42,6 +45,64 @@ uint32_t pair_blocks(uint64_t max_n) {
 // Tokenizer work of the units: a unit of at most one wave's bytes is small
 // (one wave, no tiles); the others are cut into tiles.  Returns the tile
 // count and fills n_small.
+// Pinned host buffers for results.  Allocating and pinning a large host
+// buffer per call costs about as much as the copy into it, so released
+// result buffers are kept for reuse: a request takes the smallest cached
+// buffer of at least its size and at most about twice it; at most
+// kPoolBytes stay cached.  The pool is process-wide and never torn down (the
+// runtime may be gone at exit).
+struct PinnedPool {
+    std::mutex mu;
+    std::unordered_map<void *, size_t> cap;  // every buffer handed out or cached
+    std::multimap<size_t, void *> idle;      // cached: capacity -> buffer
+    size_t cached = 0;
+};
+constexpr size_t kPoolBytes = 8ull << 30;
+
+PinnedPool &pinned_pool() {
+    static PinnedPool *p = new PinnedPool();
+    return *p;
+}
+
+void *pinned_get(size_t bytes) {
+    PinnedPool &P = pinned_pool();
+    bytes = std::max<size_t>(bytes, 64);
+    {
+        std::lock_guard<std::mutex> g(P.mu);
+        auto it = P.idle.lower_bound(bytes);
+        if (it != P.idle.end() && it->first <= 2 * bytes + (1u << 20)) {
+            void *p = it->second;
+            P.cached -= it->first;
+            P.idle.erase(it);
+            return p;
+        }
+    }
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> g(P.mu);
+    P.cap[p] = bytes;
+    return p;
+}
+
+void pinned_put(void *p) {
+    if (!p) return;
+    PinnedPool &P = pinned_pool();
+    {
+        std::lock_guard<std::mutex> g(P.mu);
+        auto it = P.cap.find(p);
+        if (it != P.cap.end()) {
+            const size_t c = it->second;
+            if (P.cached + c <= kPoolBytes) {
+                P.idle.emplace(c, p);
+                P.cached += c;
+                return;
+            }
+            P.cap.erase(it);
+        }
+    }
+    (void)hipHostFree(p);
+}
+
 uint64_t assign_tiles(std::vector<UnitDesc> &units, uint32_t &n_small) {
     uint64_t tiles = 0;
     n_small = 0;
@@ -188,6 +249,33 @@ LaunchBufs bufs_of(cjx_context *ctx) {
     return b;
 }
 
+// Phase 3 of a launch (unit bases, matching, sort path) on the context
+// stream, after the tokenizer.
+int enqueue_back(cjx_context *ctx, const Shape &sh, unsigned flags, int32_t *d_pair) {
+    const bool u16 = (flags & CJX_FLAG_DEPTH_U16) != 0;
+    LaunchBufs b = bufs_of(ctx);
+    hipStream_t s = ctx->stream;
+    // Concatenated results number pairs across units (JSON Lines).
+    if (launch_unit_bases(s, b, sh.n_units, sh.concat) != hipSuccess) return CJX_ERR_DEVICE;
+    // Well-nested units are matched directly; the rest take the sort path.
+    if (launch_match(s, b, sh.n_units, (uint32_t)sh.chunks, (uint32_t)sh.groups, d_pair) !=
+        hipSuccess)
+        return CJX_ERR_DEVICE;
+    if (sh.n_units == 1) {  // one unit: the multi-block sort path
+        const uint32_t G = phase3_blocks(sh.max_rows);
+        const uint32_t PG = pair_blocks(sh.max_rows);
+        if (launch_phase3(s, b, 0, G, PG, u16, d_pair) != hipSuccess) return CJX_ERR_DEVICE;
+    } else if (launch_sort_units(s, b, sh.n_units, u16, d_pair) != hipSuccess) {
+        return CJX_ERR_DEVICE;
+    }
+    if (ctx->timing) {
+        hipEvent_t *tev = ctx->tev[ctx->timing_count % cjx_context::kTimingRing];
+        (void)hipEventRecord(tev[2], s);
+        ctx->timing_count++;
+    }
+    return CJX_OK;
+}
+
 // Enqueues the whole parse of `units` (already in ctx->units on the device)
 // on the context stream.
 int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsigned flags,
@@ -206,24 +294,63 @@ int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsign
                         !strict, d_struct) != hipSuccess)
         return CJX_ERR_DEVICE;
     if (ctx->timing) (void)hipEventRecord(tev[1], s);
-    // Concatenated results number pairs across units (JSON Lines).
-    if (launch_unit_bases(s, b, sh.n_units, sh.concat) != hipSuccess) return CJX_ERR_DEVICE;
-    // Well-nested units are matched directly; the rest take the sort path.
-    if (launch_match(s, b, sh.n_units, (uint32_t)sh.chunks, (uint32_t)sh.groups, d_pair) !=
-        hipSuccess)
-        return CJX_ERR_DEVICE;
-    if (sh.n_units == 1) {  // one unit: the multi-block sort path
-        const uint32_t G = phase3_blocks(sh.max_rows);
-        const uint32_t PG = pair_blocks(sh.max_rows);
-        if (launch_phase3(s, b, 0, G, PG, u16, d_pair) != hipSuccess) return CJX_ERR_DEVICE;
-    } else if (launch_sort_units(s, b, sh.n_units, u16, d_pair) != hipSuccess) {
-        return CJX_ERR_DEVICE;
+    return enqueue_back(ctx, sh, flags, d_pair);
+}
+
+// A single large unit whose bytes are still in host memory: the input is
+// copied in parts of `part` bytes (whole tiles) on the copy stream, and the
+// tokenizer runs as one launch per part, each waiting for its part's copy,
+// so phases 1-2 overlap the transfer.  Tile tickets continue across the
+// launches, which run in order on the context stream, so the tiles and the
+// look-back are exactly those of one launch.
+int enqueue_parse_chunked(cjx_context *ctx, const uint8_t *host, uint64_t n, const Shape &sh,
+                          unsigned flags, uint32_t *d_struct, int32_t *d_pair, uint64_t part) {
+    const bool u16 = (flags & CJX_FLAG_DEPTH_U16) != 0;
+    if (u16 && !ctx->tmp.ensure(sh.rows * 4 + 16)) return CJX_ERR_ALLOC;
+    if (!ctx->copy_stream) {
+        if (hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&ctx->idle_ev, hipEventDisableTiming) != hipSuccess)
+            return CJX_ERR_DEVICE;
+        for (hipEvent_t &e : ctx->copy_ev)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+                return CJX_ERR_DEVICE;
     }
-    if (ctx->timing) {
-        (void)hipEventRecord(tev[2], s);
-        ctx->timing_count++;
+    const uint64_t parts = (n + part - 1) / part;
+    if (parts > (uint64_t)cjx_context::kCopyEvents || part % kTileBytes) return CJX_ERR_INPUT;
+    LaunchBufs b = bufs_of(ctx);
+    hipStream_t s = ctx->stream;
+    uint8_t *d_in = ctx->in.as<uint8_t>();
+    // the copies start once earlier work on the context stream (which may
+    // still read the input buffer) is done
+    if (hipEventRecord(ctx->idle_ev, s) != hipSuccess ||
+        hipStreamWaitEvent(ctx->copy_stream, ctx->idle_ev, 0) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    for (uint64_t k = 0; k < parts; k++) {
+        const uint64_t off = k * part, len = std::min<uint64_t>(part, n - off);
+        if (hipMemcpyAsync(d_in + off, host + off, len, hipMemcpyHostToDevice,
+                           ctx->copy_stream) != hipSuccess ||
+            hipEventRecord(ctx->copy_ev[k], ctx->copy_stream) != hipSuccess)
+            return CJX_ERR_DEVICE;
     }
-    return CJX_OK;
+    if (launch_reset(s, b, sh.ntiles, sh.n_units) != hipSuccess) return CJX_ERR_DEVICE;
+    hipEvent_t *tev = ctx->tev[ctx->timing_count % cjx_context::kTimingRing];
+    const bool strict = (flags & CJX_FLAG_UTF8_STRICT) != 0;
+    const bool jsonl = (flags & CJX_FLAG_JSONL) != 0;
+    uint32_t launched = 0;
+    for (uint64_t k = 0; k < parts; k++) {
+        if (hipStreamWaitEvent(s, ctx->copy_ev[k], 0) != hipSuccess) return CJX_ERR_DEVICE;
+        if (k == 0 && ctx->timing) (void)hipEventRecord(tev[0], s);
+        const uint32_t upto =
+            k + 1 == parts ? sh.ntiles : (uint32_t)std::min<uint64_t>(sh.ntiles, (k + 1) * part / kTileBytes);
+        if (launch_tokenize_tiles(s, b, d_in, upto - launched, jsonl, !strict, d_struct) !=
+            hipSuccess)
+            return CJX_ERR_DEVICE;
+        launched = upto;
+    }
+    if (strict && launch_utf8_strict(s, b, d_in, sh.ntiles, sh.n_small) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    if (ctx->timing) (void)hipEventRecord(tev[1], s);
+    return enqueue_back(ctx, sh, flags, d_pair);
 }
 
 // enqueue_parse, or (CJX_FLAG_GRAPH) a replay of the same sequence captured
@@ -314,6 +441,17 @@ void zero_result(cjx_result *out) {
 // pinned staging buffer when there are many small ones) and parses them,
 // retrying once with full output regions if the first guess overflows.
 // Leaves the rows in ctx->structural / ctx->pair and the status in h_uout/h_ls.
+// Part size for copying a single large unit in parts (0: copy it whole):
+// units of at least 64 MiB go in at most 64 parts of at least 16 MiB, whole
+// tiles each.
+uint64_t chunked_part(const std::vector<UnitDesc> &units) {
+    constexpr uint64_t kMinChunked = 64ull << 20, kMinPart = 16ull << 20;
+    if (units.size() != 1 || units[0].n < kMinChunked) return 0;
+    uint64_t part = std::max<uint64_t>(kMinPart, (units[0].n + 63) / 64);
+    part = (part + kTileBytes - 1) / kTileBytes * kTileBytes;
+    return part;
+}
+
 int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint64_t> &offs,
                    const std::vector<uint64_t> &sizes, const std::vector<uint64_t> &pos_base,
                    unsigned flags, bool concat, std::vector<UnitDesc> &units, float *ms) {
@@ -327,7 +465,11 @@ int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint
     if (total == 0 || total > kMaxLaunchBytes) return CJX_ERR_INPUT;
     if (!ctx->in.ensure(dev_bytes)) return CJX_ERR_ALLOC;
     hipStream_t s = ctx->stream;
-    if (units.size() > 16) {  // one pinned staging copy and one H2D instead of one per unit
+    // One large unit: copy in parts overlapped with the tokenizer (first
+    // attempt only; a re-run with exact regions finds the input in place).
+    const uint64_t part = chunked_part(units);
+    if (part) {
+    } else if (units.size() > 16) {  // one pinned staging copy and one H2D instead of one per unit
         if (!ctx->h_stage.ensure(dev_bytes)) return CJX_ERR_ALLOC;
         uint8_t *st = static_cast<uint8_t *>(ctx->h_stage.p);
         if (hipStreamSynchronize(s) != hipSuccess) return CJX_ERR_DEVICE;  // staging reuse
@@ -351,8 +493,13 @@ int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint
             return CJX_ERR_ALLOC;
         if ((rc = upload_units(ctx, units, ntiles, n_small))) return rc;
         (void)hipEventRecord(ctx->ev0, s);
-        rc = enqueue_parse(ctx, ctx->in.as<uint8_t>(), sh, flags, ctx->structural.as<uint32_t>(),
-                           ctx->pair.as<int32_t>());
+        if (part && attempt == 0)  // device_ms then includes waiting for the copies
+            rc = enqueue_parse_chunked(ctx, data + offs[0], sizes[0], sh, flags,
+                                       ctx->structural.as<uint32_t>(), ctx->pair.as<int32_t>(),
+                                       part);
+        else
+            rc = enqueue_parse(ctx, ctx->in.as<uint8_t>(), sh, flags,
+                               ctx->structural.as<uint32_t>(), ctx->pair.as<int32_t>());
         if (rc) return rc;
         (void)hipEventRecord(ctx->ev1, s);
         if ((rc = read_status(ctx, sh.n_units))) return rc;
@@ -382,7 +529,7 @@ int collect_records(cjx_context *ctx, uint32_t n_units, cjx_result *out) {
     if (R == 0) return CJX_OK;
     if (!ctx->recs.ensure(R * sizeof(RecordOut) + 16)) return CJX_ERR_ALLOC;
     cjx_record *h = nullptr;
-    if (hipHostMalloc((void **)&h, R * sizeof(cjx_record), hipHostMallocDefault) != hipSuccess)
+    if (!(h = static_cast<cjx_record *>(pinned_get(R * sizeof(cjx_record)))))
         return CJX_ERR_ALLOC;
     if (launch_records(s, b, ctx->in.as<uint8_t>(), n_units, ctx->structural.as<uint32_t>(),
                        ctx->rec_count.as<uint32_t>(), ctx->recs.as<RecordOut>(), R, true) !=
@@ -390,7 +537,7 @@ int collect_records(cjx_context *ctx, uint32_t n_units, cjx_result *out) {
         hipMemcpyAsync(h, ctx->recs.p, R * sizeof(cjx_record), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
-        (void)hipHostFree(h);
+        pinned_put(h);
         return CJX_ERR_DEVICE;
     }
     out->records = h;
@@ -425,7 +572,7 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
     // Host layout, parse_standard_json.cu:1659-1680 / parse_json_lines.cu:1070-1089.
     int32_t *buf = nullptr;
     const size_t cells = 2 * (N + 2);
-    if (hipHostMalloc((void **)&buf, cells * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+    if (!(buf = static_cast<int32_t *>(pinned_get(cells * sizeof(int32_t)))))
         return CJX_ERR_ALLOC;
     bool ok = true;
     if (units.size() == 1) {  // one region starting at row 0: copy it as is
@@ -442,7 +589,7 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
              hipMemcpyAsync(buf, ctx->layout.p, cells * 4, hipMemcpyDeviceToHost, s) == hipSuccess;
     }
     if (!ok || hipStreamSynchronize(s) != hipSuccess) {
-        (void)hipHostFree(buf);
+        pinned_put(buf);
         return ok ? CJX_ERR_DEVICE : CJX_ERR_ALLOC;
     }
     buf[0] = 0;
@@ -454,7 +601,7 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
         const int rr = collect_records(ctx, (uint32_t)units.size(), out);
         if (rr) {
             out->buf = nullptr;
-            (void)hipHostFree(buf);
+            pinned_put(buf);
             return rr;
         }
     }
@@ -488,8 +635,7 @@ int parse_host_batch(cjx_context *ctx, const uint8_t *data, const std::vector<ui
     const LaunchState *ls = static_cast<const LaunchState *>(ctx->h_ls.p);
     const uint64_t N = ls->n_total;
     int32_t *rows = nullptr;
-    if (hipHostMalloc((void **)&rows, (2 * N + 2) * sizeof(int32_t), hipHostMallocDefault) !=
-        hipSuccess)
+    if (!(rows = static_cast<int32_t *>(pinned_get((2 * N + 2) * sizeof(int32_t)))))
         return CJX_ERR_ALLOC;
     bool ok = true;
     if (N) {
@@ -502,7 +648,7 @@ int parse_host_batch(cjx_context *ctx, const uint8_t *data, const std::vector<ui
                  hipSuccess;
     }
     if (!ok || hipStreamSynchronize(s) != hipSuccess) {
-        (void)hipHostFree(rows);
+        pinned_put(rows);
         return ok ? CJX_ERR_DEVICE : CJX_ERR_ALLOC;
     }
     out->structural = rows;
@@ -594,6 +740,13 @@ void cjx_context_destroy(cjx_context *ctx) {
     for (auto &tr : ctx->tev)
         for (hipEvent_t e : tr)
             if (e) (void)hipEventDestroy(e);
+    if (ctx->copy_stream) {
+        (void)hipStreamSynchronize(ctx->copy_stream);
+        (void)hipStreamDestroy(ctx->copy_stream);
+    }
+    for (hipEvent_t e : ctx->copy_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->idle_ev) (void)hipEventDestroy(ctx->idle_ev);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -755,8 +908,8 @@ int cjx_parse_lines(cjx_context *ctx, const uint8_t *data, size_t n, const size_
 
 void cjx_free_result(cjx_result *r) {
     if (!r) return;
-    if (r->buf) (void)hipHostFree(r->buf);
-    if (r->records) (void)hipHostFree(r->records);
+    pinned_put(r->buf);
+    pinned_put(r->records);
     std::free(r->result_sizes);
     std::free(r->result_sizes_prefix);
     std::memset(r, 0, sizeof(*r));
@@ -824,7 +977,7 @@ int cjx_parse_batch(cjx_context *ctx, const uint8_t *data, const size_t *doc_off
 
 void cjx_free_batch_result(cjx_batch_result *r) {
     if (!r) return;
-    if (r->structural) (void)hipHostFree(r->structural);
+    pinned_put(r->structural);
     std::free(r->docs);
     std::memset(r, 0, sizeof(*r));
 }

@@ -91,6 +91,12 @@ struct cjx_context {
     DevBuf oc_char, oc_idx, keys, sorted, tmp, spos, block_delta, hist;
     DevBuf ck, gmin, gmin1, gmin2, gagg, gd0, lows, highs;
     PinnedBuf h_units, h_uout, h_ls, h_stage;
+    // Chunked host input: a second stream copies the input in parts while the
+    // tokenizer runs on the parts already in (created on first use).
+    static constexpr int kCopyEvents = 64;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_ev[kCopyEvents] = {};
+    hipEvent_t idle_ev = nullptr;
     std::vector<uint64_t> batch_off;  // byte offsets of the documents of the batch in flight
     // Shape of the launch in flight (device form).
     uint32_t n_units = 0;

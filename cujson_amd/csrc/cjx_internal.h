@@ -144,6 +144,11 @@ hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uin
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
                            uint32_t ntiles, uint32_t n_small, bool jsonl, bool weak_utf8,
                            uint32_t *out_struct);
+// `count` more big-tile workgroups (tickets continue from the launches before
+// on the same stream), no small units: the tokenizer of a launch split into
+// parts that each wait for their input bytes.
+hipError_t launch_tokenize_tiles(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
+                                 uint32_t count, bool jsonl, bool weak_utf8, uint32_t *out_struct);
 hipError_t launch_utf8_strict(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
                               uint32_t ntiles, uint32_t n_small);
 hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool concat);

@@ -703,6 +703,21 @@ hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_
     return hipGetLastError();
 }
 
+hipError_t launch_tokenize_tiles(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
+                                 uint32_t count, bool jsonl, bool weak_utf8, uint32_t *out_struct) {
+    if (count == 0) return hipSuccess;
+    const uint32_t weak = weak_utf8 ? 1u : 0u;
+    if (jsonl)
+        hipLaunchKernelGGL((k_tokenize<true, false>), dim3(count), dim3(kTokThreads), 0, s, d_in,
+                           b.units, b.tile_unit, b.small_units, 0u, b.tile_st, b.ls, b.uout,
+                           out_struct, b.oc_char, b.oc_idx, weak);
+    else
+        hipLaunchKernelGGL((k_tokenize<false, false>), dim3(count), dim3(kTokThreads), 0, s, d_in,
+                           b.units, b.tile_unit, b.small_units, 0u, b.tile_st, b.ls, b.uout,
+                           out_struct, b.oc_char, b.oc_idx, weak);
+    return hipGetLastError();
+}
+
 }  // namespace cjx
 
 // Host-side model of the byte classifier (same tables, v_perm emulated), for

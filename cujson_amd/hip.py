@@ -36,6 +36,8 @@ def hip():
         L.hipEventRecord.argtypes = [vp, vp]
         L.hipEventSynchronize.argtypes = [vp]
         L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+        L.hipHostMalloc.argtypes = [ctypes.POINTER(vp), sz, ctypes.c_uint]
+        L.hipHostFree.argtypes = [vp]
         L.hipGetErrorString.argtypes = [ctypes.c_int]
         L.hipGetErrorString.restype = ctypes.c_char_p
         _hip = L
@@ -108,3 +110,26 @@ class Event:
 
 def synchronize():
     _check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class PinnedHostBuffer:
+    """Pinned host memory (hipHostMalloc) viewed as a numpy uint8 array."""
+
+    def __init__(self, nbytes: int):
+        p = ctypes.c_void_p()
+        _check(hip().hipHostMalloc(ctypes.byref(p), max(nbytes, 1), 0), "hipHostMalloc")
+        self.ptr = p.value
+        self.nbytes = nbytes
+        self.array = np.ctypeslib.as_array((ctypes.c_uint8 * max(nbytes, 1)).from_address(self.ptr))
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            hip().hipHostFree(ctypes.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

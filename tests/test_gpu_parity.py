@@ -190,6 +190,36 @@ def test_config2_shape_64mb(ctx):
     check_standard(ctx, d, label="config2 64MB")
 
 
+@pytest.fixture(scope="module")
+def doc96():
+    d = gen.config2_parallel(total_bytes=96 << 20)
+    assert d.size >= 64 << 20  # large enough for the chunked host copy
+    return d
+
+
+@pytest.mark.parametrize("flags", [0, oracle.FLAG_UTF8_STRICT, oracle.FLAG_DEPTH_U16])
+def test_chunked_host_input(ctx, doc96, tmp_path, flags):
+    """Host input of at least 64 MiB is copied in parts while the tokenizer
+    runs on the parts already in (one tokenizer launch per part): pageable
+    input and pinned input (loadJSON) both equal the oracle."""
+    o = oracle.parse_standard(doc96, flags)
+    for label, inp in (("pageable", doc96), ("pinned", None)):
+        if inp is None:
+            p = tmp_path / "doc.json"
+            p.write_bytes(doc96.tobytes())
+            inp = cj.loadJSON(str(p))
+        g = cj.parse_standard_json(inp, flags, ctx)
+        assert g.error == o.code, (label, g.error, o.code)
+        assert g.N == o.N, label
+        assert np.array_equal(g.buf, o.buf), label
+
+
+def test_chunked_host_input_fault(ctx, doc96):
+    d = doc96.copy()
+    assert gen.plant_fault(d, 3, 99)
+    check_standard(ctx, d, label="chunked with a fault")
+
+
 def test_jsonl_generated_records(ctx):
     d = gen.lines(3000)
     data = bytes(d)
