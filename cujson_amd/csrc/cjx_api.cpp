@@ -246,6 +246,7 @@ LaunchBufs bufs_of(cjx_context *ctx) {
     b.gd0 = ctx->gd0.as<int32_t>();
     b.lows = ctx->lows.as<uint32_t>();
     b.highs = ctx->highs.as<uint32_t>();
+    b.row_nl = ctx->want_nl ? ctx->row_nl.as<uint8_t>() : nullptr;
     return b;
 }
 
@@ -485,12 +486,15 @@ int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint
     }
 
     Shape sh{ntiles, (uint32_t)units.size(), n_small, 0, 0, 0, 0, concat};
+    const bool records = (flags & CJX_FLAG_JSONL) && (flags & CJX_FLAG_RECORDS);
     for (int attempt = 0; attempt < 2; attempt++) {
         set_regions(units, attempt == 1);
         shape_rows(sh, units);
         if ((rc = ensure_work(ctx, sh))) return rc;
-        if (!ctx->structural.ensure(sh.rows * 4 + 16) || !ctx->pair.ensure(sh.rows * 4 + 16))
+        if (!ctx->structural.ensure(sh.rows * 4 + 16) || !ctx->pair.ensure(sh.rows * 4 + 16) ||
+            (records && !ctx->row_nl.ensure(sh.rows + 16)))
             return CJX_ERR_ALLOC;
+        ctx->want_nl = records;  // the tokenizer also writes the newline flags
         if ((rc = upload_units(ctx, units, ntiles, n_small))) return rc;
         (void)hipEventRecord(ctx->ev0, s);
         if (part && attempt == 0)  // device_ms then includes waiting for the copies
@@ -500,6 +504,7 @@ int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint
         else
             rc = enqueue_parse(ctx, ctx->in.as<uint8_t>(), sh, flags,
                                ctx->structural.as<uint32_t>(), ctx->pair.as<int32_t>());
+        ctx->want_nl = false;
         if (rc) return rc;
         (void)hipEventRecord(ctx->ev1, s);
         if ((rc = read_status(ctx, sh.n_units))) return rc;
@@ -517,9 +522,11 @@ int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint
 // fill the table on the device, copy it out.
 int collect_records(cjx_context *ctx, uint32_t n_units, cjx_result *out) {
     hipStream_t s = ctx->stream;
+    ctx->want_nl = true;  // the parse just run wrote the newline flags
     LaunchBufs b = bufs_of(ctx);
+    ctx->want_nl = false;
     if (!ctx->rec_count.ensure((size_t)n_units * 4 + 16)) return CJX_ERR_ALLOC;
-    if (launch_records(s, b, ctx->in.as<uint8_t>(), n_units, ctx->structural.as<uint32_t>(),
+    if (launch_records(s, b, n_units, ctx->structural.as<uint32_t>(),
                        ctx->rec_count.as<uint32_t>(), nullptr, 0, false) != hipSuccess)
         return CJX_ERR_DEVICE;
     int rc = read_status(ctx, n_units);
@@ -531,7 +538,7 @@ int collect_records(cjx_context *ctx, uint32_t n_units, cjx_result *out) {
     cjx_record *h = nullptr;
     if (!(h = static_cast<cjx_record *>(pinned_get(R * sizeof(cjx_record)))))
         return CJX_ERR_ALLOC;
-    if (launch_records(s, b, ctx->in.as<uint8_t>(), n_units, ctx->structural.as<uint32_t>(),
+    if (launch_records(s, b, n_units, ctx->structural.as<uint32_t>(),
                        ctx->rec_count.as<uint32_t>(), ctx->recs.as<RecordOut>(), R, true) !=
             hipSuccess ||
         hipMemcpyAsync(h, ctx->recs.p, R * sizeof(cjx_record), hipMemcpyDeviceToHost, s) !=
@@ -724,7 +731,7 @@ void cjx_context_destroy(cjx_context *ctx) {
                       &ctx->ck,      &ctx->gmin,       &ctx->gmin1,  &ctx->gmin2,
                       &ctx->gagg,    &ctx->gd0,        &ctx->lows,   &ctx->highs,
                       &ctx->chunk_unit, &ctx->big_units, &ctx->unit_part, &ctx->rec_count,
-                      &ctx->recs};
+                      &ctx->recs,       &ctx->row_nl};
     for (DevBuf *b : bufs) b->release();
     ctx->h_units.release();
     ctx->h_uout.release();
@@ -1041,21 +1048,28 @@ int parse_units_device(cjx_context *ctx, const uint8_t *d_input, const size_t *d
     ctx->n_units = (uint32_t)n_docs;
     ctx->batch_off.assign(doc_off, doc_off + n_docs);
     (void)hipEventRecord(ctx->ev0, ctx->stream);
+    const bool records = jsonl && (flags & CJX_FLAG_RECORDS) && d_records;
+    if (records && (!ctx->row_nl.ensure(sh.rows + 16) || !ctx->rec_count.ensure(n_docs * 4 + 16)))
+        return CJX_ERR_ALLOC;
+    ctx->want_nl = records;  // the tokenizer also writes the newline flags
     rc = enqueue_maybe_graph(ctx, d_input, sh, flags & ~CJX_FLAG_RECORDS,
                              reinterpret_cast<uint32_t *>(d_structural), d_pair);
-    if (!rc && jsonl && (flags & CJX_FLAG_RECORDS) && d_records) {
-        LaunchBufs b = bufs_of(ctx);
-        if (!ctx->rec_count.ensure(n_docs * 4 + 16)) return CJX_ERR_ALLOC;
-        b = bufs_of(ctx);
-        if (launch_records(ctx->stream, b, d_input, (uint32_t)n_docs,
+    if (!rc && records) {
+        const LaunchBufs b = bufs_of(ctx);
+        if (launch_records(ctx->stream, b, (uint32_t)n_docs,
                            reinterpret_cast<uint32_t *>(d_structural),
                            ctx->rec_count.as<uint32_t>(), nullptr, 0, false) != hipSuccess ||
-            launch_records(ctx->stream, b, d_input, (uint32_t)n_docs,
+            launch_records(ctx->stream, b, (uint32_t)n_docs,
                            reinterpret_cast<uint32_t *>(d_structural),
                            ctx->rec_count.as<uint32_t>(), reinterpret_cast<RecordOut *>(d_records),
                            max_records, true) != hipSuccess)
-            return CJX_ERR_DEVICE;
+            rc = CJX_ERR_DEVICE;
+        // the timing ring's end event of this launch moves behind the table
+        if (!rc && ctx->timing && ctx->timing_count)
+            (void)hipEventRecord(
+                ctx->tev[(ctx->timing_count - 1) % cjx_context::kTimingRing][2], ctx->stream);
     }
+    ctx->want_nl = false;
     (void)hipEventRecord(ctx->ev1, ctx->stream);
     ctx->events_recorded = true;
     return rc;

@@ -90,6 +90,8 @@ struct cjx_context {
     uint64_t up_count = 0;
     DevBuf oc_char, oc_idx, keys, sorted, tmp, spos, block_delta, hist;
     DevBuf ck, gmin, gmin1, gmin2, gagg, gd0, lows, highs;
+    DevBuf row_nl;         // newline flag per structural row (record tables)
+    bool want_nl = false;  // the launch being enqueued writes row_nl
     PinnedBuf h_units, h_uout, h_ls, h_stage;
     // Chunked host input: a second stream copies the input in parts while the
     // tokenizer runs on the parts already in (created on first use).

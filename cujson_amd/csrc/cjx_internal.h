@@ -136,6 +136,8 @@ struct LaunchBufs {
     int32_t *gd0;     // per 64 chunks: start depth
     uint32_t *lows;   // per chunk x key: first closer reaching that depth (bracket index)
     uint32_t *highs;  // per chunk x key: last opener left open
+    uint8_t *row_nl;  // JSON Lines with a record table: 1 per structural row that is a '\n'
+                      // (written by the tokenizer), else null
 };
 
 hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uint32_t n_units);
@@ -177,9 +179,9 @@ hipError_t launch_sort_units(hipStream_t s, const LaunchBufs &b, uint32_t n_unit
 // Per-record table of JSON Lines units: write = false counts the records of
 // each unit and numbers them (rec_count becomes the bases, ls->n_records the
 // total); write = true fills rec.
-hipError_t launch_records(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                          uint32_t n_units, const uint32_t *d_struct, uint32_t *rec_count,
-                          RecordOut *rec, uint64_t max_records, bool write);
+hipError_t launch_records(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
+                          const uint32_t *d_struct, uint32_t *rec_count, RecordOut *rec,
+                          uint64_t max_records, bool write);
 hipError_t launch_phase3(hipStream_t s, const LaunchBufs &b, uint32_t u, uint32_t G,
                          uint32_t pair_grid, bool depth_u16, int32_t *pair_row);
 

@@ -348,7 +348,7 @@ __global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
     const uint32_t *__restrict__ tile_unit, const uint32_t *__restrict__ small_units,
     uint32_t n_small, unsigned long long *__restrict__ tile_st, LaunchState *__restrict__ ls,
     UnitOut *__restrict__ uo, uint32_t *__restrict__ out_struct, uint8_t *__restrict__ oc_char,
-    uint32_t *__restrict__ oc_idx, uint32_t weak_utf8) {
+    uint32_t *__restrict__ oc_idx, uint8_t *__restrict__ row_nl, uint32_t weak_utf8) {
     __shared__ uint32_t s_tile, s_unit;
     __shared__ uint32_t s_map[kWaves];
     __shared__ uint32_t s_cnt0[kWaves], s_cnt_all[kWaves];
@@ -358,7 +358,11 @@ __global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
     // Bracket characters as two bit masks per lane-round, kept in LDS until
     // compaction: byte bit 1 ('[' / '{' have it, ']' / '}' do not) and byte
     // bit 5 (curly), both restricted to bracket bytes.
+    // For JSON Lines s_b5 also holds the '\n' bytes, which are never brackets:
+    // class 0 without byte bit 5 (':' has it, '\n' not); they give the
+    // per-row newline flags of record tables (row_nl).
     __shared__ uint32_t s_b1[kWaves][kTokRounds][64], s_b5[kWaves][kTokRounds][64];
+    const bool want_nl = JSONL && row_nl != nullptr;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t w = tid >> 6, lane = tid & 63;
@@ -464,7 +468,7 @@ __global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
         br_r[i] = k[2];
         op_r[i] = k[0] | k[2] | k[3];
         s_b1[w][i][lane] = k[1] & k[2];
-        s_b5[w][i][lane] = k[5] & k[2];
+        s_b5[w][i][lane] = (k[5] & k[2]) | (JSONL ? (k[0] & ~k[5]) : 0u);
 
         const uint32_t all_bs = (bs == 0xFFFFFFFFu);
         const uint32_t run = (uint32_t)__clz(~bs);  // trailing backslash run of the lane
@@ -598,26 +602,41 @@ __global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
         // than kStageRows of them, or one that reaches the end of the region,
         // is stored lane by lane.
         const uint32_t rn = rtot & 0xFFFFu;
+        const uint32_t nlm = want_nl ? (s_b5[w][i][lane] & ~br_r[i]) : 0u;
         if (rn <= (uint32_t)kStageRows && (uint64_t)nb + rn <= cap_end) {
             uint32_t k = excl & 0xFFFFu;
-            while (m) {
-                const uint32_t j = (uint32_t)__builtin_ctz(m);
-                m &= m - 1;
-                s_stage[w][k++] = pos0 + j;
+            if (want_nl) {  // and the row's newline flag (in s_bchar, free until the brackets)
+                while (m) {
+                    const uint32_t j = (uint32_t)__builtin_ctz(m);
+                    m &= m - 1;
+                    s_stage[w][k] = pos0 + j;
+                    s_bchar[w][k] = (uint8_t)((nlm >> j) & 1u);
+                    k++;
+                }
+            } else {
+                while (m) {
+                    const uint32_t j = (uint32_t)__builtin_ctz(m);
+                    m &= m - 1;
+                    s_stage[w][k++] = pos0 + j;
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             for (uint32_t t = lane; t < rn; t += 64) out_struct[nb + t] = s_stage[w][t];
+            if (want_nl)
+                for (uint32_t t = lane; t < rn; t += 64) row_nl[nb + t] = s_bchar[w][t];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         } else if ((uint64_t)idx0 + (uint32_t)__popc(sti) <= cap_end) {
-            uint32_t *p = out_struct + idx0;
+            uint32_t idx = idx0;
             while (m) {
                 const uint32_t j = (uint32_t)__builtin_ctz(m);
                 m &= m - 1;
-                *p++ = pos0 + j;
+                out_struct[idx] = pos0 + j;
+                if (want_nl) row_nl[idx] = (uint8_t)((nlm >> j) & 1u);
+                idx++;
             }
         } else if (m) {
             overflow = 1;
@@ -625,7 +644,10 @@ __global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
             while (m) {
                 const uint32_t j = (uint32_t)__builtin_ctz(m);
                 m &= m - 1;
-                if (idx < cap_end) out_struct[idx] = pos0 + j;
+                if (idx < cap_end) {
+                    out_struct[idx] = pos0 + j;
+                    if (want_nl) row_nl[idx] = (uint8_t)((nlm >> j) & 1u);
+                }
                 idx++;
             }
         }
@@ -685,20 +707,20 @@ hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_
         if (ntiles)
             hipLaunchKernelGGL((k_tokenize<true, false>), dim3(ntiles), dim3(kTokThreads), 0, s,
                                d_in, b.units, b.tile_unit, b.small_units, n_small, b.tile_st, b.ls,
-                               b.uout, out_struct, b.oc_char, b.oc_idx, weak);
+                               b.uout, out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
         if (n_small)
             hipLaunchKernelGGL((k_tokenize<true, true>), dim3(gs), dim3(kTokThreads), 0, s, d_in,
                                b.units, b.tile_unit, b.small_units, n_small, b.tile_st, b.ls,
-                               b.uout, out_struct, b.oc_char, b.oc_idx, weak);
+                               b.uout, out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
     } else {
         if (ntiles)
             hipLaunchKernelGGL((k_tokenize<false, false>), dim3(ntiles), dim3(kTokThreads), 0, s,
                                d_in, b.units, b.tile_unit, b.small_units, n_small, b.tile_st, b.ls,
-                               b.uout, out_struct, b.oc_char, b.oc_idx, weak);
+                               b.uout, out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
         if (n_small)
             hipLaunchKernelGGL((k_tokenize<false, true>), dim3(gs), dim3(kTokThreads), 0, s, d_in,
                                b.units, b.tile_unit, b.small_units, n_small, b.tile_st, b.ls,
-                               b.uout, out_struct, b.oc_char, b.oc_idx, weak);
+                               b.uout, out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
     }
     return hipGetLastError();
 }
@@ -710,11 +732,11 @@ hipError_t launch_tokenize_tiles(hipStream_t s, const LaunchBufs &b, const uint8
     if (jsonl)
         hipLaunchKernelGGL((k_tokenize<true, false>), dim3(count), dim3(kTokThreads), 0, s, d_in,
                            b.units, b.tile_unit, b.small_units, 0u, b.tile_st, b.ls, b.uout,
-                           out_struct, b.oc_char, b.oc_idx, weak);
+                           out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
     else
         hipLaunchKernelGGL((k_tokenize<false, false>), dim3(count), dim3(kTokThreads), 0, s, d_in,
                            b.units, b.tile_unit, b.small_units, 0u, b.tile_st, b.ls, b.uout,
-                           out_struct, b.oc_char, b.oc_idx, weak);
+                           out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
     return hipGetLastError();
 }
 

@@ -132,3 +132,22 @@ def test_device_lines_match_host_lines(ctx):
     rc, docs, N, M, over = ctx.batch_wait()
     assert rc == cj.CJX_ERR_CAPACITY and over
     assert np.all(small.download(np.uint8, 64, 16 * cj.RECORD_DTYPE.itemsize) == 0x5A)
+
+
+def test_records_over_dirty_device_memory(twitter_small):
+    """A fresh context whose buffers reuse memory that held other bytes: the
+    record table depends only on the rows of the units (rows beside them in
+    the flag buffer hold leftovers)."""
+    from cujson_amd.hip import DeviceBuffer
+    junk = [DeviceBuffer(64 << 20) for _ in range(4)]
+    for j in junk:
+        j.fill(0xA7)
+    for j in junk:
+        j.free()
+    c = cj.Context(0)
+    try:
+        for cc in (3, 7, 51):
+            check_records(c, twitter_small, oracle.chunks_by_count(twitter_small, cc),
+                          f"dirty cc={cc}")
+    finally:
+        c.close()
