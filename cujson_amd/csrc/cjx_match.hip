@@ -300,8 +300,9 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_local(
     __shared__ __align__(16) uint8_t s_ch[kMatchChunk];
     __shared__ uint32_t s_idx[kMatchChunk];
     __shared__ uint16_t s_part[kMatchChunk];
-    // Stack columns and low closers during matching; the row tile afterwards.
-    __shared__ union {
+    // Stack columns and low closers during matching; the row tile afterwards
+    // (16-byte aligned: the tile is filled and read as int4).
+    __shared__ __align__(16) union {
         struct {
             uint16_t stk[kLaneBrackets][kMatchThreads];
             uint16_t low[kLaneBrackets][kMatchThreads];
@@ -449,9 +450,15 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_local(
             const uint32_t p = s_part[e];
             pv[k] = p != kNoPart ? (int32_t)(s_idx[p] - nb + pbase) : -1;
         }
-        for (uint32_t t0 = r0; t0 < r1; t0 += kRowTile) {
+        // Tiles start at a multiple of 4 rows so that the fill and the store
+        // move 16 bytes per access; rows before r0 or from r1 on belong to the
+        // neighbouring chunks and are skipped (edge groups are stored by row).
+        const bool vec = (reinterpret_cast<uintptr_t>(pair_row) & 15u) == 0;
+        int4 *const rw = reinterpret_cast<int4 *>(s_u.row);
+        for (uint32_t t0 = vec ? (r0 & ~3u) : r0; t0 < r1; t0 += kRowTile) {
             __syncthreads();  // matching state / previous tile no longer read
-            for (uint32_t i = tid; i < (uint32_t)kRowTile; i += kMatchThreads) s_u.row[i] = -1;
+            for (uint32_t i = tid; i < (uint32_t)kRowTile / 4; i += kMatchThreads)
+                rw[i] = make_int4(-1, -1, -1, -1);
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < kLaneBrackets; k++) {
@@ -459,8 +466,22 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_local(
                 if (pv[k] >= 0 && off < (uint32_t)kRowTile) s_u.row[off] = pv[k];
             }
             __syncthreads();
-            const uint32_t len = min((uint32_t)kRowTile, r1 - t0);
-            for (uint32_t i = tid; i < len; i += kMatchThreads) pair_row[t0 + i] = s_u.row[i];
+            const uint32_t hi = min(r1, t0 + (uint32_t)kRowTile);
+            if (vec) {
+                for (uint32_t g = tid; g < (uint32_t)kRowTile / 4; g += kMatchThreads) {
+                    const uint32_t a = t0 + 4u * g;
+                    if (a >= hi) break;
+                    if (a >= r0 && a + 4u <= hi) {
+                        *reinterpret_cast<int4 *>(pair_row + a) = rw[g];
+                    } else {
+#pragma unroll
+                        for (uint32_t j = 0; j < 4; j++)
+                            if (a + j >= r0 && a + j < hi) pair_row[a + j] = s_u.row[4u * g + j];
+                    }
+                }
+            } else {
+                for (uint32_t i = tid; t0 + i < hi; i += kMatchThreads) pair_row[t0 + i] = s_u.row[i];
+            }
         }
     }
 }
