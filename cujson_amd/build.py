@@ -82,6 +82,19 @@ def build_wrapper_check() -> str:
     return WRAPPER_CHECK
 
 
+QUERY_CHECK = os.path.join(HERE, "cjx_query_check")
+
+
+def build_query_check() -> str:
+    """The C++ program that runs queries through include/cujson/query_iterator.h."""
+    tmp = QUERY_CHECK + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", tmp,
+          os.path.join(ROOT, "tests", "cpp", "query_check.cpp"), "-L", HERE, "-lcujson_amd",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, QUERY_CHECK)
+    return QUERY_CHECK
+
+
 LOADGEN_LIB = os.path.join(HERE, "libcjx_loadgen.so")
 
 
@@ -103,5 +116,6 @@ if __name__ == "__main__":
     print(build_library(verbose=True))
     print(build_generator())
     print(build_wrapper_check())
+    print(build_query_check())
     print(build_loadgen())
     build_oracle()
