@@ -57,7 +57,12 @@ def calibrate_copy(nbytes: int, reps: int):
           flush=True)
 
 
-def many_docs_bench(cj, gen, n_docs: int, reps: int):
+def _num(x: float, nd: int):
+    """Rounded figure, or None (JSON null) for one not measured (NaN)."""
+    return None if x != x else round(x, nd)
+
+
+def many_docs_bench(cj, gen, n_docs: int, reps: int, plain_compare: bool = True):
     """The 50,000-document batch (Pareto sizes, faults, deep documents),
     device-resident: batched (one launch sequence for all documents) against
     plain (one device-resident parse per document, the reference's
@@ -102,19 +107,21 @@ def many_docs_bench(cj, gen, n_docs: int, reps: int):
     codes = np.bincount(docs[:, 3], minlength=4).tolist()
 
     # plain: one parse per document, enqueued back to back on the same stream
-    synchronize()
-    e0.record(stream)
     lat = []
     ev = [Event() for _ in range(2)]
-    for i in range(n_docs):
+    plain_ms = float("nan")
+    synchronize()
+    e0.record(stream)
+    for i in range(n_docs if plain_compare else 0):
         rc = ctx.parse_device(d_in.ptr + offs_l[i], sizes_l[i], d_st.ptr, d_pr.ptr, cap)
         if rc:
             raise RuntimeError(f"parse_device doc {i}: {cj.strerror(rc)}")
     e1.record(stream)
     e1.synchronize()
-    plain_ms = e0.elapsed_ms(e1)
+    if plain_compare:
+        plain_ms = e0.elapsed_ms(e1)
     # per-document latency of the plain path on a sample (launch to completion)
-    for i in range(0, n_docs, max(1, n_docs // 200)):
+    for i in range(0, n_docs if plain_compare else 0, max(1, n_docs // 200)):
         ev[0].record(stream)
         ctx.parse_device(d_in.ptr + offs_l[i], sizes_l[i], d_st.ptr, d_pr.ptr, cap)
         ev[1].record(stream)
@@ -132,15 +139,15 @@ def many_docs_bench(cj, gen, n_docs: int, reps: int):
         "batched_form": "one HIP graph (captured launch sequence) per batch; batched_ms is "
                         "device time, batched_call_ms adds the host's unit planning and upload",
         "batched_gbs": round(total / (batched_ms * 1e-3) / 1e9, 2),
-        "plain_ms": round(plain_ms, 3),
-        "plain_gbs": round(total / (plain_ms * 1e-3) / 1e9, 3),
-        "batched_vs_plain": round(plain_ms / batched_ms, 1),
-        "plain_doc_p50_ms": round(float(np.percentile(lat, 50)), 4),
-        "plain_doc_p99_ms": round(float(np.percentile(lat, 99)), 4),
+        "plain_ms": _num(plain_ms, 3),
+        "plain_gbs": _num(total / (plain_ms * 1e-3) / 1e9, 3),
+        "batched_vs_plain": _num(plain_ms / batched_ms, 1),
+        "plain_doc_p50_ms": round(float(np.percentile(lat, 50)), 4) if lat else None,
+        "plain_doc_p99_ms": round(float(np.percentile(lat, 99)), 4) if lat else None,
     }
 
 
-def lines_bench(cj, gen, n_records: int, reps: int):
+def lines_bench(cj, gen, n_records: int, reps: int, plain_compare: bool = True):
     """8M JSON Lines records (log-normal sizes, median 400 B, sigma 1.2,
     clipped to 64 KiB, depth 1-6), device-resident, in launches of at most
     2^31 - 1 bytes (positions are int32 in the host layout).  Batched: every
@@ -216,7 +223,7 @@ def lines_bench(cj, gen, n_records: int, reps: int):
     synchronize()
     e0.record(stream)
     n_chunks = 0
-    for L in launches:
+    for L in (launches if plain_compare else []):
         for off, sz in zip(L["offs"], L["sizes"]):
             rc = ctx.parse_lines_device(L["dev"].ptr + int(off), [0], [int(sz)], L["st"].ptr,
                                         L["pr"].ptr, L["cap"])
@@ -225,7 +232,8 @@ def lines_bench(cj, gen, n_records: int, reps: int):
             n_chunks += 1
     e1.record(stream)
     e1.synchronize()
-    plain_ms = e0.elapsed_ms(e1)
+    plain_ms = e0.elapsed_ms(e1) if plain_compare else float("nan")
+    n_chunks = n_chunks or sum(len(L["sizes"]) for L in launches)
     ctx.close()
     return {
         "workload": f"{n_records} JSON Lines records, log-normal sizes (median 400 B, sigma 1.2, "
@@ -238,9 +246,9 @@ def lines_bench(cj, gen, n_records: int, reps: int):
         "batched_form": "all chunks of a launch in one launch sequence, with the per-record "
                         "table; batched_ms is device time (tokenizer start to table end, summed "
                         "over launches), batched_call_ms adds the host's chunk planning",
-        "plain_ms": round(plain_ms, 3),
-        "plain_gbs": round(total / (plain_ms * 1e-3) / 1e9, 2),
-        "batched_vs_plain": round(plain_ms / batched_ms, 2),
+        "plain_ms": _num(plain_ms, 3),
+        "plain_gbs": _num(total / (plain_ms * 1e-3) / 1e9, 2),
+        "batched_vs_plain": _num(plain_ms / batched_ms, 2),
     }
 
 
@@ -371,6 +379,8 @@ def main():
     ap.add_argument("--no-lines", action="store_true",
                     help="skip the JSON Lines batched-vs-plain measurement")
     ap.add_argument("--records", type=int, default=8_000_000)
+    ap.add_argument("--no-plain-compare", action="store_true",
+                    help="skip the plain per-document / per-chunk loops (profiling runs)")
     ap.add_argument("--no-host-path", action="store_true",
                     help="skip the PCIe-inclusive host-path measurement")
     ap.add_argument("--no-serving", action="store_true",
@@ -532,10 +542,12 @@ def main():
     del d_in, d_st, d_pr
     if not args.no_batched:
         log(f"batched vs plain on {args.docs} documents")
-        out["batched"] = many_docs_bench(cj, gen, args.docs, reps=5)
+        out["batched"] = many_docs_bench(cj, gen, args.docs, reps=5,
+                                         plain_compare=not args.no_plain_compare)
     if not args.no_lines:
         log(f"JSON Lines batched vs plain on {args.records} records")
-        out["lines"] = lines_bench(cj, gen, args.records, reps=3)
+        out["lines"] = lines_bench(cj, gen, args.records, reps=3,
+                                   plain_compare=not args.no_plain_compare)
     if not args.no_serving:
         log(f"serving: {args.serve_rate:.0f} req/s for {args.serve_seconds:.0f} s")
         out["serving"] = serving_bench(args.serve_seconds, args.serve_rate,
