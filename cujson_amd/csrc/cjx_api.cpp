@@ -42,9 +42,6 @@ uint32_t pair_blocks(uint64_t max_n) {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 2048));
 }
 
-// Tokenizer work of the units: a unit of at most one wave's bytes is small
-// (one wave, no tiles); the others are cut into tiles.  Returns the tile
-// count and fills n_small.
 // Pinned host buffers for results.  Allocating and pinning a large host
 // buffer per call costs about as much as the copy into it, so released
 // result buffers are kept for reuse: a request takes the smallest cached
@@ -103,6 +100,9 @@ void pinned_put(void *p) {
     (void)hipHostFree(p);
 }
 
+// Tokenizer work of the units: a unit of at most one wave's bytes is small
+// (one wave, no tiles); the others are cut into tiles.  Returns the tile
+// count and fills n_small.
 uint64_t assign_tiles(std::vector<UnitDesc> &units, uint32_t &n_small) {
     uint64_t tiles = 0;
     n_small = 0;

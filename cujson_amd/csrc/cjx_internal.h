@@ -8,7 +8,7 @@ namespace cjx {
 
 // Tokenizer tile geometry: a tile is one workgroup of CJX_TOK_WAVES wave64s;
 // each lane owns 32 contiguous bytes per round, so a wave covers 2 KiB per
-// round and runs CJX_TOK_ROUNDS rounds.  8 x 8 (64 KiB tiles) measured
+// round and runs CJX_TOK_ROUNDS rounds.  8 x 8 (128 KiB tiles) measured
 // fastest on a 2 GB document: fewer tiles shorten the look-back chain.
 #ifndef CJX_TOK_ROUNDS
 #define CJX_TOK_ROUNDS 8
