@@ -234,6 +234,7 @@ def lines_bench(cj, gen, n_records: int, reps: int, plain_compare: bool = True):
     e1.synchronize()
     plain_ms = e0.elapsed_ms(e1) if plain_compare else float("nan")
     n_chunks = n_chunks or sum(len(L["sizes"]) for L in launches)
+    loader = lines_loader_bench(cj, ctx, data, offs, sizes, best, parts, launches)
     ctx.close()
     return {
         "workload": f"{n_records} JSON Lines records, log-normal sizes (median 400 B, sigma 1.2, "
@@ -249,6 +250,79 @@ def lines_bench(cj, gen, n_records: int, reps: int, plain_compare: bool = True):
         "plain_ms": _num(plain_ms, 3),
         "plain_gbs": _num(total / (plain_ms * 1e-3) / 1e9, 2),
         "batched_vs_plain": _num(plain_ms / batched_ms, 2),
+        "loader": loader,
+    }
+
+
+def lines_loader_bench(cj, ctx, data, offs, sizes, chunk_bytes, parts, launches, reps: int = 3):
+    """The loader's line scan and chunking (load_file.cu:72-103, :153-176) on
+    the whole text resident in HBM, against the host chunker on the same
+    bytes; then the first launch's chunks parsed where they lie in that text
+    (unaligned offsets, relocated by the library) against its aligned copy."""
+    import time
+    from cujson_amd.hip import DeviceBuffer, Event
+    total = int(data.size)
+    dev = DeviceBuffer(total + 64)
+    dev.upload(data)
+    stream = ctx.stream
+    rc, count = ctx.scan_lines_device(dev.ptr, total, 0, 0)
+    nl = DeviceBuffer(count * 8 + 64)
+    e0, e1 = Event(), Event()
+
+    def timed(fn):
+        out = []
+        for _ in range(reps):
+            e0.record(stream)
+            r = fn()
+            e1.record(stream)
+            e1.synchronize()
+            out.append(e0.elapsed_ms(e1))
+        return float(np.median(out)), r
+
+    scan_ms, (rc, k) = timed(lambda: ctx.scan_lines_device(dev.ptr, total, nl.ptr, count))
+    if rc or k != count:
+        raise RuntimeError(f"newline scan: {cj.strerror(rc)}")
+    chunk_ms, (doffs, dsizes) = timed(lambda: ctx.chunk_lines_device(dev.ptr, total, 1,
+                                                                      chunk_bytes))
+    if not (np.array_equal(doffs, offs) and np.array_equal(dsizes, sizes)):
+        raise RuntimeError("device chunking differs from the host chunker")
+    count_ms, _ = timed(lambda: ctx.chunk_lines_device(dev.ptr, total, 0, len(offs)))
+    t0 = time.perf_counter()
+    cj.chunk_lines(data, 1, chunk_bytes)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    a, b = parts[0]
+    L = launches[0]
+
+    def launch(ptr, o):
+        rc = ctx.parse_lines_device(ptr, o, sizes[a:b], L["st"].ptr, L["pr"].ptr, L["cap"],
+                                    L["rec"].ptr, L["max_rec"])
+        if rc:
+            raise RuntimeError(f"parse_lines_device: {cj.strerror(rc)}")
+
+    aligned_ms, _ = timed(lambda: launch(L["dev"].ptr, L["offs"]))
+    _, _, n_al, m_al, _ = ctx.batch_wait()
+    reloc_ms, _ = timed(lambda: launch(dev.ptr, offs[a:b]))
+    rc, _, n_re, m_re, over = ctx.batch_wait()
+    if rc or over or (n_re, m_re) != (n_al, m_al):
+        raise RuntimeError("relocated launch differs from the aligned one")
+    scan_bytes = 2 * total + 8 * count
+    dev.free()
+    nl.free()
+    return {
+        "newlines": int(count),
+        "scan_ms": round(scan_ms, 3),
+        "scan_gbs": round(total / (scan_ms * 1e-3) / 1e9, 1),
+        "scan_hbm_frac": round(scan_bytes / (scan_ms * 1e-3) / 8e12, 3),
+        "chunk_by_size_ms": round(chunk_ms, 3),
+        "chunk_by_count_ms": round(count_ms, 3),
+        "host_chunk_by_size_ms": round(host_ms, 1),
+        "relocated_launch_ms": round(reloc_ms, 3),
+        "aligned_launch_ms": round(aligned_ms, 3),
+        "form": "device text of the whole file; scan = count + write passes, synchronous calls "
+                "timed by events on the context stream (host round trip included); chunk_by_size "
+                "copies the newline offsets to the host for the greedy size rule; the launches "
+                "are the first launch's chunks with the per-record table, in place (relocated) "
+                "and in the aligned copy",
     }
 
 

@@ -59,6 +59,7 @@ ABI_SYMBOLS = (
     "cjx_parse_batch", "cjx_free_batch_result", "cjx_parse_batch_device", "cjx_batch_wait",
     "cjx_parse_lines_device", "cjx_server_create", "cjx_server_submit", "cjx_server_wait",
     "cjx_server_release", "cjx_server_reserve", "cjx_server_get_stats", "cjx_server_destroy",
+    "cjx_scan_lines_device", "cjx_chunk_lines_device", "cjx_load_device",
 )
 
 
@@ -187,6 +188,12 @@ def lib():
     L.cjx_batch_wait.restype = ctypes.c_int
     L.cjx_parse_lines_device.argtypes = [vp, vp, szp, szp, sz, ctypes.c_uint, vp, vp, sz, vp, sz]
     L.cjx_parse_lines_device.restype = ctypes.c_int
+    L.cjx_scan_lines_device.argtypes = [vp, vp, sz, vp, sz, szp]
+    L.cjx_scan_lines_device.restype = ctypes.c_int
+    L.cjx_chunk_lines_device.argtypes = [vp, vp, sz, ctypes.c_int, sz, szp, szp, sz, szp]
+    L.cjx_chunk_lines_device.restype = ctypes.c_int
+    L.cjx_load_device.argtypes = [vp, ctypes.c_char_p, vp, sz, szp]
+    L.cjx_load_device.restype = ctypes.c_int
     L.cjx_server_create.argtypes = [ctypes.c_int, sz, ctypes.c_int, sz, ctypes.POINTER(vp)]
     L.cjx_server_create.restype = ctypes.c_int
     L.cjx_server_reserve.argtypes = [vp, sz]
@@ -265,7 +272,8 @@ class Context:
 
     def parse_batch_device(self, d_input: int, doc_off, doc_size, d_structural: int,
                            d_pair: int, cap: int, flags: int = 0) -> int:
-        """Enqueue a batch of device-resident documents (offsets 16-byte aligned)."""
+        """Enqueue a batch of device-resident documents (documents at unaligned
+        offsets are copied into an aligned layout first)."""
         offs = np.ascontiguousarray(doc_off, dtype=np.uint64)
         sizes = np.ascontiguousarray(doc_size, dtype=np.uint64)
         k = offs.size
@@ -279,8 +287,9 @@ class Context:
     def parse_lines_device(self, d_input: int, chunk_off, chunk_size, d_structural: int,
                            d_pair: int, cap: int, d_records: int = 0, max_records: int = 0,
                            flags: int = 0) -> int:
-        """Enqueue device-resident JSON Lines chunks (offsets 16-byte aligned);
-        with d_records the per-record table goes there (CJX_FLAG_RECORDS is set)."""
+        """Enqueue device-resident JSON Lines chunks (chunks at unaligned offsets
+        are copied into an aligned layout first); with d_records the per-record
+        table goes there (CJX_FLAG_RECORDS is set)."""
         offs = np.ascontiguousarray(chunk_off, dtype=np.uint64)
         sizes = np.ascontiguousarray(chunk_size, dtype=np.uint64)
         self._batch_n = offs.size
@@ -292,6 +301,39 @@ class Context:
                                             offs.size, flags, ctypes.c_void_p(d_structural),
                                             ctypes.c_void_p(d_pair), cap,
                                             ctypes.c_void_p(d_records), max_records)
+
+    def scan_lines_device(self, d_input: int, n: int, d_newlines: int, cap: int):
+        """(rc, count): offsets of every '\\n' of device text into d_newlines
+        (uint64[cap])."""
+        k = ctypes.c_size_t()
+        rc = lib().cjx_scan_lines_device(self._h, ctypes.c_void_p(d_input), n,
+                                         ctypes.c_void_p(d_newlines), cap, ctypes.byref(k))
+        return rc, k.value
+
+    def chunk_lines_device(self, d_input: int, n: int, mode: int, param: int):
+        """The chunks of chunk_lines(mode, param) for device text: (offsets, sizes)."""
+        k = ctypes.c_size_t()
+        rc = lib().cjx_chunk_lines_device(self._h, ctypes.c_void_p(d_input), n, mode, param, None,
+                                          None, 0, ctypes.byref(k))
+        if rc not in (CJX_OK, CJX_ERR_CAPACITY):
+            raise ValueError(strerror(rc))
+        cnt = k.value
+        offs = np.zeros(max(cnt, 1), np.uint64)
+        sizes = np.zeros(max(cnt, 1), np.uint64)
+        szp = ctypes.POINTER(ctypes.c_size_t)
+        rc = lib().cjx_chunk_lines_device(self._h, ctypes.c_void_p(d_input), n, mode, param,
+                                          offs.ctypes.data_as(szp), sizes.ctypes.data_as(szp), cnt,
+                                          ctypes.byref(k))
+        if rc != CJX_OK:
+            raise ValueError(strerror(rc))
+        return offs[:cnt].astype(np.int64), sizes[:cnt].astype(np.int64)
+
+    def load_device(self, path: str, d_buf: int, cap: int):
+        """(rc, size): the file at path read into device memory d_buf[0..cap)."""
+        k = ctypes.c_size_t()
+        rc = lib().cjx_load_device(self._h, os.fsencode(path), ctypes.c_void_p(d_buf), cap,
+                                   ctypes.byref(k))
+        return rc, k.value
 
     def batch_wait(self):
         """(rc, docs, N, M, overflow): docs is an int64 array [k, 4] of

@@ -8,6 +8,11 @@
 // kernels need is read on the device.
 #include <hip/hip_runtime.h>
 
+#include <errno.h>
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -194,15 +199,20 @@ int ensure_work(cjx_context *ctx, const Shape &sh) {
 
 // Uploads the unit descriptors and the tile -> unit map (one copy through a
 // pinned upload slot) into ctx->units.
+// The unit table, the tile -> unit map, the small units and, for units that
+// are relocated, their source offsets (unit_src, one per unit), in one copy.
 int upload_units(cjx_context *ctx, const std::vector<UnitDesc> &units, uint32_t ntiles,
-                 uint32_t n_small) {
-    const size_t nb = units.size() * sizeof(UnitDesc), tb = ((size_t)ntiles + n_small) * 4;
-    if (!ctx->units.ensure(nb + tb + 16)) return CJX_ERR_ALLOC;
+                 uint32_t n_small, const uint64_t *unit_src) {
+    const size_t nb = units.size() * sizeof(UnitDesc);
+    const size_t tb0 = ((size_t)ntiles + n_small) * 4;
+    const size_t sb = unit_src ? units.size() * 8 : 0;
+    const size_t tb = sb ? (tb0 + 7) & ~(size_t)7 : tb0;  // source offsets 8-byte aligned
+    if (!ctx->units.ensure(nb + tb + sb + 16)) return CJX_ERR_ALLOC;
     const int slot = (int)(ctx->up_count++ % cjx_context::kUploadSlots);
     if (!ctx->up_ev[slot] && hipEventCreate(&ctx->up_ev[slot]) != hipSuccess)
         return CJX_ERR_DEVICE;
     if (hipEventSynchronize(ctx->up_ev[slot]) != hipSuccess) return CJX_ERR_DEVICE;
-    if (!ctx->up[slot].ensure(nb + tb + 16)) return CJX_ERR_ALLOC;
+    if (!ctx->up[slot].ensure(nb + tb + sb + 16)) return CJX_ERR_ALLOC;
     uint8_t *h = static_cast<uint8_t *>(ctx->up[slot].p);
     std::memcpy(h, units.data(), nb);
     uint32_t *tu = reinterpret_cast<uint32_t *>(h + nb);
@@ -211,11 +221,14 @@ int upload_units(cjx_context *ctx, const std::vector<UnitDesc> &units, uint32_t 
         for (uint32_t t = 0; t < units[u].ntiles; t++) tu[units[u].tile0 + t] = (uint32_t)u;
         if (units[u].ntiles == 0) *su++ = (uint32_t)u;
     }
-    if (hipMemcpyAsync(ctx->units.p, h, nb + tb, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    if (sb) std::memcpy(h + nb + tb, unit_src, sb);
+    if (hipMemcpyAsync(ctx->units.p, h, nb + tb + sb, hipMemcpyHostToDevice, ctx->stream) !=
+        hipSuccess)
         return CJX_ERR_DEVICE;
     if (hipEventRecord(ctx->up_ev[slot], ctx->stream) != hipSuccess) return CJX_ERR_DEVICE;
     ctx->tile_unit = reinterpret_cast<uint32_t *>(ctx->units.as<uint8_t>() + nb);
     ctx->small_units = ctx->tile_unit + ntiles;
+    ctx->unit_src = sb ? reinterpret_cast<uint64_t *>(ctx->units.as<uint8_t>() + nb + tb) : nullptr;
     return CJX_OK;
 }
 
@@ -298,6 +311,20 @@ int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsign
     return enqueue_back(ctx, sh, flags, d_pair);
 }
 
+// The second stream for host-to-device copies and its events (created on
+// first use).
+int ensure_copy_stream(cjx_context *ctx) {
+    if (ctx->copy_stream) return CJX_OK;
+    if (hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->idle_ev, hipEventDisableTiming) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    for (hipEvent_t &e : ctx->copy_ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return CJX_ERR_DEVICE;
+    for (hipEvent_t &e : ctx->load_ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return CJX_ERR_DEVICE;
+    return CJX_OK;
+}
+
 // A single large unit whose bytes are still in host memory: the input is
 // copied in parts of `part` bytes (whole tiles) on the copy stream, and the
 // tokenizer runs as one launch per part, each waiting for its part's copy,
@@ -308,14 +335,7 @@ int enqueue_parse_chunked(cjx_context *ctx, const uint8_t *host, uint64_t n, con
                           unsigned flags, uint32_t *d_struct, int32_t *d_pair, uint64_t part) {
     const bool u16 = (flags & CJX_FLAG_DEPTH_U16) != 0;
     if (u16 && !ctx->tmp.ensure(sh.rows * 4 + 16)) return CJX_ERR_ALLOC;
-    if (!ctx->copy_stream) {
-        if (hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&ctx->idle_ev, hipEventDisableTiming) != hipSuccess)
-            return CJX_ERR_DEVICE;
-        for (hipEvent_t &e : ctx->copy_ev)
-            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
-                return CJX_ERR_DEVICE;
-    }
+    if (int rc = ensure_copy_stream(ctx)) return rc;
     const uint64_t parts = (n + part - 1) / part;
     if (parts > (uint64_t)cjx_context::kCopyEvents || part % kTileBytes) return CJX_ERR_INPUT;
     LaunchBufs b = bufs_of(ctx);
@@ -731,12 +751,14 @@ void cjx_context_destroy(cjx_context *ctx) {
                       &ctx->ck,      &ctx->gmin,       &ctx->gmin1,  &ctx->gmin2,
                       &ctx->gagg,    &ctx->gd0,        &ctx->lows,   &ctx->highs,
                       &ctx->chunk_unit, &ctx->big_units, &ctx->unit_part, &ctx->rec_count,
-                      &ctx->recs,       &ctx->row_nl};
+                      &ctx->recs,       &ctx->row_nl,   &ctx->nl_seg, &ctx->nl_off,
+                      &ctx->nl_idx};
     for (DevBuf *b : bufs) b->release();
     ctx->h_units.release();
     ctx->h_uout.release();
     ctx->h_ls.release();
     ctx->h_stage.release();
+    ctx->h_nl.release();
     for (int i = 0; i < cjx_context::kUploadSlots; i++) {
         if (ctx->up_ev[i]) {
             (void)hipEventSynchronize(ctx->up_ev[i]);
@@ -753,6 +775,10 @@ void cjx_context_destroy(cjx_context *ctx) {
     }
     for (hipEvent_t e : ctx->copy_ev)
         if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < cjx_context::kLoadBufs; i++) {
+        if (ctx->load_ev[i]) (void)hipEventDestroy(ctx->load_ev[i]);
+        ctx->load_buf[i].release();
+    }
     if (ctx->idle_ev) (void)hipEventDestroy(ctx->idle_ev);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -827,52 +853,250 @@ void cjx_free_input(cjx_input *in) {
     in->size = 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+using ChunkList = std::vector<std::pair<uint64_t, uint64_t>>;
+
+// Chunk boundaries under the count rule (load_file.cu:83-103), as line
+// indices li[0..count]: chunk i takes lines / count lines, one more while
+// i < lines % count.
+void count_rule(uint64_t lines, uint64_t count, std::vector<uint64_t> &li) {
+    li.assign(count + 1, 0);
+    const uint64_t per = lines / count, extra = lines % count;
+    for (uint64_t i = 0; i < count; i++) li[i + 1] = li[i] + per + (i < extra ? 1 : 0);
+}
+
+// Greedy split under the size rule (load_file.cu:153-176) over the line
+// starts at(0..len) (at(0) = 0, ascending, the last one n): a chunk grows
+// line by line; the first line start more than `param` bytes past the chunk
+// start closes the chunk at the line start before it, which starts the next
+// chunk, and the line start that closed it is not tested again.  A chunk may
+// be empty (a line longer than param).  Binary searches give the same cuts
+// as the reference's line-by-line walk.
+template <class LineStart>
+void bytes_rule(LineStart at, size_t len, uint64_t n, uint64_t param, ChunkList &out) {
+    uint64_t cstart = 0;
+    size_t i = 1;
+    while (i < len) {
+        const uint64_t lim = param > UINT64_MAX - cstart ? UINT64_MAX : cstart + param;
+        size_t lo = i, hi = len;  // first j in [i, len) with at(j) > lim, else len
+        while (lo < hi) {
+            const size_t mid = lo + (hi - lo) / 2;
+            if (at(mid) > lim) hi = mid; else lo = mid + 1;
+        }
+        const size_t j = lo;
+        if (j >= len) break;
+        out.emplace_back(cstart, at(j - 1) - cstart);
+        cstart = at(j - 1);
+        i = j + 1;
+    }
+    if (cstart < n) out.emplace_back(cstart, n - cstart);
+}
+
+// Mode 0 with 1 .. 2^32 chunks, or mode 1.
+bool chunk_rule_ok(int mode, size_t param) {
+    return (mode == 0 && param != 0 && param <= (size_t(1) << 32)) || mode == 1;
+}
+
+int emit_chunks(const ChunkList &out, size_t *chunk_off, size_t *chunk_size, size_t cap,
+                size_t *n_chunks) {
+    *n_chunks = out.size();
+    for (size_t i = 0; i < out.size() && i < cap; i++) {
+        if (chunk_off) chunk_off[i] = (size_t)out[i].first;
+        if (chunk_size) chunk_size[i] = (size_t)out[i].second;
+    }
+    return out.size() <= cap ? CJX_OK : CJX_ERR_CAPACITY;
+}
+
+}  // namespace
+
+extern "C" {
+
 int cjx_chunk_lines(const uint8_t *data, size_t n, int mode, size_t param, size_t *chunk_off,
                     size_t *chunk_size, size_t cap, size_t *n_chunks) {
     if (!n_chunks) return CJX_ERR_INPUT;
     *n_chunks = 0;
     if (!data && n) return CJX_ERR_INPUT;
+    if (!chunk_rule_ok(mode, param)) return CJX_ERR_INPUT;
     // Line starts (load_file.cu:72-81): 0, every byte after '\n', and n if the
     // last line is unterminated.
-    std::vector<size_t> offs;
+    std::vector<uint64_t> offs;
     offs.push_back(0);
     const uint8_t *p = data, *end = data + n;
     while (p < end) {
         const void *hit = std::memchr(p, '\n', (size_t)(end - p));
         if (!hit) break;
         p = static_cast<const uint8_t *>(hit) + 1;
-        offs.push_back((size_t)(p - data));
+        offs.push_back((uint64_t)(p - data));
     }
     if (offs.back() < n) offs.push_back(n);
-    std::vector<std::pair<size_t, size_t>> out;
-    if (mode == 0) {  // chunk count, load_file.cu:83-103
-        if (param == 0) return CJX_ERR_INPUT;
-        const size_t total = offs.size() - 1, per = total / param, extra = total % param;
-        size_t li = 0;
-        for (size_t i = 0; i < param; i++) {
-            const size_t k = per + (i < extra ? 1 : 0);
-            out.emplace_back(offs[li], offs[li + k] - offs[li]);
-            li += k;
-        }
-    } else if (mode == 1) {  // max bytes per chunk, load_file.cu:153-176
-        size_t cstart = 0, cur = 0;
-        for (size_t i = 1; i < offs.size(); i++) {
-            if (offs[i] - cstart > param) {
-                out.emplace_back(cstart, cur - cstart);
-                cstart = offs[i - 1];
-            }
-            cur = offs[i];
-        }
-        if (cstart < n) out.emplace_back(cstart, n - cstart);
+    ChunkList out;
+    if (mode == 0) {
+        std::vector<uint64_t> li;
+        count_rule(offs.size() - 1, param, li);
+        for (size_t i = 0; i < param; i++) out.emplace_back(offs[li[i]], offs[li[i + 1]] - offs[li[i]]);
     } else {
+        bytes_rule([&](size_t j) { return offs[j]; }, offs.size(), n, param, out);
+    }
+    return emit_chunks(out, chunk_off, chunk_size, cap, n_chunks);
+}
+
+namespace {
+
+// Newline count and last byte of device text; the segment counts and bases
+// stay in ctx->nl_seg for launch_nl_write.  Synchronous.
+int nl_scan_total(cjx_context *ctx, const uint8_t *d_in, uint64_t n, uint64_t **seg_base,
+                  uint64_t *total, uint8_t *last) {
+    const uint64_t g = cjx::nl_segments(n);
+    const size_t cnt_bytes = (size_t)((g * 4 + 15) & ~15ull);
+    if (!ctx->nl_seg.ensure(cnt_bytes + (size_t)(g + 1) * 8 + 16) || !ctx->h_nl.ensure(64))
+        return CJX_ERR_ALLOC;
+    uint8_t *base = ctx->nl_seg.as<uint8_t>();
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(base);
+    *seg_base = reinterpret_cast<uint64_t *>(base + cnt_bytes);
+    uint64_t *stat = *seg_base + g + 1;
+    uint64_t *h = static_cast<uint64_t *>(ctx->h_nl.p);
+    if (cjx::launch_nl_count(ctx->stream, d_in, n, cnt, *seg_base, stat) != hipSuccess ||
+        hipMemcpyAsync(h, stat, 16, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    *total = h[0];
+    *last = (uint8_t)h[1];
+    return CJX_OK;
+}
+
+bool device_text_ok(const uint8_t *d_in, size_t n) {
+    return (d_in || !n) && (reinterpret_cast<uintptr_t>(d_in) & 15u) == 0;
+}
+
+}  // namespace
+
+int cjx_scan_lines_device(cjx_context *ctx, const uint8_t *d_input, size_t n,
+                          uint64_t *d_newlines, size_t cap, size_t *n_newlines) {
+    if (!n_newlines) return CJX_ERR_INPUT;
+    *n_newlines = 0;
+    if (!ctx || !device_text_ok(d_input, n)) return CJX_ERR_INPUT;
+    if (int rc = set_device(ctx)) return rc;
+    uint64_t *seg_base = nullptr, total = 0;
+    uint8_t last = 0;
+    if (int rc = nl_scan_total(ctx, d_input, n, &seg_base, &total, &last)) return rc;
+    *n_newlines = (size_t)total;
+    if (total > cap) return CJX_ERR_CAPACITY;
+    if (total && !d_newlines) return CJX_ERR_INPUT;
+    if (total && (cjx::launch_nl_write(ctx->stream, d_input, n, seg_base, d_newlines, cap) !=
+                      hipSuccess ||
+                  hipStreamSynchronize(ctx->stream) != hipSuccess))
+        return CJX_ERR_DEVICE;
+    return CJX_OK;
+}
+
+int cjx_chunk_lines_device(cjx_context *ctx, const uint8_t *d_input, size_t n, int mode,
+                           size_t param, size_t *chunk_off, size_t *chunk_size, size_t cap,
+                           size_t *n_chunks) {
+    if (!n_chunks) return CJX_ERR_INPUT;
+    *n_chunks = 0;
+    if (!ctx || !device_text_ok(d_input, n)) return CJX_ERR_INPUT;
+    if (!chunk_rule_ok(mode, param)) return CJX_ERR_INPUT;
+    if (int rc = set_device(ctx)) return rc;
+    uint64_t *seg_base = nullptr, total = 0;
+    uint8_t last = 0;
+    if (int rc = nl_scan_total(ctx, d_input, n, &seg_base, &total, &last)) return rc;
+    // line starts: 0, the byte after each '\n', and n after an unterminated last line
+    const uint64_t lines = total + (n && last != '\n' ? 1 : 0);
+    hipStream_t s = ctx->stream;
+    if (!ctx->nl_off.ensure((size_t)(total ? total : 1) * 8)) return CJX_ERR_ALLOC;
+    uint64_t *d_nl = ctx->nl_off.as<uint64_t>();
+    if (total && cjx::launch_nl_write(s, d_input, n, seg_base, d_nl, total) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    ChunkList out;
+    if (mode == 0) {
+        // the count rule needs param + 1 line starts: gathered on the device
+        std::vector<uint64_t> li;
+        count_rule(lines, param, li);
+        const size_t k = li.size(), bytes = k * 8;
+        if (!ctx->nl_idx.ensure(2 * bytes) || !ctx->h_nl.ensure(bytes)) return CJX_ERR_ALLOC;
+        uint64_t *d_idx = ctx->nl_idx.as<uint64_t>(), *d_at = d_idx + k;
+        uint64_t *h = static_cast<uint64_t *>(ctx->h_nl.p);
+        std::memcpy(h, li.data(), bytes);
+        if (hipMemcpyAsync(d_idx, h, bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+            cjx::launch_nl_gather(s, d_nl, total, n, d_idx, k, d_at) != hipSuccess ||
+            hipMemcpyAsync(h, d_at, bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return CJX_ERR_DEVICE;
+        for (size_t i = 0; i < param; i++) out.emplace_back(h[i], h[i + 1] - h[i]);
+    } else {
+        // the size rule walks the line starts greedily: on the host, from the
+        // device's newline offsets (line j >= 1 starts after newline j - 1)
+        if (!ctx->h_nl.ensure((size_t)(total ? total : 1) * 8)) return CJX_ERR_ALLOC;
+        const uint64_t *h = static_cast<const uint64_t *>(ctx->h_nl.p);
+        if ((total && hipMemcpyAsync(ctx->h_nl.p, d_nl, total * 8, hipMemcpyDeviceToHost, s) !=
+                          hipSuccess) ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return CJX_ERR_DEVICE;
+        auto at = [&](size_t j) -> uint64_t { return j == 0 ? 0 : (j <= total ? h[j - 1] + 1 : n); };
+        bytes_rule(at, (size_t)lines + 1, n, param, out);
+    }
+    return emit_chunks(out, chunk_off, chunk_size, cap, n_chunks);
+}
+
+int cjx_load_device(cjx_context *ctx, const char *path, uint8_t *d_buf, size_t cap, size_t *n) {
+    if (!n) return CJX_ERR_INPUT;
+    *n = 0;
+    if (!ctx || !path) return CJX_ERR_INPUT;
+    const int fd = ::open(path, O_RDONLY);
+    if (fd < 0) return CJX_ERR_INPUT;  // load_file.cu:10-13
+    struct stat st;
+    if (::fstat(fd, &st) != 0 || st.st_size < 0) {
+        ::close(fd);
         return CJX_ERR_INPUT;
     }
-    *n_chunks = out.size();
-    for (size_t i = 0; i < out.size() && i < cap; i++) {
-        if (chunk_off) chunk_off[i] = out[i].first;
-        if (chunk_size) chunk_size[i] = out[i].second;
+    const uint64_t size = (uint64_t)st.st_size;
+    *n = (size_t)size;
+    int rc = CJX_OK;
+    if (size > cap) rc = CJX_ERR_CAPACITY;
+    else if (size && !d_buf) rc = CJX_ERR_INPUT;
+    else rc = set_device(ctx);
+    if (!rc) rc = ensure_copy_stream(ctx);
+    for (int k = 0; !rc && k < cjx_context::kLoadBufs && (uint64_t)k * cjx_context::kLoadPart < size; k++)
+        if (!ctx->load_buf[k].ensure(cjx_context::kLoadPart)) rc = CJX_ERR_ALLOC;
+    // the copies start once earlier work on the context stream (which may
+    // still read the buffer) is done
+    if (!rc && (hipEventRecord(ctx->idle_ev, ctx->stream) != hipSuccess ||
+                hipStreamWaitEvent(ctx->copy_stream, ctx->idle_ev, 0) != hipSuccess))
+        rc = CJX_ERR_DEVICE;
+    // Part i is read into staging buffer i % kLoadBufs once that buffer's
+    // previous copy has finished, so the file reads overlap the copies.
+    uint64_t part = 0;
+    for (uint64_t off = 0; !rc && off < size; off += cjx_context::kLoadPart, part++) {
+        const int k = (int)(part % cjx_context::kLoadBufs);
+        const uint64_t len = std::min<uint64_t>(cjx_context::kLoadPart, size - off);
+        if (part >= (uint64_t)cjx_context::kLoadBufs &&
+            hipEventSynchronize(ctx->load_ev[k]) != hipSuccess) {
+            rc = CJX_ERR_DEVICE;
+            break;
+        }
+        uint8_t *buf = static_cast<uint8_t *>(ctx->load_buf[k].p);
+        for (uint64_t got = 0; got < len;) {
+            const ssize_t r = ::pread(fd, buf + got, (size_t)(len - got), (off_t)(off + got));
+            if (r < 0 && errno == EINTR) continue;
+            if (r <= 0) {
+                rc = CJX_ERR_INPUT;  // a short file: read error (load_file.cu:29-34)
+                break;
+            }
+            got += (uint64_t)r;
+        }
+        if (!rc && (hipMemcpyAsync(d_buf + off, buf, len, hipMemcpyHostToDevice,
+                                   ctx->copy_stream) != hipSuccess ||
+                    hipEventRecord(ctx->load_ev[k], ctx->copy_stream) != hipSuccess))
+            rc = CJX_ERR_DEVICE;
     }
-    return out.size() <= cap ? CJX_OK : CJX_ERR_CAPACITY;
+    ::close(fd);
+    // every queued copy finishes before the staging buffers can be reused
+    if (ctx->copy_stream && hipStreamSynchronize(ctx->copy_stream) != hipSuccess && !rc)
+        rc = CJX_ERR_DEVICE;
+    return rc;
 }
 
 int cjx_parse_standard(cjx_context *ctx, const uint8_t *data, size_t n, unsigned flags,
@@ -926,7 +1150,7 @@ void cjx_free_result(cjx_result *r) {
 int cjx_parse_device(cjx_context *ctx, const uint8_t *d_input, size_t n, unsigned flags,
                      int32_t *d_structural, int32_t *d_pair, size_t cap) {
     if (!ctx || !d_input || n == 0 || !d_structural || !d_pair) return CJX_ERR_INPUT;
-    if (n > kMaxLaunchBytes) return CJX_ERR_INPUT;
+    if (n > kMaxLaunchBytes || (reinterpret_cast<uintptr_t>(d_input) & 15u)) return CJX_ERR_INPUT;
     int rc = set_device(ctx);
     if (rc) return rc;
     std::vector<UnitDesc> units;
@@ -1010,10 +1234,13 @@ int cjx_parse_lines_device(cjx_context *ctx, const uint8_t *d_input, const size_
 
 namespace cjx_host {
 
-// Units at 16-byte aligned offsets of a device buffer.  Independent
-// documents (no JSONL flag): positions and pair values local to each unit.
-// JSON Lines chunks: positions count from the first chunk (lastChunkIndex)
-// and pair values are numbered across chunks, as in the host layout.
+// Units at offsets of a device buffer.  Independent documents (no JSONL
+// flag): positions and pair values local to each unit.  JSON Lines chunks:
+// positions count from the first chunk (lastChunkIndex) and pair values are
+// numbered across chunks, as in the host layout.  When a unit does not start
+// at a 16-byte aligned offset, all units are first copied into a packed,
+// aligned layout in the context's input buffer (k_place_units), which the
+// tokenizer then reads.
 int parse_units_device(cjx_context *ctx, const uint8_t *d_input, const size_t *doc_off,
                        const size_t *doc_size, size_t n_docs, unsigned flags,
                        int32_t *d_structural, int32_t *d_pair, size_t cap,
@@ -1024,11 +1251,22 @@ int parse_units_device(cjx_context *ctx, const uint8_t *d_input, const size_t *d
     int rc = set_device(ctx);
     if (rc) return rc;
     std::vector<UnitDesc> units(n_docs);
-    uint64_t total = 0, full = 0;
+    uint64_t total = 0, full = 0, placed = 0, src_end = 0;
+    // Addresses, not offsets, decide: relocation reads from d_input rounded
+    // down to 16 bytes (offsets shifted by the difference).
+    const uint64_t shift = reinterpret_cast<uintptr_t>(d_input) & 15u;
+    bool relocate = false;
     for (size_t i = 0; i < n_docs; i++) {
-        if (doc_size[i] == 0 || (doc_off[i] & 15)) return CJX_ERR_INPUT;
+        if (doc_size[i] == 0) return CJX_ERR_INPUT;
+        if ((doc_off[i] + shift) & 15) relocate = true;
+    }
+    std::vector<uint64_t> src;
+    for (size_t i = 0; i < n_docs; i++) {
         UnitDesc &u = units[i];
-        u.in_off = doc_off[i];
+        u.in_off = relocate ? placed : doc_off[i];
+        placed += (doc_size[i] + 15) & ~15ull;
+        if (relocate) src.push_back(doc_off[i] + shift);
+        src_end = std::max<uint64_t>(src_end, (doc_off[i] + shift + doc_size[i] + 15) & ~15ull);
         u.n = doc_size[i];
         u.pos_base = jsonl ? total : 0;
         total += doc_size[i];
@@ -1044,15 +1282,25 @@ int parse_units_device(cjx_context *ctx, const uint8_t *d_input, const size_t *d
     Shape sh{(uint32_t)tiles, (uint32_t)n_docs, n_small, 0, 0, 0, 0, jsonl};
     shape_rows(sh, units);
     if ((rc = ensure_work(ctx, sh))) return rc;
-    if ((rc = upload_units(ctx, units, (uint32_t)tiles, n_small))) return rc;
+    if (relocate && !ctx->in.ensure(placed + 16)) return CJX_ERR_ALLOC;
+    if ((rc = upload_units(ctx, units, (uint32_t)tiles, n_small, relocate ? src.data() : nullptr)))
+        return rc;
     ctx->n_units = (uint32_t)n_docs;
     ctx->batch_off.assign(doc_off, doc_off + n_docs);
     (void)hipEventRecord(ctx->ev0, ctx->stream);
+    const uint8_t *d_text = d_input;
+    if (relocate) {
+        if (launch_place_units(ctx->stream, d_input - shift, src_end, ctx->unit_src,
+                               ctx->units.as<UnitDesc>(), (uint32_t)n_docs,
+                               ctx->in.as<uint8_t>(), placed) != hipSuccess)
+            return CJX_ERR_DEVICE;
+        d_text = ctx->in.as<uint8_t>();
+    }
     const bool records = jsonl && (flags & CJX_FLAG_RECORDS) && d_records;
     if (records && (!ctx->row_nl.ensure(sh.rows + 16) || !ctx->rec_count.ensure(n_docs * 4 + 16)))
         return CJX_ERR_ALLOC;
     ctx->want_nl = records;  // the tokenizer also writes the newline flags
-    rc = enqueue_maybe_graph(ctx, d_input, sh, flags & ~CJX_FLAG_RECORDS,
+    rc = enqueue_maybe_graph(ctx, d_text, sh, flags & ~CJX_FLAG_RECORDS,
                              reinterpret_cast<uint32_t *>(d_structural), d_pair);
     if (!rc && records) {
         const LaunchBufs b = bufs_of(ctx);

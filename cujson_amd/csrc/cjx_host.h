@@ -81,6 +81,7 @@ struct cjx_context {
     DevBuf units, uout, ls, tile_st, layout, chunk_unit, big_units, unit_part, rec_count, recs;
     uint32_t *tile_unit = nullptr;    // inside `units`, after the unit descriptors
     uint32_t *small_units = nullptr;  // after the tile -> unit map
+    uint64_t *unit_src = nullptr;     // after those: source offsets of relocated units
     // Pinned upload slots for unit tables: a call may return before its copy
     // has run, so the next calls use other slots; a slot is reused only after
     // the event recorded behind its copy has completed.
@@ -99,6 +100,16 @@ struct cjx_context {
     hipStream_t copy_stream = nullptr;
     hipEvent_t copy_ev[kCopyEvents] = {};
     hipEvent_t idle_ev = nullptr;
+    // Newline scan of device text: segment counts, bases and status; the
+    // newline offsets and gathered line starts of the device chunker.
+    DevBuf nl_seg, nl_off, nl_idx;
+    PinnedBuf h_nl;
+    // File loads into device memory: pinned staging buffers, refilled from the
+    // file while earlier ones are copied (created on first use).
+    static constexpr int kLoadBufs = 4;
+    static constexpr size_t kLoadPart = 32ull << 20;
+    PinnedBuf load_buf[kLoadBufs];
+    hipEvent_t load_ev[kLoadBufs] = {};
     std::vector<uint64_t> batch_off;  // byte offsets of the documents of the batch in flight
     // Shape of the launch in flight (device form).
     uint32_t n_units = 0;
@@ -139,11 +150,12 @@ uint64_t set_regions(std::vector<UnitDesc> &units, bool full);
 void shape_rows(Shape &sh, const std::vector<UnitDesc> &units);
 int ensure_work(cjx_context *ctx, const Shape &sh);
 int upload_units(cjx_context *ctx, const std::vector<UnitDesc> &units, uint32_t ntiles,
-                 uint32_t n_small);
+                 uint32_t n_small, const uint64_t *unit_src = nullptr);
 LaunchBufs bufs_of(cjx_context *ctx);
 int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsigned flags,
                   uint32_t *d_struct, int32_t *d_pair);
 int read_status(cjx_context *ctx, uint32_t n_units);
+int ensure_copy_stream(cjx_context *ctx);
 int unit_code(const UnitOut &o);
 
 }  // namespace cjx_host

@@ -179,6 +179,19 @@ hipError_t launch_sort_units(hipStream_t s, const LaunchBufs &b, uint32_t n_unit
 // Per-record table of JSON Lines units: write = false counts the records of
 // each unit and numbers them (rec_count becomes the bases, ls->n_records the
 // total); write = true fills rec.
+// Newline scan of device text (cjx_lines.hip): segment counts, bases and
+// status {newline count, last byte}; the newline offsets; line starts at
+// given line indices.
+uint32_t nl_segments(uint64_t n);
+hipError_t launch_nl_count(hipStream_t s, const uint8_t *d_in, uint64_t n, uint32_t *seg_count,
+                           uint64_t *seg_base, uint64_t *stat);
+hipError_t launch_nl_write(hipStream_t s, const uint8_t *d_in, uint64_t n,
+                           const uint64_t *seg_base, uint64_t *out, uint64_t cap);
+hipError_t launch_place_units(hipStream_t s, const uint8_t *src, uint64_t src_limit,
+                              const uint64_t *src_off, const UnitDesc *units, uint32_t n_units,
+                              uint8_t *dst, uint64_t dst_bytes);
+hipError_t launch_nl_gather(hipStream_t s, const uint64_t *nl, uint64_t n_nl, uint64_t n,
+                            const uint64_t *line_idx, uint64_t count, uint64_t *out);
 hipError_t launch_records(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
                           const uint32_t *d_struct, uint32_t *rec_count, RecordOut *rec,
                           uint64_t max_records, bool write);

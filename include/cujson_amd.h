@@ -108,7 +108,8 @@ void cjx_free_input(cjx_input *in);
 /* Replaces the line scan of loadJSONLines_chunkCount / _chunkSizeBytes /
  * _chunkSizeMegaBytes, cujson/load_file.h:10-12 / load_file.cu:71-103,
  * :137-176: split a JSON Lines buffer into line-aligned chunks.
- *   mode 0: chunk_param = chunk count; mode 1: chunk_param = max bytes per chunk.
+ *   mode 0: chunk_param = chunk count (1 .. 2^32); mode 1: chunk_param = max
+ *   bytes per chunk.
  * Writes up to cap (offset, size) pairs; *n_chunks receives the count. */
 int cjx_chunk_lines(const uint8_t *data, size_t n, int mode, size_t chunk_param,
                     size_t *chunk_off, size_t *chunk_size, size_t cap, size_t *n_chunks);
@@ -224,7 +225,10 @@ int cjx_parse_batch(cjx_context *ctx, const uint8_t *data, const size_t *doc_off
 void cjx_free_batch_result(cjx_batch_result *r);
 
 /* Device-resident batch (no counterpart in the reference): documents at
- * 16-byte aligned offsets of d_input; rows stay in d_structural / d_pair
+ * offsets of d_input (16-byte aligned: read in place; otherwise all
+ * documents are first copied into an aligned layout in the context's input
+ * buffer; d_input is readable up to each document's end rounded up to 16);
+ * rows stay in d_structural / d_pair
  * (cap rows each), document i in its own region.  Returns after enqueueing;
  * cjx_batch_wait synchronizes and fills docs[0..n_docs) (struct_begin /
  * struct_end index the caller's row buffers).  If the documents' bytes sum
@@ -238,8 +242,10 @@ int cjx_batch_wait(cjx_context *ctx, cjx_doc_result *docs, size_t n_docs,
 
 /* Device-resident JSON Lines (no counterpart in the reference, whose
  * parse_json_lines copies each chunk in, parse_json_lines.cu:1160-1230):
- * line-aligned chunks placed at 16-byte aligned offsets of d_input, parsed
- * in one launch sequence with the reference's carries (positions count from
+ * line-aligned chunks at offsets of d_input (as for cjx_parse_batch_device:
+ * aligned chunks are read in place, others copied into an aligned layout
+ * first, so the chunks of cjx_chunk_lines_device can be passed as they are),
+ * parsed in one launch sequence with the reference's carries (positions count from
  * the first chunk as if the chunks were contiguous, pair values are numbered
  * across chunks).  Rows stay in per-chunk regions (cjx_batch_wait gives each
  * chunk's range; row_base ordering follows chunk order).  With
@@ -249,6 +255,33 @@ int cjx_parse_lines_device(cjx_context *ctx, const uint8_t *d_input, const size_
                            const size_t *chunk_size, size_t n_chunks, unsigned flags,
                            int32_t *d_structural, int32_t *d_pair, size_t cap,
                            cjx_record *d_records, size_t max_records);
+
+/* Replaces the host line scan of loadJSONLines_* (cujson/load_file.cu:72-81)
+ * for text already in device memory: the byte offset of every '\n' of
+ * d_input[0..n) (16-byte aligned), in order, i.e. the record ends, written
+ * to d_newlines[0..cap).  *n_newlines receives the count; a count above cap
+ * returns CJX_ERR_CAPACITY (nothing written).  Synchronous. */
+int cjx_scan_lines_device(cjx_context *ctx, const uint8_t *d_input, size_t n,
+                          uint64_t *d_newlines, size_t cap, size_t *n_newlines);
+
+/* Replaces the chunking of loadJSONLines_chunkCount / _chunkSizeBytes
+ * (cujson/load_file.cu:83-103, :153-176) for text in device memory: the
+ * chunks cjx_chunk_lines gives for the same bytes (same modes, same results,
+ * host arrays), found from the device newline scan.  Mode 0 gathers the
+ * chunk starts on the device; mode 1 (greedy by size) walks the line starts
+ * on the host after one copy of the newline offsets.  Synchronous. */
+int cjx_chunk_lines_device(cjx_context *ctx, const uint8_t *d_input, size_t n, int mode,
+                           size_t chunk_param, size_t *chunk_off, size_t *chunk_size, size_t cap,
+                           size_t *n_chunks);
+
+/* Device counterpart of loadJSON / loadJSONLines' file read
+ * (cujson/load_file.cu:7-36, :38-70): reads the file at path into
+ * d_buf[0..cap) through pinned staging buffers, reading each part from the
+ * file while earlier parts are being copied.  *n receives the file size; a
+ * file larger than cap returns CJX_ERR_CAPACITY (nothing read).  Work queued
+ * earlier on the context stream finishes before the buffer is written.
+ * Synchronous. */
+int cjx_load_device(cjx_context *ctx, const char *path, uint8_t *d_buf, size_t cap, size_t *n);
 
 /* ------------------------------------------------------------------------
  * Request server (no counterpart in the reference, whose parse calls are

@@ -61,6 +61,44 @@ def test_entry_points_reject_bad_arguments_without_device():
                                     ctypes.byref(r)) == cj.CJX_ERR_INPUT
 
 
+def test_loader_entries_reject_bad_arguments_without_device():
+    L = cj.lib()
+    n = ctypes.c_size_t(7)
+    assert L.cjx_scan_lines_device(None, None, 0, None, 0, ctypes.byref(n)) == cj.CJX_ERR_INPUT
+    assert n.value == 0
+    n = ctypes.c_size_t(7)
+    assert L.cjx_chunk_lines_device(None, None, 0, 0, 1, None, None, 0,
+                                    ctypes.byref(n)) == cj.CJX_ERR_INPUT
+    assert n.value == 0
+    assert L.cjx_load_device(None, b"/nonexistent", None, 0, ctypes.byref(n)) == cj.CJX_ERR_INPUT
+    # chunk rules: mode 0 needs 1 .. 2^32 chunks, modes other than 0 and 1 are rejected
+    text = (ctypes.c_ubyte * 2)(ord("a"), 10)
+    for mode, p in ((0, 0), (0, (1 << 32) + 1), (2, 5), (-1, 5)):
+        assert L.cjx_chunk_lines(text, 2, mode, p, None, None, 0,
+                                 ctypes.byref(n)) == cj.CJX_ERR_INPUT, (mode, p)
+
+
+def test_chunk_rules_against_oracle_edge_cases():
+    """The host chunker (whose rule functions the device chunker shares)
+    against the oracle's restatement of load_file.cu:83-103 / :153-176: lines
+    longer than the size limit (empty chunks), blank lines, unterminated last
+    lines, more chunks than lines."""
+    rng = np.random.default_rng(3)
+    cases = [b"", b"\n", b"\n\n\n", b"abc", b"abc\n", b"a\nbb\nccc\ndddd\n",
+             b"x" * 50 + b"\n" + b"y" * 3 + b"\n" + b"z" * 70]
+    for _ in range(40):
+        lines = [b"q" * int(k) for k in rng.integers(0, 40, rng.integers(1, 30))]
+        d = b"\n".join(lines) + (b"\n" if rng.random() < 0.5 else b"")
+        cases.append(d)
+    for d in cases:
+        for cc in (1, 2, 3, 5, 40):
+            offs, sizes = cj.chunk_lines(d, 0, cc)
+            assert list(zip(offs, sizes)) == oracle.chunks_by_count(d, cc), (d, cc)
+        for b in (1, 2, 5, 10, 33, 64, 1000):
+            offs, sizes = cj.chunk_lines(d, 1, b)
+            assert list(zip(offs, sizes)) == oracle.chunks_by_bytes(d, b), (d, b)
+
+
 def test_server_rejects_bad_arguments_without_device():
     h = ctypes.c_void_p()
     L = cj.lib()
