@@ -57,6 +57,41 @@ def calibrate_copy(nbytes: int, reps: int):
           flush=True)
 
 
+def values_bench(cj, ctx, d_in, n: int, d_st, N: int, reps: int = 3) -> dict:
+    """The typed value table (cjx_values_device) of the plain document, with
+    the unescaped strings: one decode per gap between structurals, timed by
+    events on the context stream around the synchronous call."""
+    from cujson_amd.hip import DeviceBuffer, Event
+    vals = DeviceBuffer((N + 1) * cj.VALUE_DTYPE.itemsize)
+    strs = DeviceBuffer(n)
+    e0, e1 = Event(), Event()
+    stream = ctx.stream
+    times, n_exact = [], 0
+    for _ in range(reps + 1):
+        e0.record(stream)
+        rc, n_exact = ctx.values_device(d_in.ptr, n, d_st.ptr, N, vals.ptr, strs.ptr)
+        e1.record(stream)
+        e1.synchronize()
+        if rc:
+            raise RuntimeError(f"values_device: {cj.strerror(rc)}")
+        times.append(e0.elapsed_ms(e1))
+    ms = float(np.median(times[1:]))
+    alg = n + 4 * N + cj.VALUE_DTYPE.itemsize * (N + 1)  # text, rows, table (strings <= n)
+    vals.free()
+    strs.free()
+    return {
+        "ms": round(ms, 3),
+        "input_gbs": round(n / (ms * 1e-3) / 1e9, 1),
+        "entries": N + 1,
+        "entries_per_s": round((N + 1) / (ms * 1e-3), 0),
+        "exact_step_numbers": int(n_exact),
+        "algorithmic_bytes": int(alg),
+        "hbm_frac": round(alg / (ms * 1e-3) / 8e12, 3),
+        "form": "one thread per gap between structurals; strings unescaped into an n-byte "
+                "buffer; numbers correctly rounded (exact step for undecided ones)",
+    }
+
+
 def _num(x: float, nd: int):
     """Rounded figure, or None (JSON null) for one not measured (NaN)."""
     return None if x != x else round(x, nd)
@@ -453,6 +488,8 @@ def main():
     ap.add_argument("--no-lines", action="store_true",
                     help="skip the JSON Lines batched-vs-plain measurement")
     ap.add_argument("--records", type=int, default=8_000_000)
+    ap.add_argument("--no-values", action="store_true",
+                    help="skip the typed value table measurement")
     ap.add_argument("--no-plain-compare", action="store_true",
                     help="skip the plain per-document / per-chunk loops (profiling runs)")
     ap.add_argument("--no-host-path", action="store_true",
@@ -612,6 +649,9 @@ def main():
     }
     if not args.no_host_path:
         out["host_path"] = host_path_bench(cj, doc, ctx, reps=3)
+    if not args.no_values:
+        log("typed value table of the same document")
+        out["values"] = values_bench(cj, ctx, d_in, n, d_st, int(N))
     ctx.close()
     del d_in, d_st, d_pr
     if not args.no_batched:

@@ -60,6 +60,7 @@ ABI_SYMBOLS = (
     "cjx_parse_lines_device", "cjx_server_create", "cjx_server_submit", "cjx_server_wait",
     "cjx_server_release", "cjx_server_reserve", "cjx_server_get_stats", "cjx_server_destroy",
     "cjx_scan_lines_device", "cjx_chunk_lines_device", "cjx_load_device",
+    "cjx_values_device", "cjx_values",
 )
 
 
@@ -84,6 +85,14 @@ class _Result(ctypes.Structure):
 
 
 # cjx_record: one JSON Lines record (CJX_FLAG_RECORDS)
+# Typed value table (cjx_value): types and flags.
+CJX_VALUE_NONE, CJX_VALUE_KEY, CJX_VALUE_STRING, CJX_VALUE_INT, CJX_VALUE_FLOAT = 0, 1, 2, 3, 4
+CJX_VALUE_TRUE, CJX_VALUE_FALSE, CJX_VALUE_NULL, CJX_VALUE_INVALID = 5, 6, 7, 8
+CJX_VALUE_ESCAPED, CJX_VALUE_BIG_INT, CJX_VALUE_UNPAIRED, CJX_VALUE_APPROX = 0x1, 0x2, 0x4, 0x8
+VALUE_DTYPE = np.dtype([("v", np.int64), ("row", np.uint32), ("text_off", np.uint32),
+                        ("text_len", np.uint32), ("type", np.uint8), ("flags", np.uint8),
+                        ("reserved", np.uint16)])
+
 RECORD_DTYPE = np.dtype([("byte_begin", np.int64), ("byte_end", np.int64),
                          ("struct_first", np.int32), ("struct_last", np.int32),
                          ("chunk", np.int32), ("error", np.int32)])
@@ -194,6 +203,10 @@ def lib():
     L.cjx_chunk_lines_device.restype = ctypes.c_int
     L.cjx_load_device.argtypes = [vp, ctypes.c_char_p, vp, sz, szp]
     L.cjx_load_device.restype = ctypes.c_int
+    L.cjx_values_device.argtypes = [vp, vp, sz, vp, sz, vp, vp, szp]
+    L.cjx_values_device.restype = ctypes.c_int
+    L.cjx_values.argtypes = [vp, u8p, sz, ctypes.POINTER(_Result), vp, vp]
+    L.cjx_values.restype = ctypes.c_int
     L.cjx_server_create.argtypes = [ctypes.c_int, sz, ctypes.c_int, sz, ctypes.POINTER(vp)]
     L.cjx_server_create.restype = ctypes.c_int
     L.cjx_server_reserve.argtypes = [vp, sz]
@@ -334,6 +347,33 @@ class Context:
         rc = lib().cjx_load_device(self._h, os.fsencode(path), ctypes.c_void_p(d_buf), cap,
                                    ctypes.byref(k))
         return rc, k.value
+
+    def values_device(self, d_input: int, n: int, d_structural: int, n_structural: int,
+                      d_values: int, d_strings: int = 0):
+        """(rc, n_exact): the typed value table (VALUE_DTYPE, n_structural + 1
+        entries) of device text and its structural row into d_values; unescaped
+        strings into d_strings (n bytes) when given."""
+        k = ctypes.c_size_t()
+        rc = lib().cjx_values_device(self._h, ctypes.c_void_p(d_input), n,
+                                     ctypes.c_void_p(d_structural), n_structural,
+                                     ctypes.c_void_p(d_values), ctypes.c_void_p(d_strings),
+                                     ctypes.byref(k))
+        return rc, k.value
+
+    def values(self, data, result: "cuJSONResult", strings: bool = True):
+        """(table, strings) for a parse result of `data`: table is a VALUE_DTYPE
+        array of N + 1 entries; strings (bytes of len(data)) holds the unescaped
+        keys and strings at their text_off."""
+        a, p = _as_u8(data)
+        N = int(result._r.n_structural)
+        table = np.zeros(N + 1, VALUE_DTYPE)
+        sbuf = np.zeros(a.size if strings else 1, np.uint8)
+        rc = lib().cjx_values(self._h, p, a.size, ctypes.byref(result._r),
+                              table.ctypes.data_as(ctypes.c_void_p),
+                              sbuf.ctypes.data_as(ctypes.c_void_p) if strings else None)
+        if rc != CJX_OK:
+            raise RuntimeError(f"cjx_values: {strerror(rc)}")
+        return table, (sbuf.tobytes() if strings else None)
 
     def batch_wait(self):
         """(rc, docs, N, M, overflow): docs is an int64 array [k, 4] of

@@ -29,6 +29,7 @@
 using namespace cjx;
 
 static_assert(sizeof(RecordOut) == sizeof(cjx_record), "record layout");
+static_assert(sizeof(ValueOut) == sizeof(cjx_value), "value layout");
 static_assert(sizeof(cjx_doc_result) == 32, "document result layout");
 
 namespace cjx_host {
@@ -752,7 +753,7 @@ void cjx_context_destroy(cjx_context *ctx) {
                       &ctx->gagg,    &ctx->gd0,        &ctx->lows,   &ctx->highs,
                       &ctx->chunk_unit, &ctx->big_units, &ctx->unit_part, &ctx->rec_count,
                       &ctx->recs,       &ctx->row_nl,   &ctx->nl_seg, &ctx->nl_off,
-                      &ctx->nl_idx};
+                      &ctx->nl_idx,     &ctx->val_exact};
     for (DevBuf *b : bufs) b->release();
     ctx->h_units.release();
     ctx->h_uout.release();
@@ -1039,6 +1040,62 @@ int cjx_chunk_lines_device(cjx_context *ctx, const uint8_t *d_input, size_t n, i
         bytes_rule(at, (size_t)lines + 1, n, param, out);
     }
     return emit_chunks(out, chunk_off, chunk_size, cap, n_chunks);
+}
+
+int cjx_values_device(cjx_context *ctx, const uint8_t *d_input, size_t n,
+                      const int32_t *d_structural, size_t n_structural, cjx_value *d_values,
+                      uint8_t *d_strings, size_t *n_exact) {
+    if (n_exact) *n_exact = 0;
+    if (!ctx || !d_input || n == 0 || n > 0xFFFFFFFFull || !d_values ||
+        (n_structural && !d_structural) || n_structural >= n + 1)
+        return CJX_ERR_INPUT;
+    if (int rc = set_device(ctx)) return rc;
+    // the list of numbers for the exact step, then its count
+    if (!ctx->val_exact.ensure((n_structural + 1) * 4 + 16)) return CJX_ERR_ALLOC;
+    uint32_t *list = ctx->val_exact.as<uint32_t>();
+    uint32_t *count = list + n_structural + 1;
+    hipStream_t s = ctx->stream;
+    if (launch_values(s, d_input, n, d_structural, n_structural,
+                      reinterpret_cast<ValueOut *>(d_values), d_strings, list, count) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    if (n_exact) {
+        if (!ctx->h_nl.ensure(64)) return CJX_ERR_ALLOC;
+        if (hipMemcpyAsync(ctx->h_nl.p, count, 4, hipMemcpyDeviceToHost, s) != hipSuccess)
+            return CJX_ERR_DEVICE;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return CJX_ERR_DEVICE;
+    if (n_exact) *n_exact = *static_cast<const uint32_t *>(ctx->h_nl.p);
+    return CJX_OK;
+}
+
+int cjx_values(cjx_context *ctx, const uint8_t *data, size_t n, const cjx_result *r,
+               cjx_value *values, uint8_t *strings) {
+    if (!ctx || !data || n == 0 || n > 0xFFFFFFFFull || !r || !r->structural || !values ||
+        r->n_structural < 0)
+        return CJX_ERR_INPUT;
+    if (int rc = set_device(ctx)) return rc;
+    const size_t N = (size_t)r->n_structural;
+    // device copies: the text, its structural row, the table and the strings
+    if (!ctx->in.ensure(n + 16) || !ctx->structural.ensure(N * 4 + 16) ||
+        !ctx->layout.ensure((N + 1) * sizeof(cjx_value) + (strings ? n : 0) + 16))
+        return CJX_ERR_ALLOC;
+    hipStream_t s = ctx->stream;
+    cjx_value *d_values = ctx->layout.as<cjx_value>();
+    uint8_t *d_strings = strings ? reinterpret_cast<uint8_t *>(d_values + N + 1) : nullptr;
+    if (hipMemcpyAsync(ctx->in.p, data, n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        (N && hipMemcpyAsync(ctx->structural.p, r->structural + 1, N * 4, hipMemcpyHostToDevice,
+                             s) != hipSuccess))
+        return CJX_ERR_DEVICE;
+    int rc = cjx_values_device(ctx, ctx->in.as<uint8_t>(), n, ctx->structural.as<int32_t>(), N,
+                               d_values, d_strings, nullptr);
+    if (rc) return rc;
+    if (hipMemcpyAsync(values, d_values, (N + 1) * sizeof(cjx_value), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        (strings &&
+         hipMemcpyAsync(strings, d_strings, n, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    return CJX_OK;
 }
 
 int cjx_load_device(cjx_context *ctx, const char *path, uint8_t *d_buf, size_t cap, size_t *n) {

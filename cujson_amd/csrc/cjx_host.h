@@ -103,6 +103,7 @@ struct cjx_context {
     // Newline scan of device text: segment counts, bases and status; the
     // newline offsets and gathered line starts of the device chunker.
     DevBuf nl_seg, nl_off, nl_idx;
+    DevBuf val_exact;  // typed values: numbers for the exact step, and their count
     PinnedBuf h_nl;
     // File loads into device memory: pinned staging buffers, refilled from the
     // file while earlier ones are copied (created on first use).

@@ -192,6 +192,16 @@ hipError_t launch_place_units(hipStream_t s, const uint8_t *src, uint64_t src_li
                               uint8_t *dst, uint64_t dst_bytes);
 hipError_t launch_nl_gather(hipStream_t s, const uint64_t *nl, uint64_t n_nl, uint64_t n,
                             const uint64_t *line_idx, uint64_t count, uint64_t *out);
+// One entry of the typed value table (the C ABI's cjx_value).
+struct ValueOut {
+    int64_t v;  // INT value, FLOAT bits, KEY / STRING unescaped length
+    uint32_t row, text_off, text_len;
+    uint8_t type, flags;
+    uint16_t reserved;
+};
+hipError_t launch_values(hipStream_t s, const uint8_t *d_in, uint64_t n, const int32_t *st,
+                         uint64_t N, ValueOut *out, uint8_t *strings, uint32_t *exact_list,
+                         uint32_t *exact_count);
 hipError_t launch_records(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
                           const uint32_t *d_struct, uint32_t *rec_count, RecordOut *rec,
                           uint64_t max_records, bool write);

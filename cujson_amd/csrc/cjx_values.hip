@@ -1,0 +1,171 @@
+// cjx_values.hip - typed value table of a parse.
+//
+// The reference returns values as raw substrings: its iterator's getValue
+// cuts the text between two structural positions
+// (cujson/query/query_iterator_standard_json.cpp:475-567).  Here every gap
+// between consecutive structural characters is decoded once on the device:
+// gap h (h = 0 .. N) is the text after structural h (host layout; 0 = before
+// the first) up to structural h + 1 (the end of the text after the last).
+// Trimmed of whitespace it is empty, a key (a string before ':'), a string,
+// a number, true / false / null, or invalid (cjx_values.h decodes).
+//   k_values        one thread per gap; numbers whose rounding needs the
+//                   exact comparison are listed (gap index, atomic append)
+//   k_values_exact  those numbers, grid-stride over the list
+// String bodies are unescaped into `strings` at the offset of their raw
+// body, which is never shorter, so no scan is needed: strings[text_off ..
+// text_off + length) holds the UTF-8 of a key or string.
+#include "cjx_device.h"
+#include "cjx_values.h"
+
+namespace cjx {
+
+namespace {
+
+struct GapText {
+    uint32_t a, b;   // trimmed text [a, b)
+    uint8_t next;    // the structural character after the gap (0 after the last)
+};
+
+__device__ __forceinline__ GapText gap_of(const uint8_t *__restrict__ in, uint64_t n,
+                                          const int32_t *__restrict__ st, uint64_t N,
+                                          uint64_t h) {
+    // st[i] is the 1-based position of structural i + 1 (host index); rows
+    // that point outside the text are clamped to it
+    uint64_t a = h == 0 ? 0 : (uint64_t)(uint32_t)st[h - 1];
+    uint64_t b = h == N ? n : (uint64_t)(uint32_t)st[h] - 1u;
+    if (b > n) b = n;
+    if (a > b) a = b;
+    GapText g;
+    g.next = (h < N && b < n) ? in[b] : 0;
+    while (a < b && is_ws(in[a])) a++;
+    while (b > a && is_ws(in[b - 1])) b--;
+    g.a = (uint32_t)a;
+    g.b = (uint32_t)b;
+    return g;
+}
+
+__device__ __forceinline__ bool literal(const uint8_t *s, uint32_t len, const char *lit,
+                                        uint32_t k) {
+    if (len != k) return false;
+    for (uint32_t i = 0; i < k; i++)
+        if (s[i] != (uint8_t)lit[i]) return false;
+    return true;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, uint64_t n,
+                                                const int32_t *__restrict__ st, uint64_t N,
+                                                ValueOut *__restrict__ out,
+                                                uint8_t *__restrict__ strings,
+                                                uint32_t *__restrict__ exact_list,
+                                                uint32_t *__restrict__ exact_count) {
+    const uint64_t h = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (h > N) return;
+    const GapText g = gap_of(in, n, st, N, h);
+    ValueOut v;
+    v.v = 0;
+    v.row = (uint32_t)h;
+    v.text_off = g.a;
+    v.text_len = g.b - g.a;
+    v.type = kValNone;
+    v.flags = 0;
+    v.reserved = 0;
+    const uint8_t *s = in + g.a;
+    const uint32_t len = g.b - g.a;
+    if (len == 0) {
+        // nothing between the two structural characters
+    } else if (s[0] == '"') {
+        uint32_t out_len = 0;
+        uint8_t flags = 0;
+        if (len >= 2 && s[len - 1] == '"' &&
+            unescape(s + 1, len - 2, strings ? strings + g.a + 1 : nullptr, out_len, flags)) {
+            v.type = g.next == ':' ? kValKey : kValString;
+            v.flags = flags;
+            v.v = out_len;
+        } else {
+            v.type = kValInvalid;
+        }
+        v.text_off = g.a + 1;
+        v.text_len = len >= 2 ? len - 2 : 0;
+    } else if (literal(s, len, "true", 4)) {
+        v.type = kValTrue;
+    } else if (literal(s, len, "false", 5)) {
+        v.type = kValFalse;
+    } else if (literal(s, len, "null", 4)) {
+        v.type = kValNull;
+    } else {
+        NumText t;
+        uint64_t m_lo = 0;
+        int64_t e_ulp = 0, iv = 0;
+        uint8_t flags = 0;
+        v.type = parse_number(s, len, iv, flags, t, m_lo, e_ulp);
+        v.v = iv;
+        if (flags & kValExact) {
+            const uint32_t k = atomicAdd(exact_count, 1u);
+            exact_list[k] = (uint32_t)h;
+        }
+        v.flags = flags & (uint8_t)~kValExact;
+    }
+    out[h] = v;
+}
+
+__global__ __launch_bounds__(64) void k_values_exact(const uint8_t *__restrict__ in, uint64_t n,
+                                                     const int32_t *__restrict__ st, uint64_t N,
+                                                     ValueOut *__restrict__ out,
+                                                     const uint32_t *__restrict__ exact_list,
+                                                     const uint32_t *__restrict__ exact_count) {
+    const uint32_t cnt = *exact_count;
+    for (uint32_t i = blockIdx.x * 64u + threadIdx.x; i < cnt; i += gridDim.x * 64u) {
+        const uint64_t h = exact_list[i];
+        const GapText g = gap_of(in, n, st, N, h);
+        NumText t;
+        uint64_t m_lo = 0;
+        int64_t e_ulp = 0, iv = 0;
+        uint8_t flags = 0;
+        parse_number(in + g.a, g.b - g.a, iv, flags, t, m_lo, e_ulp);
+        finish_exact(in + g.a, t, m_lo, e_ulp, iv, flags);
+        out[h].v = iv;
+        out[h].flags = flags;
+    }
+}
+
+__global__ void k_values_reset(uint32_t *__restrict__ exact_count) { *exact_count = 0; }
+
+hipError_t launch_values(hipStream_t s, const uint8_t *d_in, uint64_t n, const int32_t *st,
+                         uint64_t N, ValueOut *out, uint8_t *strings, uint32_t *exact_list,
+                         uint32_t *exact_count) {
+    hipLaunchKernelGGL(k_values_reset, dim3(1), dim3(1), 0, s, exact_count);
+    hipLaunchKernelGGL(k_values, dim3((uint32_t)((N + 1 + 255) / 256)), dim3(256), 0, s, d_in, n, st,
+                       N, out, strings, exact_list, exact_count);
+    hipLaunchKernelGGL(k_values_exact, dim3(64), dim3(64), 0, s, d_in, n, st, N, out, exact_list,
+                       exact_count);
+    return hipGetLastError();
+}
+
+}  // namespace cjx
+
+// Host runs of the decoders, for the self-tests that run without a GPU.
+extern "C" int cjx_debug_parse_number(const uint8_t *s, size_t len, int64_t *value,
+                                      uint8_t *flags) {
+    using namespace cjx;
+    NumText t;
+    uint64_t m_lo = 0;
+    int64_t e_ulp = 0, iv = 0;
+    uint8_t f = 0;
+    const uint8_t type = parse_number(s, (uint32_t)len, iv, f, t, m_lo, e_ulp);
+    if (f & kValExact) finish_exact(s, t, m_lo, e_ulp, iv, f);
+    *value = iv;
+    *flags = f;
+    return type;
+}
+
+extern "C" int cjx_debug_unescape(const uint8_t *s, size_t len, uint8_t *out, size_t *out_len,
+                                  uint8_t *flags) {
+    uint32_t k = 0;
+    uint8_t f = 0;
+    const bool ok = cjx::unescape(s, (uint32_t)len, out, k, f);
+    *out_len = k;
+    *flags = f;
+    return ok ? 1 : 0;
+}

@@ -284,6 +284,67 @@ int cjx_chunk_lines_device(cjx_context *ctx, const uint8_t *d_input, size_t n, i
 int cjx_load_device(cjx_context *ctx, const char *path, uint8_t *d_buf, size_t cap, size_t *n);
 
 /* ------------------------------------------------------------------------
+ * Typed value table (no counterpart in the reference, whose iterator returns
+ * values as raw substrings: query_iterator_standard_json.cpp:475-567).
+ * Entry h (h = 0 .. N) describes the text after structural h (host-layout
+ * index; 0 = before the first) up to structural h + 1 (the end of the text
+ * after the last), trimmed of JSON whitespace.
+ *   KEY / STRING  text_off / text_len: the body between the quotes; the
+ *                 unescaped UTF-8 (length v.i) is in strings[text_off ..)
+ *                 when a strings buffer is given (an unpaired surrogate
+ *                 escape is written as its 3-byte form, flag UNPAIRED)
+ *   INT           v.i: an integer literal that fits int64
+ *   FLOAT         v.d: any other number, binary64 correctly rounded (round
+ *                 half to even); BIG_INT marks an integer literal outside
+ *                 int64; APPROX one of more than 800 significant digits
+ *                 whose rounding was not decided exactly (nearest candidate)
+ *   TRUE / FALSE / NULL, NONE (empty), INVALID (no JSON scalar, e.g. 01,
+ *                 NaN, a bad escape, a raw control character in a string)
+ * ------------------------------------------------------------------------ */
+enum {
+    CJX_VALUE_NONE = 0,
+    CJX_VALUE_KEY = 1,
+    CJX_VALUE_STRING = 2,
+    CJX_VALUE_INT = 3,
+    CJX_VALUE_FLOAT = 4,
+    CJX_VALUE_TRUE = 5,
+    CJX_VALUE_FALSE = 6,
+    CJX_VALUE_NULL = 7,
+    CJX_VALUE_INVALID = 8
+};
+#define CJX_VALUE_ESCAPED  0x1u /* the string held escapes                      */
+#define CJX_VALUE_BIG_INT  0x2u /* integer literal outside int64, given as v.d  */
+#define CJX_VALUE_UNPAIRED 0x4u /* unpaired surrogate escape in the string      */
+#define CJX_VALUE_APPROX   0x8u /* v.d not decided exactly (over 800 digits)    */
+
+typedef struct cjx_value {
+    union {
+        int64_t i;
+        double d;
+    } v;
+    uint32_t row;      /* h */
+    uint32_t text_off; /* byte offset of the text (strings: of the body)    */
+    uint32_t text_len; /* its length                                         */
+    uint8_t type;      /* CJX_VALUE_*                                         */
+    uint8_t flags;     /* CJX_VALUE_ESCAPED | BIG_INT | UNPAIRED | APPROX     */
+    uint16_t reserved;
+} cjx_value;
+
+/* Device form: d_input[0..n) (n < 2^32) and the structural row of a parse of
+ * it (d_structural[0..n_structural): 1-based positions, as cjx_parse_device
+ * writes them); fills d_values[0..n_structural] and, if d_strings is not
+ * null, the unescaped strings in d_strings[0..n).  *n_exact (optional)
+ * receives how many numbers needed the exact comparison.  Synchronous. */
+int cjx_values_device(cjx_context *ctx, const uint8_t *d_input, size_t n,
+                      const int32_t *d_structural, size_t n_structural, cjx_value *d_values,
+                      uint8_t *d_strings, size_t *n_exact);
+
+/* Host form for a cjx_result of cjx_parse_standard / cjx_parse_lines over
+ * data[0..n): fills values[0..N] and, if strings is not null, strings[0..n). */
+int cjx_values(cjx_context *ctx, const uint8_t *data, size_t n, const cjx_result *r,
+               cjx_value *values, uint8_t *strings);
+
+/* ------------------------------------------------------------------------
  * Request server (no counterpart in the reference, whose parse calls are
  * synchronous one-document calls: parse_standard_json.cu:1590-1700).  For
  * serving callers with many concurrent requests: submit from any thread, a
