@@ -8,13 +8,15 @@ namespace cjx {
 
 // Tokenizer tile geometry: a tile is one workgroup of CJX_TOK_WAVES wave64s;
 // each lane owns 32 contiguous bytes per round, so a wave covers 2 KiB per
-// round and runs CJX_TOK_ROUNDS rounds.  8 x 8 (128 KiB tiles) measured
-// fastest on a 2 GB document: fewer tiles shorten the look-back chain.
+// round and runs CJX_TOK_ROUNDS rounds.  4 x 8 (64 KiB tiles) measured
+// fastest on a 2 GB document with 5 waves per SIMD: five workgroups fit a
+// CU, so one waiting in its look-back leaves four to compute (8 x 8: 5 %
+// slower, 4 x 6 or 4 x 12: 7 % and 3 % slower, 2 x 8: 50 % slower).
 #ifndef CJX_TOK_ROUNDS
 #define CJX_TOK_ROUNDS 8
 #endif
 #ifndef CJX_TOK_WAVES
-#define CJX_TOK_WAVES 8
+#define CJX_TOK_WAVES 4
 #endif
 constexpr int kTokThreads = 64 * CJX_TOK_WAVES;
 constexpr int kWaves = kTokThreads / 64;

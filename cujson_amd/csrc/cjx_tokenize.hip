@@ -7,7 +7,7 @@
 // scans (cujson/parse_standard_json.cu:446-1343) are carried by
 //   * in-lane recurrences on 32-bit masks,
 //   * ballots across the 64 lanes of a wave,
-//   * a decoupled look-back across 128 KiB tiles: the escape/string state as an
+//   * a decoupled look-back across 64 KiB tiles: the escape/string state as an
 //     affine map over GF(2) (4 bits), and the structural and bracket counts
 //     as one packed 64-bit prefix sum.
 //
@@ -327,7 +327,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     return wave_read(wave_incl_scan(x), 63);
 }
 
-// Large units (SMALL = false): one workgroup per 128 KiB tile; tiles are taken
+// Large units (SMALL = false): one workgroup per 64 KiB tile; tiles are taken
 // from a ticket counter in launch order so that every look-back waits only on
 // running workgroups.  Small units (SMALL = true, at most one wave's 16 KiB):
 // one wave per unit, which starts in state (0, 0) and owns the unit's whole

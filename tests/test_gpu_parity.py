@@ -72,7 +72,7 @@ def test_twitter_small_records_chunked(ctx, twitter_small, cc):
 
 
 def test_small_sizes_and_tile_edges(ctx):
-    # lane (32 B), wave round (2 KiB), wave (16 KiB) and tile (128 KiB) edges
+    # lane (32 B), wave round (2 KiB), wave (16 KiB) and tile (64 KiB) edges
     base = bytes(gen.config1()[:270000])
     for n in [1, 2, 3, 4, 5, 15, 16, 17, 31, 32, 33, 63, 64, 65, 1023, 1024, 1025, 2047, 2048,
               2049, 4095, 4096, 4097, 16383, 16384, 16385, 32767, 32768, 32769, 65535, 65536,
