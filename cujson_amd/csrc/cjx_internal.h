@@ -9,9 +9,9 @@ namespace cjx {
 // Tokenizer tile geometry: a tile is one workgroup of CJX_TOK_WAVES wave64s;
 // each lane owns 32 contiguous bytes per round, so a wave covers 2 KiB per
 // round and runs CJX_TOK_ROUNDS rounds.  4 x 8 (64 KiB tiles) measured
-// fastest on a 2 GB document with 5 waves per SIMD: five workgroups fit a
-// CU, so one waiting in its look-back leaves four to compute (8 x 8: 5 %
-// slower, 4 x 6 or 4 x 12: 7 % and 3 % slower, 2 x 8: 50 % slower).
+// fastest on a 2 GB document: four workgroups fit a CU (4 waves per SIMD),
+// so one waiting in its look-back leaves three to compute (8 x 8: 2-5 %
+// slower, 4 x 6 or 4 x 12: 7-14 % and 3 % slower, 2 x 8: 50 % slower).
 #ifndef CJX_TOK_ROUNDS
 #define CJX_TOK_ROUNDS 8
 #endif

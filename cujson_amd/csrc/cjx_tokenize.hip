@@ -327,15 +327,144 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     return wave_read(wave_incl_scan(x), 63);
 }
 
+// State of a wave after phases 1 and 2 (everything before the look-back).
+struct WaveScan {
+    uint32_t instr_r[kTokRounds], op_r[kTokRounds], br_r[kTokRounds];
+    uint32_t c_run, s_run, seen, dflag, utf8_bad, lane_cnt0, lane_all;
+};
+
+// Phases 1 and 2 over the rounds of one wave.  INSIDE: the wave's whole
+// range lies inside the unit (a uniform condition, true for all but the last
+// waves of a unit), so the rounds need no end checks and the loads no bounds
+// checks; a separate instantiation keeps that path free of the branches
+// (and the register copies at their joins) that the edge waves need.
+template <bool JSONL, bool INSIDE>
+__device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_off, uint64_t n,
+                                            uint32_t lane, uint64_t lt, uint32_t weak_utf8,
+                                            uint32_t (&b1)[kTokRounds][64],
+                                            uint32_t (&b5)[kTokRounds][64], WaveScan &ws) {
+    // Rounds stream through a double buffer: round i+2 is loaded as soon as
+    // round i has been transposed.
+    uint32_t x[2][8];
+    load_lane(ubase, wave_off + (uint64_t)lane * kLaneBytes, n, INSIDE, x[0]);
+    if (kTokRounds > 1)
+        load_lane(ubase, wave_off + kRoundBytes + (uint64_t)lane * kLaneBytes, n, INSIDE, x[1]);
+    uint32_t prev_dw = 0;
+    if (wave_off > 0 && wave_off < n)
+        prev_dw = *reinterpret_cast<const uint32_t *>(ubase + wave_off - 4);
+
+    // Carries entering the wave: the four bytes before it (UTF-8 context).
+    uint32_t c_k5, c_k6, c_k7, c_r7;
+    {
+        const uint32_t cp = classify<JSONL>(prev_dw);
+        c_k7 = cp;
+        c_k6 = cp << 8;
+        c_k5 = cp << 16;
+        c_r7 = prev_dw;
+    }
+
+    uint32_t(&instr_r)[kTokRounds] = ws.instr_r;
+    uint32_t(&op_r)[kTokRounds] = ws.op_r;
+    uint32_t(&br_r)[kTokRounds] = ws.br_r;
+    uint32_t &c_run = ws.c_run, &s_run = ws.s_run, &seen = ws.seen, &dflag = ws.dflag;
+    uint32_t &utf8_bad = ws.utf8_bad;
+    uint32_t &lane_cnt0 = ws.lane_cnt0, &lane_all = ws.lane_all;
+    c_run = s_run = seen = dflag = utf8_bad = lane_cnt0 = lane_all = 0;
+
+#pragma unroll
+    for (int i = 0; i < kTokRounds; i++) {
+        // Rounds past the end of the unit hold no bytes: no masks, carries
+        // unchanged (nothing after the unit's end depends on them).
+        if (!INSIDE && wave_off + (uint64_t)i * kRoundBytes >= n) {
+            instr_r[i] = op_r[i] = br_r[i] = 0;
+            continue;
+        }
+        const uint64_t lane_off = wave_off + (uint64_t)i * kRoundBytes + (uint64_t)lane * kLaneBytes;
+        uint32_t r[8], k[8];
+        transpose4(x[i & 1][0], x[i & 1][2], x[i & 1][4], x[i & 1][6], r[0], r[1], r[2], r[3]);
+        transpose4(x[i & 1][1], x[i & 1][3], x[i & 1][5], x[i & 1][7], r[4], r[5], r[6], r[7]);
+        if (i + 2 < kTokRounds)
+            load_lane(ubase, lane_off + 2 * kRoundBytes, n, INSIDE, x[i & 1]);
+#pragma unroll
+        for (int d = 0; d < 8; d++) k[d] = classify<JSONL>(r[d]);
+
+        // ---- phase 1: UTF-8 ---------------------------------------------------
+        {
+            const uint32_t pk5 = wave_shr1(k[5], c_k5), pk6 = wave_shr1(k[6], c_k6);
+            const uint32_t pk7 = wave_shr1(k[7], c_k7), pr7 = wave_shr1(r[7], c_r7);
+            const uint32_t high = ((r[0] | r[1] | r[2] | r[3] | r[4] | r[5] | r[6] | r[7]) &
+                                   0x80808080u) |
+                                  ((pk5 | pk6 | pk7) & 0x62000000u) | (pr7 & 0x80000000u);
+            if (weak_utf8 && __ballot(high != 0)) {
+                uint32_t vA = ~0u, vB = ~0u;
+                if (!INSIDE && lane_off + 32 > n) {
+                    vA = vB = 0;
+#pragma unroll
+                    for (int s = 0; s < 4; s++) {
+                        if (lane_off + 8 * s < n) vA |= 0xFFu << (8 * s);
+                        if (lane_off + 8 * s + 4 < n) vB |= 0xFFu << (8 * s);
+                    }
+                }
+                if (high && utf8_lane_error(r, k, pk5, pk6, pk7, pr7, vA, vB)) utf8_bad = 1;
+            }
+            c_k5 = wave_read(k[5], 63);
+            c_k6 = wave_read(k[6], 63);
+            c_k7 = wave_read(k[7], 63);
+            c_r7 = wave_read(r[7], 63);
+        }
+
+        // ---- phase 2: masks, escapes, strings ------------------------------
+        // Class bits 1 and 5 (CONT, E) give way to raw bits 1 and 5, then one
+        // transpose yields every mask.
+#pragma unroll
+        for (int d = 0; d < 8; d++) k[d] = (k[d] & ~0x22222222u) | (r[d] & 0x22222222u);
+        transpose8x8(k);
+        const uint32_t bs = k[4];
+        const uint32_t q = k[7];
+        br_r[i] = k[2];
+        op_r[i] = k[0] | k[2] | k[3];
+        b1[i][lane] = k[1] & k[2];
+        b5[i][lane] = (k[5] & k[2]) | (JSONL ? (k[0] & ~k[5]) : 0u);
+
+        const uint32_t all_bs = (bs == 0xFFFFFFFFu);
+        const uint32_t run = (uint32_t)__clz(~bs);  // trailing backslash run of the lane
+        const uint64_t NB = __ballot(!all_bs);
+        const uint64_t TP = __ballot(run & 1u);
+        const uint64_t mm = NB & lt;
+        const uint32_t ov = mm ? (uint32_t)((TP >> (63 - __clzll(mm))) & 1ull) : c_run;
+        const uint32_t rq = unescaped32(bs, q, ov);
+        const uint32_t px = prefix_xor32(rq);
+        const uint64_t QP = __ballot(__popc(rq) & 1u);
+        const uint32_t s_lane = s_run ^ (uint32_t)(__popcll(QP & lt) & 1);
+        instr_r[i] = px ^ (s_lane ? ~0u : 0u);
+        lane_cnt0 += (uint32_t)__popc(op_r[i] & ~instr_r[i]) |
+                     ((uint32_t)__popc(br_r[i] & ~instr_r[i]) << 16);
+        lane_all += (uint32_t)__popc(op_r[i]) | ((uint32_t)__popc(br_r[i]) << 16);
+
+        if (!seen && NB) {  // first non-backslash byte of the wave: is it a quote?
+            const uint32_t k0 = (uint32_t)__ffsll((long long)NB) - 1;
+            const uint32_t nbpos = (uint32_t)__ffs(~bs) - 1;
+            const uint32_t fq = all_bs ? 0u : ((q >> nbpos) & 1u);
+            const uint64_t FQ = __ballot(fq);
+            dflag = (uint32_t)((FQ >> k0) & 1ull);
+            seen = 1;
+        }
+        if (NB) c_run = (uint32_t)((TP >> (63 - __clzll(NB))) & 1ull);
+        s_run ^= (uint32_t)(__popcll(QP) & 1);
+    }
+
+}
+
 // Large units (SMALL = false): one workgroup per 64 KiB tile; tiles are taken
 // from a ticket counter in launch order so that every look-back waits only on
 // running workgroups.  Small units (SMALL = true, at most one wave's 16 KiB):
 // one wave per unit, which starts in state (0, 0) and owns the unit's whole
 // output region, so there is no look-back and no workgroup barrier.
-// Register budget: 5 waves per SIMD (at most 102 VGPRs, no spills) measured
-// faster than the unconstrained 107-VGPR allocation.
+// Register budget: 4 waves per SIMD (103 VGPRs, no spills).  With the
+// interior and edge rounds as separate instantiations, the 5-wave budget
+// (96 VGPRs) spills and measured 4 % slower.
 #ifndef CJX_TOK_WPE
-#define CJX_TOK_WPE 5
+#define CJX_TOK_WPE 4
 #endif
 #if CJX_TOK_WPE > 0
 #define CJX_TOK_ATTR __attribute__((amdgpu_waves_per_eu(CJX_TOK_WPE, CJX_TOK_WPE)))
@@ -389,113 +518,17 @@ __global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
     const uint64_t wave_off =
         SMALL ? 0ull : (uint64_t)tile_in_unit * kTileBytes + (uint64_t)w * kWaveBytes;
 
-    // Rounds stream through a double buffer: round i+2 is loaded as soon as
-    // round i has been transposed.
     const bool inside = wave_off + (uint64_t)kWaveBytes <= U.n;
-    uint32_t x[2][8];
-    load_lane(ubase, wave_off + (uint64_t)lane * kLaneBytes, U.n, inside, x[0]);
-    if (kTokRounds > 1)
-        load_lane(ubase, wave_off + kRoundBytes + (uint64_t)lane * kLaneBytes, U.n, inside, x[1]);
-    uint32_t prev_dw = 0;
-    if (wave_off > 0 && wave_off < U.n)
-        prev_dw = *reinterpret_cast<const uint32_t *>(ubase + wave_off - 4);
-
-    // Carries entering the wave: the four bytes before it (UTF-8 context).
-    uint32_t c_k5, c_k6, c_k7, c_r7;
-    {
-        const uint32_t cp = classify<JSONL>(prev_dw);
-        c_k7 = cp;
-        c_k6 = cp << 8;
-        c_k5 = cp << 16;
-        c_r7 = prev_dw;
-    }
-
-    uint32_t instr_r[kTokRounds], op_r[kTokRounds], br_r[kTokRounds];
-    uint32_t c_run = 0, s_run = 0, seen = 0, dflag = 0;
-    uint32_t utf8_bad = 0;
-    uint32_t lane_cnt0 = 0, lane_all = 0;  // packed n | m << 16, string fix 0 / regardless
-
-#pragma unroll
-    for (int i = 0; i < kTokRounds; i++) {
-        // Rounds past the end of the unit hold no bytes: no masks, carries
-        // unchanged (nothing after the unit's end depends on them).
-        if (wave_off + (uint64_t)i * kRoundBytes >= U.n) {
-            instr_r[i] = op_r[i] = br_r[i] = 0;
-            continue;
-        }
-        const uint64_t lane_off = wave_off + (uint64_t)i * kRoundBytes + (uint64_t)lane * kLaneBytes;
-        uint32_t r[8], k[8];
-        transpose4(x[i & 1][0], x[i & 1][2], x[i & 1][4], x[i & 1][6], r[0], r[1], r[2], r[3]);
-        transpose4(x[i & 1][1], x[i & 1][3], x[i & 1][5], x[i & 1][7], r[4], r[5], r[6], r[7]);
-        if (i + 2 < kTokRounds)
-            load_lane(ubase, lane_off + 2 * kRoundBytes, U.n, inside, x[i & 1]);
-#pragma unroll
-        for (int d = 0; d < 8; d++) k[d] = classify<JSONL>(r[d]);
-
-        // ---- phase 1: UTF-8 ---------------------------------------------------
-        {
-            const uint32_t pk5 = wave_shr1(k[5], c_k5), pk6 = wave_shr1(k[6], c_k6);
-            const uint32_t pk7 = wave_shr1(k[7], c_k7), pr7 = wave_shr1(r[7], c_r7);
-            const uint32_t high = ((r[0] | r[1] | r[2] | r[3] | r[4] | r[5] | r[6] | r[7]) &
-                                   0x80808080u) |
-                                  ((pk5 | pk6 | pk7) & 0x62000000u) | (pr7 & 0x80000000u);
-            if (weak_utf8 && __ballot(high != 0)) {
-                uint32_t vA = ~0u, vB = ~0u;
-                if (lane_off + 32 > U.n) {
-                    vA = vB = 0;
-#pragma unroll
-                    for (int s = 0; s < 4; s++) {
-                        if (lane_off + 8 * s < U.n) vA |= 0xFFu << (8 * s);
-                        if (lane_off + 8 * s + 4 < U.n) vB |= 0xFFu << (8 * s);
-                    }
-                }
-                if (high && utf8_lane_error(r, k, pk5, pk6, pk7, pr7, vA, vB)) utf8_bad = 1;
-            }
-            c_k5 = wave_read(k[5], 63);
-            c_k6 = wave_read(k[6], 63);
-            c_k7 = wave_read(k[7], 63);
-            c_r7 = wave_read(r[7], 63);
-        }
-
-        // ---- phase 2: masks, escapes, strings ------------------------------
-        // Class bits 1 and 5 (CONT, E) give way to raw bits 1 and 5, then one
-        // transpose yields every mask.
-#pragma unroll
-        for (int d = 0; d < 8; d++) k[d] = (k[d] & ~0x22222222u) | (r[d] & 0x22222222u);
-        transpose8x8(k);
-        const uint32_t bs = k[4];
-        const uint32_t q = k[7];
-        br_r[i] = k[2];
-        op_r[i] = k[0] | k[2] | k[3];
-        s_b1[w][i][lane] = k[1] & k[2];
-        s_b5[w][i][lane] = (k[5] & k[2]) | (JSONL ? (k[0] & ~k[5]) : 0u);
-
-        const uint32_t all_bs = (bs == 0xFFFFFFFFu);
-        const uint32_t run = (uint32_t)__clz(~bs);  // trailing backslash run of the lane
-        const uint64_t NB = __ballot(!all_bs);
-        const uint64_t TP = __ballot(run & 1u);
-        const uint64_t mm = NB & lt;
-        const uint32_t ov = mm ? (uint32_t)((TP >> (63 - __clzll(mm))) & 1ull) : c_run;
-        const uint32_t rq = unescaped32(bs, q, ov);
-        const uint32_t px = prefix_xor32(rq);
-        const uint64_t QP = __ballot(__popc(rq) & 1u);
-        const uint32_t s_lane = s_run ^ (uint32_t)(__popcll(QP & lt) & 1);
-        instr_r[i] = px ^ (s_lane ? ~0u : 0u);
-        lane_cnt0 += (uint32_t)__popc(op_r[i] & ~instr_r[i]) |
-                     ((uint32_t)__popc(br_r[i] & ~instr_r[i]) << 16);
-        lane_all += (uint32_t)__popc(op_r[i]) | ((uint32_t)__popc(br_r[i]) << 16);
-
-        if (!seen && NB) {  // first non-backslash byte of the wave: is it a quote?
-            const uint32_t k0 = (uint32_t)__ffsll((long long)NB) - 1;
-            const uint32_t nbpos = (uint32_t)__ffs(~bs) - 1;
-            const uint32_t fq = all_bs ? 0u : ((q >> nbpos) & 1u);
-            const uint64_t FQ = __ballot(fq);
-            dflag = (uint32_t)((FQ >> k0) & 1ull);
-            seen = 1;
-        }
-        if (NB) c_run = (uint32_t)((TP >> (63 - __clzll(NB))) & 1ull);
-        s_run ^= (uint32_t)(__popcll(QP) & 1);
-    }
+    WaveScan ws;
+    if (inside)
+        scan_rounds<JSONL, true>(ubase, wave_off, U.n, lane, lt, weak_utf8, s_b1[w], s_b5[w], ws);
+    else
+        scan_rounds<JSONL, false>(ubase, wave_off, U.n, lane, lt, weak_utf8, s_b1[w], s_b5[w], ws);
+    const uint32_t(&instr_r)[kTokRounds] = ws.instr_r;
+    const uint32_t(&op_r)[kTokRounds] = ws.op_r;
+    const uint32_t(&br_r)[kTokRounds] = ws.br_r;
+    const uint32_t c_run = ws.c_run, s_run = ws.s_run, seen = ws.seen, dflag = ws.dflag;
+    const uint32_t utf8_bad = ws.utf8_bad, lane_cnt0 = ws.lane_cnt0, lane_all = ws.lane_all;
 
     const uint64_t bad_lanes = __ballot(utf8_bad);
     const uint32_t wcnt0 = wave_sum(lane_cnt0), wall = wave_sum(lane_all);
