@@ -141,6 +141,57 @@ __global__ __launch_bounds__(kBaseThreads) void k_ub_final(UnitOut *__restrict__
         for (uint32_t j = 0; j < v[1]; j++) chunk_unit[pre[1] + j] = u;
 }
 
+// The same fields in one launch for launches of at most kOneBlockUnits
+// units: one block scans tiles of kBaseThreads units in turn with a carry,
+// then writes the chunk -> unit maps of the larger units with all threads.
+constexpr uint32_t kOneBlockUnits = 1024;
+__global__ __launch_bounds__(kBaseThreads) void k_unit_bases_one(UnitOut *__restrict__ uo,
+                                                                 uint32_t n_units, uint32_t concat,
+                                                                 LaunchState *__restrict__ ls,
+                                                                 uint32_t *__restrict__ chunk_unit,
+                                                                 uint32_t *__restrict__ big_units) {
+    __shared__ uint32_t wsum[kUnitCounters][kBaseThreads / 64];
+    __shared__ uint32_t carry[kUnitCounters];
+    __shared__ uint32_t s_nbig;
+    if (threadIdx.x < kUnitCounters) carry[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n_units; base += kBaseThreads) {
+        const uint32_t u = base + threadIdx.x;
+        uint32_t v[kUnitCounters] = {0, 0, 0, 0, 0};
+        if (u < n_units) unit_counts(uo[u], v);
+        uint32_t pre[kUnitCounters], tot[kUnitCounters];
+        block_scan4(v, pre, tot, wsum);
+        if (u < n_units) {
+#pragma unroll
+            for (int k = 0; k < kUnitCounters; k++) pre[k] += carry[k];
+            uo[u].row_base = pre[0];
+            uo[u].pair_base = concat ? pre[0] : 0u;
+            uo[u].chunk0 = pre[1];
+            uo[u].nchunks = v[1];
+            uo[u].grp0 = pre[2];
+            uo[u].sgrp0 = pre[3];
+            if (v[1] > 64) big_units[atomicAdd(&ls->n_big, 1u)] = u;
+            else
+                for (uint32_t j = 0; j < v[1]; j++) chunk_unit[pre[1] + j] = u;
+        }
+        __syncthreads();  // carry and wsum are read before they change
+        if (threadIdx.x < kUnitCounters) carry[threadIdx.x] += tot[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ls->n_total = carry[0];
+        ls->n_chunks = carry[1];
+        ls->m_total = carry[4];
+        s_nbig = atomicAdd(&ls->n_big, 0u);
+    }
+    __syncthreads();  // the big-unit list and the fields above are written
+    for (uint32_t bi = 0; bi < s_nbig; bi++) {
+        const uint32_t u = big_units[bi];
+        const uint32_t c0 = uo[u].chunk0, nch = uo[u].nchunks;
+        for (uint32_t j = threadIdx.x; j < nch; j += kBaseThreads) chunk_unit[c0 + j] = u;
+    }
+}
+
 // Chunk -> unit map of the units with more than 64 chunks, one block per unit.
 __global__ __launch_bounds__(256) void k_chunk_map(const UnitOut *__restrict__ uo,
                                                    const uint32_t *__restrict__ big_units,
@@ -726,6 +777,11 @@ hipError_t launch_assemble_docs(hipStream_t s, const LaunchBufs &b, uint32_t n_u
 }
 
 hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool concat) {
+    if (n_units <= kOneBlockUnits) {
+        hipLaunchKernelGGL(k_unit_bases_one, dim3(1), dim3(kBaseThreads), 0, s, b.uout, n_units,
+                           concat ? 1u : 0u, b.ls, b.chunk_unit, b.big_units);
+        return hipGetLastError();
+    }
     const uint32_t nb = (n_units + kBaseThreads - 1) / kBaseThreads;
     hipLaunchKernelGGL(k_ub_partial, dim3(nb), dim3(kBaseThreads), 0, s, b.uout, n_units,
                        b.unit_part);

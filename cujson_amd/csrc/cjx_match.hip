@@ -85,7 +85,8 @@ __device__ __forceinline__ DepthAgg wave_scan_agg(DepthAgg a) {
 }  // namespace
 
 // K1: depth walk of every chunk.
-__global__ __launch_bounds__(kMatchThreads) void k_chunk_stats(const uint8_t *__restrict__ oc_char,
+__device__ __forceinline__ void chunk_stats_blocks(uint32_t bid, uint32_t nblk,
+                                                   const uint8_t *__restrict__ oc_char,
                                                                const UnitOut *__restrict__ uo,
                                                                const uint32_t *__restrict__ chunk_unit,
                                                                ChunkStat *__restrict__ ck,
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_chunk_stats(const uint8_t *__
     __shared__ __align__(16) uint8_t s_ch[kMatchChunk];
     if (ls->overflow) return;
     const uint32_t n_chunks = ls->n_chunks;
-    for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    for (uint32_t c = bid; c < n_chunks; c += nblk) {
         const uint32_t u = chunk_unit[c];
         const uint32_t first = uo[u].m_base + (c - uo[u].chunk0) * kMatchChunk;
         const uint32_t cnt =
@@ -138,15 +139,15 @@ __global__ __launch_bounds__(kMatchThreads) void k_chunk_stats(const uint8_t *__
 // groups giving their start depths and minima, the 4096-chunk minima and
 // whether the unit is well nested, (c) each chunk's start depth and minimum.
 constexpr int kScanThreads = 1024;
-__global__ __launch_bounds__(kScanThreads) void k_chunk_scan(
-    UnitOut *__restrict__ uo, ChunkStat *__restrict__ ck, int32_t *__restrict__ gmin,
+__device__ __forceinline__ void chunk_scan_blocks(
+    uint32_t bid, uint32_t nblk, UnitOut *__restrict__ uo, ChunkStat *__restrict__ ck, int32_t *__restrict__ gmin,
     int32_t *__restrict__ gmin1, int32_t *__restrict__ gmin2, DepthAgg *__restrict__ gagg,
     int32_t *__restrict__ gd0, const uint32_t *__restrict__ big_units,
     const LaunchState *__restrict__ ls) {
     constexpr int kW = kScanThreads / 64;
     __shared__ DepthAgg part[kW];
     if (ls->overflow) return;
-    for (uint32_t bi = blockIdx.x; bi < ls->n_big; bi += gridDim.x) {
+    for (uint32_t bi = bid; bi < ls->n_big; bi += nblk) {
     __syncthreads();  // part[] of the previous unit no longer read
     const uint32_t u = big_units[bi], tid = threadIdx.x, w = tid >> 6, lane = lane_id();
     const uint32_t c0 = uo[u].chunk0, nch = uo[u].nchunks;
@@ -242,7 +243,7 @@ __global__ __launch_bounds__(kScanThreads) void k_chunk_scan(
 // K2 for units of at most 64 chunks, one wave per unit: the ordered scan of
 // the unit's chunk walks, chunk start depths and minima, the (single) group
 // minima, and the well-nested test.
-__global__ __launch_bounds__(256) void k_chunk_scan_small(UnitOut *__restrict__ uo,
+__device__ __forceinline__ void chunk_scan_small_block(uint32_t bid, UnitOut *__restrict__ uo,
                                                           uint32_t n_units,
                                                           ChunkStat *__restrict__ ck,
                                                           int32_t *__restrict__ gmin,
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(256) void k_chunk_scan_small(UnitOut *__restrict__ 
                                                           const LaunchState *__restrict__ ls) {
     if (ls->overflow) return;
     const uint32_t lane = lane_id();
-    const uint32_t u = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t u = bid * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (u >= n_units) return;
     const uint32_t nch = uo[u].nchunks;
     if (nch > 64) return;  // k_chunk_scan
@@ -491,7 +492,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_local(
 // the matching by ballots (an opener's closer is the next bracket of its
 // key).  The unit's pair row is written densely through a 64-cell LDS window.
 // A unit that is not well nested keeps fast = 0 and goes to the sort path.
-__global__ __launch_bounds__(256) void k_match_small(const uint8_t *__restrict__ oc_char,
+__device__ __forceinline__ void match_small_block(uint32_t bid, const uint8_t *__restrict__ oc_char,
                                                      const uint32_t *__restrict__ oc_idx,
                                                      UnitOut *__restrict__ uo, uint32_t n_units,
                                                      int32_t *__restrict__ pair_row,
@@ -499,7 +500,7 @@ __global__ __launch_bounds__(256) void k_match_small(const uint8_t *__restrict__
     __shared__ int32_t s_win[4][64];
     if (ls->overflow) return;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    const uint32_t u = blockIdx.x * 4 + w;
+    const uint32_t u = bid * 4 + w;
     if (u >= n_units) return;
     const uint32_t mb = uo[u].m_base, M = uo[u].m_end - mb;
     if (M > (uint32_t)kSmallBrackets) return;  // the chunk kernels
@@ -657,6 +658,32 @@ __global__ __launch_bounds__(64) void k_match_cross(
     }
 }
 
+// The first matching steps in two launches: blocks [0, g_stats) take the
+// chunk walks, the rest the units with at most kSmallBrackets brackets (four
+// per block); then blocks [0, g_big) scan the chunk walks of units with more
+// than 64 chunks, the rest those of the other units (one per wave).
+__global__ __launch_bounds__(kMatchThreads) void k_match_front(
+    const uint8_t *__restrict__ oc_char, const uint32_t *__restrict__ oc_idx,
+    UnitOut *__restrict__ uo, uint32_t n_units, const uint32_t *__restrict__ chunk_unit,
+    ChunkStat *__restrict__ ck, int32_t *__restrict__ pair_row, const LaunchState *__restrict__ ls,
+    uint32_t g_stats) {
+    if (blockIdx.x < g_stats)
+        chunk_stats_blocks(blockIdx.x, g_stats, oc_char, uo, chunk_unit, ck, ls);
+    else
+        match_small_block(blockIdx.x - g_stats, oc_char, oc_idx, uo, n_units, pair_row, ls);
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_chunk_scans(
+    UnitOut *__restrict__ uo, uint32_t n_units, ChunkStat *__restrict__ ck,
+    int32_t *__restrict__ gmin, int32_t *__restrict__ gmin1, int32_t *__restrict__ gmin2,
+    DepthAgg *__restrict__ gagg, int32_t *__restrict__ gd0, const uint32_t *__restrict__ big_units,
+    const LaunchState *__restrict__ ls, uint32_t g_big) {
+    if (blockIdx.x < g_big)
+        chunk_scan_blocks(blockIdx.x, g_big, uo, ck, gmin, gmin1, gmin2, gagg, gd0, big_units, ls);
+    else
+        chunk_scan_small_block(blockIdx.x - g_big, uo, n_units, ck, gmin, gmin1, gmin2, ls);
+}
+
 hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, uint32_t chunk_bound,
                         uint32_t group_bound, int32_t *pair_row) {
     (void)group_bound;
@@ -665,14 +692,11 @@ hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, ui
     const uint32_t g = chunk_bound < 2048u ? (chunk_bound ? chunk_bound : 1u) : 2048u;
     const uint32_t gc = chunk_bound < 8192u ? (chunk_bound ? chunk_bound : 1u) : 8192u;
     const uint32_t gb = n_units < 256u ? n_units : 256u;
-    hipLaunchKernelGGL(k_match_small, dim3((n_units + 3) / 4), dim3(256), 0, s, b.oc_char,
-                       b.oc_idx, b.uout, n_units, pair_row, b.ls);
-    hipLaunchKernelGGL(k_chunk_stats, dim3(g), dim3(kMatchThreads), 0, s, b.oc_char, b.uout,
-                       b.chunk_unit, b.ck, b.ls);
-    hipLaunchKernelGGL(k_chunk_scan_small, dim3((n_units + 3) / 4), dim3(256), 0, s, b.uout,
-                       n_units, b.ck, b.gmin, b.gmin1, b.gmin2, b.ls);
-    hipLaunchKernelGGL(k_chunk_scan, dim3(gb), dim3(kScanThreads), 0, s, b.uout, b.ck, b.gmin,
-                       b.gmin1, b.gmin2, b.gagg, b.gd0, b.big_units, b.ls);
+    hipLaunchKernelGGL(k_match_front, dim3(g + (n_units + 3) / 4), dim3(kMatchThreads), 0, s,
+                       b.oc_char, b.oc_idx, b.uout, n_units, b.chunk_unit, b.ck, pair_row, b.ls, g);
+    hipLaunchKernelGGL(k_chunk_scans, dim3(gb + (n_units + kScanThreads / 64 - 1) / (kScanThreads / 64)),
+                       dim3(kScanThreads), 0, s, b.uout, n_units, b.ck, b.gmin, b.gmin1, b.gmin2,
+                       b.gagg, b.gd0, b.big_units, b.ls, gb);
     hipLaunchKernelGGL(k_match_local, dim3(g), dim3(kMatchThreads), 0, s, b.oc_char, b.oc_idx,
                        b.uout, b.chunk_unit, b.ck, b.lows, b.highs, pair_row, b.ls);
     hipLaunchKernelGGL(k_match_cross, dim3(gc), dim3(64), 0, s, b.oc_char, b.oc_idx, b.uout,
