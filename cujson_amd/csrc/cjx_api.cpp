@@ -81,7 +81,9 @@ void *pinned_get(size_t bytes) {
         }
     }
     void *p = nullptr;
-    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+    // coherent: the chunked host parse writes structural rows into result
+    // buffers from a kernel
+    if (hipHostMalloc(&p, bytes, hipHostMallocCoherent) != hipSuccess) return nullptr;
     std::lock_guard<std::mutex> g(P.mu);
     P.cap[p] = bytes;
     return p;
@@ -317,23 +319,39 @@ int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsign
 int ensure_copy_stream(cjx_context *ctx) {
     if (ctx->copy_stream) return CJX_OK;
     if (hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->idle_ev, hipEventDisableTiming) != hipSuccess)
+        hipStreamCreateWithFlags(&ctx->out_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->idle_ev, hipEventDisableTiming) != hipSuccess ||
+        !ctx->h_part.ensure(cjx_context::kCopyEvents * sizeof(uint64_t)))
         return CJX_ERR_DEVICE;
     for (hipEvent_t &e : ctx->copy_ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return CJX_ERR_DEVICE;
+    for (hipEvent_t &e : ctx->part_ev)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return CJX_ERR_DEVICE;
     for (hipEvent_t &e : ctx->load_ev)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return CJX_ERR_DEVICE;
     return CJX_OK;
 }
 
+// Structural rows streamed to a host result buffer during a chunked parse.
+struct RowStream {
+    int32_t *buf = nullptr;  // pinned, host layout: row i of the unit at buf[1 + i]
+    int32_t *dev = nullptr;  // buf's address on the device
+    uint64_t cap_rows = 0;   // rows the buffer's layout holds (2 (cap_rows + 2) cells)
+    uint64_t copied = 0;     // rows [0, copied) are queued on the output stream
+    bool ok = true;          // false: the result is copied after the parse instead
+};
+
 // A single large unit whose bytes are still in host memory: the input is
 // copied in parts of `part` bytes (whole tiles) on the copy stream, and the
 // tokenizer runs as one launch per part, each waiting for its part's copy,
 // so phases 1-2 overlap the transfer.  Tile tickets continue across the
 // launches, which run in order on the context stream, so the tiles and the
-// look-back are exactly those of one launch.
+// look-back are exactly those of one launch.  With `counts`, each part's
+// row count (the inclusive count in its last tile's status) is copied to
+// ctx->h_part after the part's launch and ctx->part_ev[k] marks it.
 int enqueue_parse_chunked(cjx_context *ctx, const uint8_t *host, uint64_t n, const Shape &sh,
-                          unsigned flags, uint32_t *d_struct, int32_t *d_pair, uint64_t part) {
+                          unsigned flags, uint32_t *d_struct, int32_t *d_pair, uint64_t part,
+                          bool counts) {
     const bool u16 = (flags & CJX_FLAG_DEPTH_U16) != 0;
     if (u16 && !ctx->tmp.ensure(sh.rows * 4 + 16)) return CJX_ERR_ALLOC;
     if (int rc = ensure_copy_stream(ctx)) return rc;
@@ -366,6 +384,11 @@ int enqueue_parse_chunked(cjx_context *ctx, const uint8_t *host, uint64_t n, con
             k + 1 == parts ? sh.ntiles : (uint32_t)std::min<uint64_t>(sh.ntiles, (k + 1) * part / kTileBytes);
         if (launch_tokenize_tiles(s, b, d_in, upto - launched, jsonl, !strict, d_struct) !=
             hipSuccess)
+            return CJX_ERR_DEVICE;
+        if (counts && (hipMemcpyAsync(static_cast<uint64_t *>(ctx->h_part.p) + k,
+                                      b.tile_st + 2 * (uint64_t)(upto - 1), 8,
+                                      hipMemcpyDeviceToHost, s) != hipSuccess ||
+                       hipEventRecord(ctx->part_ev[k], s) != hipSuccess))
             return CJX_ERR_DEVICE;
         launched = upto;
     }
@@ -474,9 +497,56 @@ uint64_t chunked_part(const std::vector<UnitDesc> &units) {
     return part;
 }
 
+// While the parts of a chunked parse come in, copies each tokenized part's
+// structural rows to the host result buffer (k_rows_out on the output
+// stream), so the copy out overlaps the copy in.  The buffer is sized after the first part
+// from its row density (25 % and 64 Ki rows of margin, at most the unit's
+// row region); rows beyond it stop the streaming and the result is copied
+// after the parse instead.  Tiles publish inclusive counts of the rows
+// before their end, so a part's count is the row where the next part starts.
+void stream_rows(cjx_context *ctx, RowStream &rs, uint64_t n, uint64_t part, uint32_t out_cap,
+                 const uint32_t *d_struct) {
+    const uint64_t parts = (n + part - 1) / part;
+    const uint64_t *cnt = static_cast<const uint64_t *>(ctx->h_part.p);
+    for (uint64_t k = 0; k < parts && rs.ok; k++) {
+        if (hipEventSynchronize(ctx->part_ev[k]) != hipSuccess) {
+            rs.ok = false;
+            break;
+        }
+        const uint64_t c = std::min<uint64_t>(cnt[k] & kStCountMask, out_cap);
+        if (k == 0) {
+            const double rho = (double)c / (double)std::min(part, n);
+            const uint64_t est = std::min<uint64_t>(
+                (uint64_t)(rho * 1.25 * (double)n) + (64u << 10), out_cap);
+            rs.buf = static_cast<int32_t *>(pinned_get(2 * (est + 2) * sizeof(int32_t)));
+            rs.cap_rows = est;
+            void *dv = nullptr;
+            if (!rs.buf || hipHostGetDevicePointer(&dv, rs.buf, 0) != hipSuccess)
+                rs.ok = false;
+            rs.dev = static_cast<int32_t *>(dv);
+        }
+        if (!rs.ok || c > rs.cap_rows) {
+            rs.ok = false;
+            break;
+        }
+        // a kernel writes the rows, so the copy out does not queue behind the
+        // input's DMA copies
+        if (c > rs.copied &&
+            (hipStreamWaitEvent(ctx->out_stream, ctx->part_ev[k], 0) != hipSuccess ||
+             launch_rows_out(ctx->out_stream, d_struct + rs.copied,
+                             reinterpret_cast<uint32_t *>(rs.dev + 1 + rs.copied),
+                             c - rs.copied) != hipSuccess)) {
+            rs.ok = false;
+            break;
+        }
+        rs.copied = std::max(rs.copied, c);
+    }
+}
+
 int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint64_t> &offs,
                    const std::vector<uint64_t> &sizes, const std::vector<uint64_t> &pos_base,
-                   unsigned flags, bool concat, std::vector<UnitDesc> &units, float *ms) {
+                   unsigned flags, bool concat, std::vector<UnitDesc> &units, float *ms,
+                   RowStream *rs = nullptr) {
     int rc = set_device(ctx);
     if (rc) return rc;
     uint64_t dev_bytes = 0;
@@ -518,13 +588,18 @@ int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint
         ctx->want_nl = records;  // the tokenizer also writes the newline flags
         if ((rc = upload_units(ctx, units, ntiles, n_small))) return rc;
         (void)hipEventRecord(ctx->ev0, s);
-        if (part && attempt == 0)  // device_ms then includes waiting for the copies
+        if (part && attempt == 0) {  // device_ms then includes waiting for the copies
             rc = enqueue_parse_chunked(ctx, data + offs[0], sizes[0], sh, flags,
                                        ctx->structural.as<uint32_t>(), ctx->pair.as<int32_t>(),
-                                       part);
-        else
+                                       part, rs != nullptr);
+            if (!rc && rs)
+                stream_rows(ctx, *rs, sizes[0], part, units[0].out_cap,
+                            ctx->structural.as<uint32_t>());
+        } else {
+            if (rs) rs->ok = false;  // a re-run rewrites the rows
             rc = enqueue_parse(ctx, ctx->in.as<uint8_t>(), sh, flags,
                                ctx->structural.as<uint32_t>(), ctx->pair.as<int32_t>());
+        }
         ctx->want_nl = false;
         if (rc) return rc;
         (void)hipEventRecord(ctx->ev1, s);
@@ -577,8 +652,19 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
                      unsigned flags, cjx_result *out) {
     std::vector<UnitDesc> units;
     float ms = 0.f;
-    int rc = run_host_units(ctx, data, offs, sizes, pos_base, flags, sizes.size() > 1, units, &ms);
-    if (rc) return rc;
+    RowStream rs;
+    int rc = run_host_units(ctx, data, offs, sizes, pos_base, flags, sizes.size() > 1, units, &ms,
+                            &rs);
+    auto drop_stream = [&] {  // rows still in flight to rs.buf land before it is reused
+        if (!rs.buf) return;
+        (void)hipStreamSynchronize(ctx->out_stream);
+        pinned_put(rs.buf);
+        rs.buf = nullptr;
+    };
+    if (rc) {
+        drop_stream();
+        return rc;
+    }
     hipStream_t s = ctx->stream;
     const UnitOut *uo = static_cast<const UnitOut *>(ctx->h_uout.p);
     const LaunchState *ls = static_cast<const LaunchState *>(ctx->h_ls.p);
@@ -598,12 +684,26 @@ int parse_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<ui
         }
     }
     // Host layout, parse_standard_json.cu:1659-1680 / parse_json_lines.cu:1070-1089.
-    int32_t *buf = nullptr;
     const size_t cells = 2 * (N + 2);
-    if (!(buf = static_cast<int32_t *>(pinned_get(cells * sizeof(int32_t)))))
-        return CJX_ERR_ALLOC;
+    // every structural row already queued for rs.buf, which holds the layout
+    const bool streamed = rs.buf && rs.ok && units.size() == 1 && rs.copied == N &&
+                          N <= rs.cap_rows;
+    int32_t *buf = nullptr;
+    if (streamed) {
+        buf = rs.buf;
+        rs.buf = nullptr;
+    } else {
+        drop_stream();
+        if (!(buf = static_cast<int32_t *>(pinned_get(cells * sizeof(int32_t)))))
+            return CJX_ERR_ALLOC;
+    }
     bool ok = true;
-    if (units.size() == 1) {  // one region starting at row 0: copy it as is
+    if (streamed) {  // the pair values remain
+        if (N)
+            ok = hipMemcpyAsync(buf + N + 2, ctx->pair.p, N * 4, hipMemcpyDeviceToHost, s) ==
+                 hipSuccess;
+        if (hipStreamSynchronize(ctx->out_stream) != hipSuccess) ok = false;
+    } else if (units.size() == 1) {  // one region starting at row 0: copy it as is
         if (N)
             ok = hipMemcpyAsync(buf + 1, ctx->structural.p, N * 4, hipMemcpyDeviceToHost, s) ==
                      hipSuccess &&
@@ -774,8 +874,15 @@ void cjx_context_destroy(cjx_context *ctx) {
         (void)hipStreamSynchronize(ctx->copy_stream);
         (void)hipStreamDestroy(ctx->copy_stream);
     }
+    if (ctx->out_stream) {
+        (void)hipStreamSynchronize(ctx->out_stream);
+        (void)hipStreamDestroy(ctx->out_stream);
+    }
     for (hipEvent_t e : ctx->copy_ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->part_ev)
+        if (e) (void)hipEventDestroy(e);
+    ctx->h_part.release();
     for (int i = 0; i < cjx_context::kLoadBufs; i++) {
         if (ctx->load_ev[i]) (void)hipEventDestroy(ctx->load_ev[i]);
         ctx->load_buf[i].release();

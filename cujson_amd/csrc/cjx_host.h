@@ -100,6 +100,12 @@ struct cjx_context {
     hipStream_t copy_stream = nullptr;
     hipEvent_t copy_ev[kCopyEvents] = {};
     hipEvent_t idle_ev = nullptr;
+    // Structural rows of a chunked host parse copied out while later parts
+    // are still coming in: each part's row count (its last tile's status)
+    // lands in h_part, and the rows go out on out_stream.
+    hipStream_t out_stream = nullptr;
+    hipEvent_t part_ev[kCopyEvents] = {};
+    PinnedBuf h_part;
     // Newline scan of device text: segment counts, bases and status; the
     // newline offsets and gathered line starts of the device chunker.
     DevBuf nl_seg, nl_off, nl_idx;

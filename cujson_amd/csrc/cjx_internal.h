@@ -106,6 +106,9 @@ struct RecordOut {
 
 // Look-back status encoding (two 64-bit words per tile, see cjx_tokenize.hip).
 constexpr uint32_t kStInvalid = 0, kStAgg = 1, kStInc = 2;
+// Word A of an inclusive status: 2 << 62 | exit state << 60 | the unit's
+// structural count up to the tile's end (the row where the next tile starts).
+constexpr unsigned long long kStCountMask = (1ull << 60) - 1;
 
 }  // namespace cjx
 
@@ -143,6 +146,7 @@ struct LaunchBufs {
 };
 
 hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uint32_t n_units);
+hipError_t launch_rows_out(hipStream_t s, const uint32_t *src, uint32_t *host_dst, uint64_t count);
 // Units of at most kWaveBytes bytes are "small": no tiles, one wave each
 // (listed in small_units); the others are cut into kTileBytes tiles.
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,

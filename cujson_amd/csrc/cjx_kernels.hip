@@ -702,6 +702,43 @@ hipError_t launch_sort_units(hipStream_t s, const LaunchBufs &b, uint32_t n_unit
     return hipGetLastError();
 }
 
+// k_rows_out: copies rows src[0, count) into host memory dst (a coherent
+// pinned buffer, written over the host link by the device).  dst need only
+// be 4-byte aligned: rows before its first 16-byte boundary and after its
+// last one go one by one, the rest as 16-byte stores, so each wave writes
+// whole host cache lines.  It moves results out while the DMA engines carry
+// input in.
+__global__ __launch_bounds__(256) void k_rows_out(const uint32_t *__restrict__ src,
+                                                  uint32_t *__restrict__ dst, uint64_t count) {
+    const uint64_t tid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * 256u;
+    const uint64_t lead = (uint64_t)((16u - ((uintptr_t)dst & 15u)) & 15u) / 4u;
+    const uint64_t head = lead < count ? lead : count;
+    const uint64_t body = (count - head) / 4u;
+    if (tid < head) dst[tid] = src[tid];
+    const uint32_t *s = src + head;
+    uint4 *d4 = reinterpret_cast<uint4 *>(dst + head);
+    for (uint64_t i = tid; i < body; i += stride) {
+        uint4 v;
+        v.x = s[4 * i];
+        v.y = s[4 * i + 1];
+        v.z = s[4 * i + 2];
+        v.w = s[4 * i + 3];
+        d4[i] = v;
+    }
+    const uint64_t tail = head + 4 * body;
+    if (tid < count - tail) dst[tail + tid] = src[tail + tid];
+}
+
+hipError_t launch_rows_out(hipStream_t s, const uint32_t *src, uint32_t *host_dst,
+                           uint64_t count) {
+    if (count == 0) return hipSuccess;
+    uint64_t grid = (count / 4 + 255) / 256;
+    grid = grid < 1 ? 1 : (grid > 128 ? 128 : grid);
+    hipLaunchKernelGGL(k_rows_out, dim3((uint32_t)grid), dim3(256), 0, s, src, host_dst, count);
+    return hipGetLastError();
+}
+
 hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uint32_t n_units) {
     const uint32_t work = 2 * ntiles > n_units ? 2 * ntiles : n_units;
     uint32_t grid = (work + 255) / 256;

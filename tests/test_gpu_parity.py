@@ -220,6 +220,25 @@ def test_chunked_host_input_fault(ctx, doc96):
     check_standard(ctx, d, label="chunked with a fault")
 
 
+def test_chunked_host_sparse_head(ctx):
+    """The structural rows of a chunked host parse are copied out part by
+    part into a buffer sized from the first part's density.  Here the first
+    part lies inside one 20 MiB string, so the rest outgrows that buffer and
+    the result is copied after the parse instead: same output."""
+    body = b'{"key":12345},' * ((52 << 20) // 14)
+    d = np.frombuffer(b'["' + b"a" * (20 << 20) + b'",' + body + b"0]", np.uint8)
+    assert d.size >= 64 << 20
+    check_standard(ctx, d, label="sparse head")
+
+
+def test_chunked_host_dense_rerun(ctx):
+    """More than one structural per three bytes overflows the first
+    attempt's row region: the parse re-runs with full regions and the rows
+    streamed during the first attempt are not used."""
+    d = np.frombuffer(b"[" + b"[]," * ((66 << 20) // 3) + b"0]", np.uint8)
+    check_standard(ctx, d, label="dense re-run")
+
+
 def test_jsonl_generated_records(ctx):
     d = gen.lines(3000)
     data = bytes(d)
