@@ -482,10 +482,42 @@ CJX_HD uint32_t put_utf8(uint8_t *out, uint32_t cp) {
 // null: lengths only); returns false for an invalid body (a raw byte below
 // 0x20, an unknown escape, an unescaped quote).  The output is never longer
 // than the body.
+typedef uint32_t __attribute__((may_alias)) word_alias;
+
+// True if a byte of w is below 0x20, a quote or a backslash (exact as a
+// whole: each test is the zero-byte test on w, w ^ 0x22.. or w ^ 0x5C..,
+// and the below-0x20 test of hasless, all exact for any set byte).
+CJX_HD bool word_special(uint32_t w) {
+    const uint32_t q = w ^ 0x22222222u, b = w ^ 0x5C5C5C5Cu;
+    const uint32_t lo = (w - 0x20202020u) & ~w;
+    const uint32_t zq = (q - 0x01010101u) & ~q, zb = (b - 0x01010101u) & ~b;
+    return ((lo | zq | zb) & 0x80808080u) != 0;
+}
+
 CJX_HD bool unescape(const uint8_t *s, uint32_t len, uint8_t *out, uint32_t &out_len,
                      uint8_t &flags) {
-    uint32_t o = 0;
-    for (uint32_t i = 0; i < len;) {
+    uint32_t o = 0, i0 = 0;
+    // Up to the first escape the output offset equals the input offset, so
+    // when both pointers share their alignment mod 4, whole 4-byte words
+    // without a special byte are copied at once.
+    if (out && ((uintptr_t)s & 3u) == ((uintptr_t)out & 3u)) {
+        uint32_t i = 0;
+        while (i < len && ((uintptr_t)(s + i) & 3u)) {
+            const uint8_t c = s[i];
+            if (c < 0x20 || c == '"' || c == '\\') break;
+            out[i] = c;
+            i++;
+        }
+        if (i < len && !((uintptr_t)(s + i) & 3u)) {
+            for (; i + 4 <= len; i += 4) {
+                const uint32_t w = *reinterpret_cast<const word_alias *>(s + i);
+                if (word_special(w)) break;
+                *reinterpret_cast<word_alias *>(out + i) = w;
+            }
+        }
+        i0 = o = i;
+    }
+    for (uint32_t i = i0; i < len;) {
         const uint8_t c = s[i];
         if (c < 0x20 || c == '"') return false;
         if (c != '\\') {

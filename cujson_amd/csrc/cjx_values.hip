@@ -13,7 +13,9 @@
 //   k_values_exact  those numbers, grid-stride over the list
 // String bodies are unescaped into `strings` at the offset of their raw
 // body, which is never shorter, so no scan is needed: strings[text_off ..
-// text_off + length) holds the UTF-8 of a key or string.
+// text_off + length) holds the UTF-8 of a key or string.  A block of
+// k_values first copies its text span into LDS (coalesced 16-byte loads)
+// when it fits.
 #include "cjx_device.h"
 #include "cjx_values.h"
 
@@ -26,7 +28,9 @@ struct GapText {
     uint8_t next;    // the structural character after the gap (0 after the last)
 };
 
-__device__ __forceinline__ GapText gap_of(const uint8_t *__restrict__ in, uint64_t n,
+// Byte x of the text is src[x - off]: src is the text itself (off = 0) or
+// the block's LDS copy of the span that starts at text byte off.
+__device__ __forceinline__ GapText gap_of(const uint8_t *src, uint64_t off, uint64_t n,
                                           const int32_t *__restrict__ st, uint64_t N,
                                           uint64_t h) {
     // st[i] is the 1-based position of structural i + 1 (host index); rows
@@ -36,9 +40,9 @@ __device__ __forceinline__ GapText gap_of(const uint8_t *__restrict__ in, uint64
     if (b > n) b = n;
     if (a > b) a = b;
     GapText g;
-    g.next = (h < N && b < n) ? in[b] : 0;
-    while (a < b && is_ws(in[a])) a++;
-    while (b > a && is_ws(in[b - 1])) b--;
+    g.next = (h < N && b < n) ? src[b - off] : 0;
+    while (a < b && is_ws(src[a - off])) a++;
+    while (b > a && is_ws(src[b - 1 - off])) b--;
     g.a = (uint32_t)a;
     g.b = (uint32_t)b;
     return g;
@@ -54,15 +58,53 @@ __device__ __forceinline__ bool literal(const uint8_t *s, uint32_t len, const ch
 
 }  // namespace
 
+// Text bytes a block of k_values stages in LDS: the span from its first
+// gap's start to the structural character after its last gap, when that
+// fits and the text is 16-byte aligned (else the block reads the text in
+// place).  The decoders then walk LDS instead of waiting on a global load
+// per byte.
+#ifndef CJX_VAL_STAGE
+#define CJX_VAL_STAGE 16384
+#endif
+constexpr uint32_t kValStage = CJX_VAL_STAGE;
+
 __global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, uint64_t n,
                                                 const int32_t *__restrict__ st, uint64_t N,
                                                 ValueOut *__restrict__ out,
                                                 uint8_t *__restrict__ strings,
                                                 uint32_t *__restrict__ exact_list,
                                                 uint32_t *__restrict__ exact_count) {
-    const uint64_t h = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint64_t h0 = (uint64_t)blockIdx.x * 256u;
+    const uint64_t h = h0 + threadIdx.x;
+    const uint8_t *src = in;
+    uint64_t off = 0;
+#if CJX_VAL_STAGE > 0
+    __shared__ uint4 stage[kValStage / 16];
+    {
+        const uint64_t hl = h0 + 255 < N ? h0 + 255 : N;
+        const uint64_t A = h0 == 0 ? 0 : (uint64_t)(uint32_t)st[h0 - 1];
+        uint64_t B = hl == N ? n : (uint64_t)(uint32_t)st[hl];  // through the next structural
+        if (B > n) B = n;
+        const uint64_t a0 = (A < B ? A : B) & ~15ull;
+        if (((uintptr_t)in & 15u) == 0 && B - a0 <= kValStage) {  // uniform over the block
+            const uint32_t words = (uint32_t)((B - a0 + 15) / 16);
+            for (uint32_t w = threadIdx.x; w < words; w += 256) {
+                const uint64_t p = a0 + 16ull * w;
+                if (p + 16 <= n) {
+                    stage[w] = *reinterpret_cast<const uint4 *>(in + p);
+                } else {  // the text's last partial word
+                    uint8_t *d = reinterpret_cast<uint8_t *>(&stage[w]);
+                    for (uint32_t k = 0; k < 16; k++) d[k] = p + k < n ? in[p + k] : 0;
+                }
+            }
+            __syncthreads();
+            src = reinterpret_cast<const uint8_t *>(stage);
+            off = a0;
+        }
+    }
+#endif
     if (h > N) return;
-    const GapText g = gap_of(in, n, st, N, h);
+    const GapText g = gap_of(src, off, n, st, N, h);
     ValueOut v;
     v.v = 0;
     v.row = (uint32_t)h;
@@ -71,7 +113,7 @@ __global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, 
     v.type = kValNone;
     v.flags = 0;
     v.reserved = 0;
-    const uint8_t *s = in + g.a;
+    const uint8_t *s = src + (g.a - off);
     const uint32_t len = g.b - g.a;
     if (len == 0) {
         // nothing between the two structural characters
@@ -118,7 +160,7 @@ __global__ __launch_bounds__(64) void k_values_exact(const uint8_t *__restrict__
     const uint32_t cnt = *exact_count;
     for (uint32_t i = blockIdx.x * 64u + threadIdx.x; i < cnt; i += gridDim.x * 64u) {
         const uint64_t h = exact_list[i];
-        const GapText g = gap_of(in, n, st, N, h);
+        const GapText g = gap_of(in, 0, n, st, N, h);
         NumText t;
         uint64_t m_lo = 0;
         int64_t e_ulp = 0, iv = 0;

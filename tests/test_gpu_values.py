@@ -193,6 +193,45 @@ def test_device_entry_and_exact_step(ctx):
     assert np.array_equal(got, table)
 
 
+def test_long_strings_and_unaligned_text(ctx):
+    """Blocks whose text span exceeds the LDS stage (strings of 20-60 KB)
+    read the text in place; device text at an odd address is never staged.
+    Both must give the same table and strings as the host form."""
+    rng = random.Random(22)
+    pieces = ["ab", "c ", "xyz", "\\\\", "\\n", "\\u00e9", "\u00e9"]
+    parts = []
+    for i in range(40):
+        body = "".join(rng.choice(pieces) for _ in range(rng.choice((3, 8000, 25000))))
+        parts.append(f'"k{i}": "{body}", "n{i}": {rng.random() * 1e6!r}, "b{i}": true')
+    data = ("{" + ", ".join(parts) + "}").encode()
+    table = run_standard(ctx, data, "long strings")
+    n = len(data)
+    dev = DeviceBuffer(n + 64)
+    dev.upload(np.frombuffer(data, np.uint8))
+    st, pr = DeviceBuffer(n * 4), DeviceBuffer(n * 4)
+    assert ctx.parse_device(dev.ptr, n, st.ptr, pr.ptr, n) == 0
+    rc, N, M, over = ctx.wait()
+    assert rc == 0 and not over and N + 1 == table.size
+    for shift in (0, 1):
+        text = DeviceBuffer(n + 64)
+        buf = np.zeros(n + shift, np.uint8)
+        buf[shift:] = np.frombuffer(data, np.uint8)
+        text.upload(buf)
+        vals = DeviceBuffer((N + 1) * cj.VALUE_DTYPE.itemsize)
+        strs = DeviceBuffer(n)
+        rc, _ = ctx.values_device(text.ptr + shift, n, st.ptr, N, vals.ptr, strs.ptr)
+        assert rc == 0
+        got = np.frombuffer(vals.download(np.uint8, (N + 1) * cj.VALUE_DTYPE.itemsize).tobytes(),
+                            dtype=cj.VALUE_DTYPE)
+        assert np.array_equal(got, table), shift
+        sb = strs.download(np.uint8, n).tobytes()
+        for e in table:
+            if int(e["type"]) in (cj.CJX_VALUE_KEY, cj.CJX_VALUE_STRING):
+                o, k = int(e["text_off"]), int(e["v"])
+                want = json.loads(b'"' + data[o:o + int(e["text_len"])] + b'"').encode()
+                assert sb[o:o + k] == want, (shift, o)
+
+
 def test_bad_arguments(ctx):
     rc, _ = ctx.values_device(0, 10, 0, 0, 0)
     assert rc == cj.CJX_ERR_INPUT

@@ -175,3 +175,44 @@ def test_unescape_against_json():
         elif not ok or out.raw[:n.value] != want or fl.value != want_flags:
             bad.append((body, ok, out.raw[:n.value], fl.value, want, want_flags))
     assert not bad, bad[:5]
+
+
+def test_unescape_word_copy_alignments():
+    """Bodies and outputs at the same offset mod 4 take the word-at-a-time
+    copy up to the first special byte; every offset, with the special byte
+    (escape, quote, control byte) at every position of a word."""
+    f = unescape_fn()
+    f.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                  ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_uint8)]
+    rng = random.Random(15)
+    plain = "abcdefghij klmnopqrstuvwxyz0123456789é中"
+    bad = []
+    for body_len in range(0, 40):
+        chars = [rng.choice(plain) for _ in range(body_len)]
+        cases = ["".join(chars).encode("utf-8")]
+        for pos in range(0, body_len + 1):  # specials between characters
+            head, tail = "".join(chars[:pos]).encode(), "".join(chars[pos:]).encode()
+            for sp in (b"\\n", b"\\u00e9", b'"', b"\x01", b"\\", b"\\q"):
+                cases.append(head + sp + tail)
+        for body in cases:
+            try:
+                s = json.loads(b'"' + body + b'"')
+                want = s.encode("utf-8", "surrogatepass")
+            except ValueError:
+                want = None
+            for k in range(4):
+                src = ctypes.create_string_buffer(len(body) + 16)
+                dst = ctypes.create_string_buffer(len(body) + 16)
+                sa, da = ctypes.addressof(src), ctypes.addressof(dst)
+                so = (k - sa) % 4  # body at address = k mod 4, output likewise
+                do = (k - da) % 4
+                ctypes.memmove(sa + so, body, len(body))
+                n, fl = ctypes.c_size_t(), ctypes.c_uint8()
+                ok = f(sa + so, len(body), da + do, ctypes.byref(n), ctypes.byref(fl))
+                got = dst.raw[do:do + n.value]
+                if want is None:
+                    if ok:
+                        bad.append((body, k, "accepted"))
+                elif not ok or got != want:
+                    bad.append((body, k, ok, got, want))
+    assert not bad, bad[:5]
