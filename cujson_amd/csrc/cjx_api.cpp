@@ -81,9 +81,7 @@ void *pinned_get(size_t bytes) {
         }
     }
     void *p = nullptr;
-    // coherent: the chunked host parse writes structural rows into result
-    // buffers from a kernel
-    if (hipHostMalloc(&p, bytes, hipHostMallocCoherent) != hipSuccess) return nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
     std::lock_guard<std::mutex> g(P.mu);
     P.cap[p] = bytes;
     return p;
@@ -319,23 +317,36 @@ int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsign
 int ensure_copy_stream(cjx_context *ctx) {
     if (ctx->copy_stream) return CJX_OK;
     if (hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&ctx->out_stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->idle_ev, hipEventDisableTiming) != hipSuccess ||
-        !ctx->h_part.ensure(cjx_context::kCopyEvents * sizeof(uint64_t)))
+        hipEventCreateWithFlags(&ctx->idle_ev, hipEventDisableTiming) != hipSuccess)
         return CJX_ERR_DEVICE;
     for (hipEvent_t &e : ctx->copy_ev)
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return CJX_ERR_DEVICE;
-    for (hipEvent_t &e : ctx->part_ev)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return CJX_ERR_DEVICE;
     for (hipEvent_t &e : ctx->load_ev)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return CJX_ERR_DEVICE;
     return CJX_OK;
 }
 
+// The output stream of the streamed copy out, its part events and the
+// pinned part counts (created on first use: a context that never streams
+// rows holds no third stream, and so takes no third hardware queue).
+int ensure_out_stream(cjx_context *ctx) {
+    if (ctx->out_stream) return CJX_OK;
+    if (!ctx->h_part.ensure(cjx_context::kCopyEvents * sizeof(uint64_t))) return CJX_ERR_ALLOC;
+    for (hipEvent_t &e : ctx->part_ev)
+        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return CJX_ERR_DEVICE;
+    if (hipStreamCreateWithFlags(&ctx->out_stream, hipStreamNonBlocking) != hipSuccess) {
+        ctx->out_stream = nullptr;
+        return CJX_ERR_DEVICE;
+    }
+    return CJX_OK;
+}
+
 // Structural rows streamed to a host result buffer during a chunked parse.
 struct RowStream {
     int32_t *buf = nullptr;  // pinned, host layout: row i of the unit at buf[1 + i]
-    int32_t *dev = nullptr;  // buf's address on the device
+    int32_t *dev = nullptr;  // buf's address on the device (k_rows_out writes through it;
+                             // the host reads after the output stream is synchronized)
     uint64_t cap_rows = 0;   // rows the buffer's layout holds (2 (cap_rows + 2) cells)
     uint64_t copied = 0;     // rows [0, copied) are queued on the output stream
     bool ok = true;          // false: the result is copied after the parse instead
@@ -589,10 +600,12 @@ int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint
         if ((rc = upload_units(ctx, units, ntiles, n_small))) return rc;
         (void)hipEventRecord(ctx->ev0, s);
         if (part && attempt == 0) {  // device_ms then includes waiting for the copies
+            if (rs && ensure_out_stream(ctx) != CJX_OK) rs->ok = false;
+            const bool counts = rs && rs->ok;
             rc = enqueue_parse_chunked(ctx, data + offs[0], sizes[0], sh, flags,
                                        ctx->structural.as<uint32_t>(), ctx->pair.as<int32_t>(),
-                                       part, rs != nullptr);
-            if (!rc && rs)
+                                       part, counts);
+            if (!rc && counts)
                 stream_rows(ctx, *rs, sizes[0], part, units[0].out_cap,
                             ctx->structural.as<uint32_t>());
         } else {

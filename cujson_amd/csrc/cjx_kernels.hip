@@ -702,8 +702,9 @@ hipError_t launch_sort_units(hipStream_t s, const LaunchBufs &b, uint32_t n_unit
     return hipGetLastError();
 }
 
-// k_rows_out: copies rows src[0, count) into host memory dst (a coherent
-// pinned buffer, written over the host link by the device).  dst need only
+// k_rows_out: copies rows src[0, count) into host memory dst (a pinned
+// buffer, written over the host link by the device; the host reads it after
+// synchronizing with the stream).  dst need only
 // be 4-byte aligned: rows before its first 16-byte boundary and after its
 // last one go one by one, the rest as 16-byte stores, so each wave writes
 // whole host cache lines.  It moves results out while the DMA engines carry

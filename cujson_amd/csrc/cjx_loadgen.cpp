@@ -272,6 +272,10 @@ int cjx_loadgen_run(const char *args, char *out, size_t cap) {
         d.sum = r.buf ? cell_sum(r.buf, r.n_structural) : 0;
         cjx_free_result(&r);
     }
+    // The server's slot streams should have the hardware queues to
+    // themselves: no other context lives during the timed run.
+    cjx_context_destroy(ctx);
+    ctx = nullptr;
     // Arrival schedule.
     std::mt19937_64 rng(c.seed ^ 0x5DEECE66Dull);
     std::exponential_distribution<double> gap(c.rate / 1000.0);  // in ms
@@ -335,6 +339,10 @@ int cjx_loadgen_run(const char *args, char *out, size_t cap) {
     std::unordered_map<uint64_t, size_t> req_of;
     req_of.reserve(n_req * 2);
     std::atomic<size_t> mismatches{0}, submitted{0};
+    // mismatches by kind: another error code (and the last such code), another
+    // structural count, another cell sum
+    std::atomic<size_t> mis_code{0}, mis_count{0}, mis_sum{0};
+    std::atomic<int> last_code{0};
     std::atomic<bool> submit_failed{false};
     const auto t0 = clk::now();
     double last_done = 0;
@@ -373,9 +381,18 @@ int cjx_loadgen_run(const char *args, char *out, size_t cap) {
             }
             const Doc &d = pool[(size_t)doc_of[i]];
             bool ok = cc.error == d.error;
-            if (ok && d.error == CJX_OK)
-                ok = cc.n_structural == d.n_structural && cc.buf &&
-                     cell_sum(cc.buf, cc.n_structural) == d.sum;
+            if (!ok) {
+                mis_code++;
+                last_code = cc.error;
+            } else if (d.error == CJX_OK) {
+                if (cc.n_structural != d.n_structural || !cc.buf) {
+                    ok = false;
+                    mis_count++;
+                } else if (cell_sum(cc.buf, cc.n_structural) != d.sum) {
+                    ok = false;
+                    mis_sum++;
+                }
+            }
             if (!ok) mismatches++;
             lat[i] = now - t_arr[i];
             srv_lat[i] = cc.latency_ms;
@@ -419,12 +436,14 @@ int cjx_loadgen_run(const char *args, char *out, size_t cap) {
     size_t plain_n = 0;
     double plain_bytes = 0;
     std::vector<double> plain_lat;
+    if (cjx_context_create(0, &ctx) != CJX_OK) ctx = nullptr;
     const auto tp = clk::now();
     for (size_t i = 0; i < n_req && ms_since(tp) < c.plain_seconds * 1000.0; i++) {
         const Doc &d = pool[(size_t)doc_of[i]];
         const auto ts = clk::now();
         cjx_result r;
         std::memset(&r, 0, sizeof(r));
+        if (!ctx) break;
         cjx_parse_standard(ctx, d.p, d.n, 0, &r);
         cjx_free_result(&r);
         plain_lat.push_back(ms_since(ts));
@@ -438,7 +457,9 @@ int cjx_loadgen_run(const char *args, char *out, size_t cap) {
     const size_t n_done = all.size();
     std::snprintf(
         out, cap,
-        "{\"requests\": %zu, \"completed\": %zu, \"mismatches\": %zu, \"submit_failed\": %s, "
+        "{\"requests\": %zu, \"completed\": %zu, \"mismatches\": %zu, "
+        "\"mismatch_kinds\": {\"error_code\": %zu, \"last_error_code\": %d, "
+        "\"structural_count\": %zu, \"cell_sum\": %zu}, \"submit_failed\": %s, "
         "\"rate_req_s\": %.1f, \"seconds\": %.1f, \"offered_GBps\": %.3f, "
         "\"sustained_GBps\": %.3f, \"completed_req_s\": %.1f, "
         "\"p50_ms\": %.4f, \"p99_ms\": %.4f, \"p999_ms\": %.4f, \"max_ms\": %.4f, "
@@ -451,7 +472,8 @@ int cjx_loadgen_run(const char *args, char *out, size_t cap) {
         "\"plain_p50_ms\": %.4f, \"plain_p99_ms\": %.4f, \"pool_docs\": %zu, "
         "\"pool_generate_s\": %.2f, \"budget_GB\": %.1f, \"slots\": %d, \"max_batch\": %zu, "
         "\"result_bytes_per_input_byte\": %.4f, \"link_ceiling_GBps\": %.3f}",
-        n_req, n_done, mismatches.load(), submit_failed ? "true" : "false", c.rate, c.seconds,
+        n_req, n_done, mismatches.load(), mis_code.load(), last_code.load(), mis_count.load(),
+        mis_sum.load(), submit_failed ? "true" : "false", c.rate, c.seconds,
         offered_bytes / (c.seconds * 1e9), span_ms > 0 ? bytes / (span_ms * 1e6) : 0.0,
         span_ms > 0 ? (double)n_done / (span_ms / 1000.0) : 0.0, pct(all, 0.5), pct(all, 0.99),
         pct(all, 0.999), all.empty() ? 0.0 : *std::max_element(all.begin(), all.end()),

@@ -24,8 +24,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
@@ -153,6 +155,16 @@ Region *acquire_region(cjx_server *srv, size_t bytes) {
     return r;
 }
 
+// Where the calling slot thread last met a failed HIP call (source line and
+// error), printed for failed batches when CJX_SERVER_TRACE is set.
+thread_local int t_fail_line = 0;
+thread_local hipError_t t_fail_err = hipSuccess;
+int dev_fail(int line, hipError_t e = hipErrorUnknown) {
+    t_fail_line = line;
+    t_fail_err = e != hipErrorUnknown ? e : hipGetLastError();
+    return CJX_ERR_DEVICE;
+}
+
 // Enqueues the batch of slot s (inputs, kernels, status copies) and records
 // its event.
 int launch_batch(Slot &s) {
@@ -187,7 +199,7 @@ int launch_batch(Slot &s) {
         !s.status.ensure(status_bytes + 64, hipHostMallocCoherent))
         return CJX_ERR_ALLOC;
     void *stage_dev = device_view(s.stage.p), *status_dev = device_view(s.status.p);
-    if (!stage_dev || !status_dev) return CJX_ERR_DEVICE;
+    if (!stage_dev || !status_dev) return dev_fail(__LINE__);
     // A small batch is gathered whole in the slot's pinned staging buffer
     // and pulled in by a copy kernel.  Otherwise small inputs are gathered and
     // copied in runs of consecutive staged documents, and large ones are
@@ -198,7 +210,7 @@ int launch_batch(Slot &s) {
     for (size_t i = 0; i <= k && zero_copy; i++) {
         if (i == k) {
             if (launch_copy(st, stage_dev, d_in, off, nullptr, nullptr, 0) != hipSuccess)
-                return CJX_ERR_DEVICE;
+                return dev_fail(__LINE__);
             break;
         }
         std::memcpy(stage + units[i].in_off, s.reqs[i].data, s.reqs[i].n);
@@ -208,7 +220,7 @@ int launch_batch(Slot &s) {
         if (!staged && hi > lo) {
             if (hipMemcpyAsync(d_in + lo, stage + lo, hi - lo, hipMemcpyHostToDevice, st) !=
                 hipSuccess)
-                return CJX_ERR_DEVICE;
+                return dev_fail(__LINE__);
             lo = hi = 0;
         }
         if (i == k) break;
@@ -219,18 +231,18 @@ int launch_batch(Slot &s) {
             hi = units[i].in_off + r.n;
         } else if (hipMemcpyAsync(d_in + units[i].in_off, r.data, r.n, hipMemcpyHostToDevice,
                                   st) != hipSuccess) {
-            return CJX_ERR_DEVICE;
+            return dev_fail(__LINE__);
         }
     }
     if ((rc = upload_units(ctx, units, (uint32_t)tiles, n_small))) return rc;
     if ((rc = enqueue_parse(ctx, d_in, s.sh, s.flags, ctx->structural.as<uint32_t>(),
                             ctx->pair.as<int32_t>())))
-        return rc;
+        return rc == CJX_ERR_DEVICE ? dev_fail(__LINE__) : rc;
     uint8_t *sd = static_cast<uint8_t *>(status_dev);
     if (launch_copy(st, ctx->uout.p, sd, k * sizeof(UnitOut), ctx->ls.p,
                     sd + k * sizeof(UnitOut), sizeof(LaunchState)) != hipSuccess ||
         hipEventRecord(s.ev, st) != hipSuccess)
-        return CJX_ERR_DEVICE;
+        return dev_fail(__LINE__);
     s.uo = static_cast<const UnitOut *>(s.status.p);
     s.ls = reinterpret_cast<const LaunchState *>(static_cast<uint8_t *>(s.status.p) +
                                                  k * sizeof(UnitOut));
@@ -266,13 +278,13 @@ int after_kernels(cjx_server *srv, Slot &s) {
     if (!s.ctx->layout.ensure(cells * 4 + 16)) return CJX_ERR_ALLOC;
     if (launch_assemble_docs(st, b, (uint32_t)k, d_st, d_pr, s.ctx->layout.as<int32_t>(),
                              (uint32_t)s.N) != hipSuccess)
-        return CJX_ERR_DEVICE;
+        return dev_fail(__LINE__);
     // small results: a copy kernel into the region; large ones: the copy engine
     const hipError_t e =
         cells * 4 <= kZeroCopyBytes
             ? launch_copy(st, s.ctx->layout.p, s.region->dp, cells * 4, nullptr, nullptr, 0)
             : hipMemcpyAsync(s.region->p, s.ctx->layout.p, cells * 4, hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess || hipEventRecord(s.ev, st) != hipSuccess) return CJX_ERR_DEVICE;
+    if (e != hipSuccess || hipEventRecord(s.ev, st) != hipSuccess) return dev_fail(__LINE__);
     s.state = Slot::kCopyOut;
     return CJX_OK;
 }
@@ -305,6 +317,11 @@ void finish_batch(cjx_server *srv, Slot &s) {
 }
 
 void fail_batch(cjx_server *srv, Slot &s, int rc) {
+    static std::atomic<int> traced{0};
+    static const bool trace = std::getenv("CJX_SERVER_TRACE") != nullptr;
+    if (trace && traced.fetch_add(1) < 20)
+        std::fprintf(stderr, "[cjx server] batch of %zu documents failed: code %d, line %d, %s\n",
+                     s.reqs.size(), rc, t_fail_line, hipGetErrorString(t_fail_err));
     std::lock_guard<std::mutex> g(srv->mu);
     for (const Request &r : s.reqs) {
         cjx_completion c;
@@ -449,8 +466,9 @@ void slot_main(cjx_server *srv, Slot *sp) {
         s.full_regions = false;
         int rc = launch_batch(s);
         while (!rc) {
-            if (wait_event(s) != hipSuccess) {
-                rc = CJX_ERR_DEVICE;
+            const hipError_t we = wait_event(s);
+            if (we != hipSuccess) {
+                rc = dev_fail(__LINE__, we);
                 break;
             }
             if (s.state == Slot::kKernels) {
