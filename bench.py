@@ -270,6 +270,7 @@ def lines_bench(cj, gen, n_records: int, reps: int, plain_compare: bool = True):
     plain_ms = e0.elapsed_ms(e1) if plain_compare else float("nan")
     n_chunks = n_chunks or sum(len(L["sizes"]) for L in launches)
     loader = lines_loader_bench(cj, ctx, data, offs, sizes, best, parts, launches)
+    host = lines_host_bench(cj, ctx, data, offs, sizes, parts[0])
     ctx.close()
     return {
         "workload": f"{n_records} JSON Lines records, log-normal sizes (median 400 B, sigma 1.2, "
@@ -286,7 +287,45 @@ def lines_bench(cj, gen, n_records: int, reps: int, plain_compare: bool = True):
         "plain_gbs": _num(total / (plain_ms * 1e-3) / 1e9, 2),
         "batched_vs_plain": _num(plain_ms / batched_ms, 2),
         "loader": loader,
+        "host_path": host,
     }
+
+
+def lines_host_bench(cj, ctx, data, offs, sizes, part, reps: int = 3,
+                     warmup_s: float = 1.5) -> dict:
+    """PCIe-inclusive JSON Lines path: cjx_parse_lines (parse_json_lines,
+    parse_json_lines.cu:1092-1264) on the first launch's chunks, from pinned
+    host memory (the loader's buffer), result in the reference's pinned host
+    layout.  The chunks lie back to back, so the library copies their span
+    once and places them on the device.  Calls run for warmup_s seconds (at
+    least two) before the timed ones, for the host link's speed to settle."""
+    from cujson_amd.hip import PinnedHostBuffer
+    a, b = part
+    lo = int(offs[a])
+    nbytes = int(offs[b - 1] + sizes[b - 1]) - lo
+    pin = PinnedHostBuffer(nbytes)
+    pin.array[:] = data[lo:lo + nbytes]
+    inp = cj.cuJSONLinesInput(pin.array, b - a, nbytes, (offs[a:b] - lo).tolist(),
+                              sizes[a:b].tolist())
+    times, start, warm = [], time.perf_counter(), None
+    while warm is None or len(times) < warm + reps:
+        t0 = time.perf_counter()
+        r = cj.parse_json_lines(inp, 0, ctx)
+        t1 = time.perf_counter()
+        if r.error != 0:
+            raise RuntimeError(f"lines host path: {cj.strerror(r.error)}")
+        times.append((t1 - t0) * 1e3)
+        del r
+        if warm is None and len(times) >= 2 and t1 - start >= warmup_s:
+            warm = len(times)
+    del inp
+    pin.free()
+    ms = float(np.median(times[warm:]))
+    return {"bytes": nbytes, "chunks": b - a, "ms": round(ms, 2),
+            "gbs": round(nbytes / (ms * 1e-3) / 1e9, 3), "first_call_ms": round(times[0], 2),
+            "form": "pinned host input, chunks' span copied once and placed on the device; "
+                    "result packed on the device and copied into the pinned host layout; "
+                    "wall clock"}
 
 
 def lines_loader_bench(cj, ctx, data, offs, sizes, chunk_bytes, parts, launches, reps: int = 3):

@@ -554,6 +554,25 @@ void stream_rows(cjx_context *ctx, RowStream &rs, uint64_t n, uint64_t part, uin
     }
 }
 
+// Whether the units' bytes lie densely in the source: the span from the
+// first unit's start (rounded down to a 16-byte address) to the last unit's
+// end holds at most an eighth more than the units themselves, plus 1 MiB.
+// Then one copy of the span and a placement on the device replace the
+// host's gathering of the units into a staging buffer.
+bool dense_span(const uint8_t *data, const std::vector<uint64_t> &offs,
+                const std::vector<uint64_t> &sizes, uint64_t total, const uint8_t *&lo,
+                uint64_t &bytes) {
+    uint64_t a = UINT64_MAX, b = 0;
+    for (size_t i = 0; i < offs.size(); i++) {
+        a = std::min(a, offs[i]);
+        b = std::max(b, offs[i] + sizes[i]);
+    }
+    lo = reinterpret_cast<const uint8_t *>(reinterpret_cast<uintptr_t>(data + a) &
+                                           ~(uintptr_t)15);
+    bytes = (uint64_t)((data + b) - lo);
+    return bytes <= total + total / 8 + (1u << 20);
+}
+
 int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint64_t> &offs,
                    const std::vector<uint64_t> &sizes, const std::vector<uint64_t> &pos_base,
                    unsigned flags, bool concat, std::vector<UnitDesc> &units, float *ms,
@@ -571,7 +590,21 @@ int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint
     // One large unit: copy in parts overlapped with the tokenizer (first
     // attempt only; a re-run with exact regions finds the input in place).
     const uint64_t part = chunked_part(units);
+    const uint8_t *span_lo = nullptr;
+    uint64_t span_bytes = 0;
+    std::vector<uint64_t> unit_src;  // units' offsets in the copied span
     if (part) {
+    } else if (units.size() > 16 && dense_span(data, offs, sizes, total, span_lo, span_bytes)) {
+        // One copy of the source span from its 16-byte aligned start, then
+        // k_place_units moves every unit to its aligned offset (the JSON
+        // Lines chunks of a file lie back to back).
+        if (!ctx->span.ensure(span_bytes + 32)) return CJX_ERR_ALLOC;
+        if (hipMemcpyAsync(ctx->span.p, span_lo, span_bytes, hipMemcpyHostToDevice, s) !=
+            hipSuccess)
+            return CJX_ERR_DEVICE;
+        unit_src.resize(units.size());
+        for (size_t i = 0; i < units.size(); i++)
+            unit_src[i] = (uint64_t)((data + offs[i]) - span_lo);
     } else if (units.size() > 16) {  // one pinned staging copy and one H2D instead of one per unit
         if (!ctx->h_stage.ensure(dev_bytes)) return CJX_ERR_ALLOC;
         uint8_t *st = static_cast<uint8_t *>(ctx->h_stage.p);
@@ -597,8 +630,14 @@ int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint
             (records && !ctx->row_nl.ensure(sh.rows + 16)))
             return CJX_ERR_ALLOC;
         ctx->want_nl = records;  // the tokenizer also writes the newline flags
-        if ((rc = upload_units(ctx, units, ntiles, n_small))) return rc;
+        const bool place = !unit_src.empty() && attempt == 0;
+        if ((rc = upload_units(ctx, units, ntiles, n_small, place ? unit_src.data() : nullptr)))
+            return rc;
         (void)hipEventRecord(ctx->ev0, s);
+        if (place && launch_place_units(s, ctx->span.as<uint8_t>(), span_bytes, ctx->unit_src,
+                                        ctx->units.as<UnitDesc>(), (uint32_t)units.size(),
+                                        ctx->in.as<uint8_t>(), dev_bytes) != hipSuccess)
+            return CJX_ERR_DEVICE;
         if (part && attempt == 0) {  // device_ms then includes waiting for the copies
             if (rs && ensure_out_stream(ctx) != CJX_OK) rs->ok = false;
             const bool counts = rs && rs->ok;
@@ -866,7 +905,7 @@ void cjx_context_destroy(cjx_context *ctx) {
                       &ctx->gagg,    &ctx->gd0,        &ctx->lows,   &ctx->highs,
                       &ctx->chunk_unit, &ctx->big_units, &ctx->unit_part, &ctx->rec_count,
                       &ctx->recs,       &ctx->row_nl,   &ctx->nl_seg, &ctx->nl_off,
-                      &ctx->nl_idx,     &ctx->val_exact};
+                      &ctx->nl_idx,     &ctx->val_exact, &ctx->span};
     for (DevBuf *b : bufs) b->release();
     ctx->h_units.release();
     ctx->h_uout.release();

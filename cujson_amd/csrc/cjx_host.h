@@ -78,6 +78,7 @@ struct cjx_context {
     uint64_t timing_count = 0;
     bool timing = false;
     DevBuf in, structural, pair;
+    DevBuf span;  // host units copied as one span of their source, before placement
     DevBuf units, uout, ls, tile_st, layout, chunk_unit, big_units, unit_part, rec_count, recs;
     uint32_t *tile_unit = nullptr;    // inside `units`, after the unit descriptors
     uint32_t *small_units = nullptr;  // after the tile -> unit map

@@ -220,6 +220,37 @@ def test_chunked_host_input_fault(ctx, doc96):
     check_standard(ctx, d, label="chunked with a fault")
 
 
+@pytest.mark.parametrize("shift", [0, 3])
+def test_lines_host_span_placement(ctx, shift):
+    """More than 16 chunks from host memory: the source span goes in as one
+    copy and the device places every chunk at its aligned offset (source at
+    an aligned and at an odd address); chunks far apart in their buffer (a
+    sparse span) take the staging copy instead.  All equal the oracle."""
+    d = bytes(gen.lines(3000))
+    chunks = oracle.chunks_by_count(d, 40)
+    o = oracle.parse_lines(d, chunks)
+    assert len(chunks) > 16 and o.code == oracle.OK
+    buf = np.zeros(len(d) + 16, np.uint8)
+    buf[shift:shift + len(d)] = np.frombuffer(d, np.uint8)
+    view = buf[shift:shift + len(d)]
+    inp = cj.cuJSONLinesInput(view, len(chunks), len(d), [c[0] for c in chunks],
+                              [c[1] for c in chunks])
+    g = cj.parse_json_lines(inp, 0, ctx)
+    assert g.error == 0 and g.N == o.N and np.array_equal(g.buf, o.buf), shift
+    # sparse: every chunk followed by a 64 KiB gap in the caller's buffer
+    gap = 64 << 10
+    sparse = np.zeros(len(d) + gap * len(chunks), np.uint8)
+    offs = []
+    for i, (a, z) in enumerate(chunks):
+        offs.append(a + gap * i)
+        sparse[offs[-1]:offs[-1] + z] = np.frombuffer(d[a:a + z], np.uint8)
+    r = cj.parse_json_lines(cj.cuJSONLinesInput(sparse, len(chunks), sparse.size, offs,
+                                                [c[1] for c in chunks]), 0, ctx)
+    assert r.error == 0 and r.N == o.N
+    # positions count bytes of the chunks only, so the layout is the same
+    assert np.array_equal(r.buf, o.buf)
+
+
 def test_chunked_host_sparse_head(ctx):
     """The structural rows of a chunked host parse are copied out part by
     part into a buffer sized from the first part's density.  Here the first
