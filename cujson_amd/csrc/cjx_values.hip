@@ -103,53 +103,98 @@ __global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, 
         }
     }
 #endif
-    if (h > N) return;
-    const GapText g = gap_of(src, off, n, st, N, h);
-    ValueOut v;
-    v.v = 0;
-    v.row = (uint32_t)h;
-    v.text_off = g.a;
-    v.text_len = g.b - g.a;
-    v.type = kValNone;
-    v.flags = 0;
-    v.reserved = 0;
-    const uint8_t *s = src + (g.a - off);
-    const uint32_t len = g.b - g.a;
-    if (len == 0) {
-        // nothing between the two structural characters
-    } else if (s[0] == '"') {
+    // Classify every gap; empty gaps and literals are done at once.  Strings
+    // and numbers go to two block lists, then threads 0.. of the block take
+    // one list entry each: a wave runs one decoder for all its lanes instead
+    // of every decoder for some of them.
+    __shared__ uint32_t s_a[256], s_b[256];
+    __shared__ uint8_t s_next[256];
+    __shared__ uint8_t s_list[2][256];
+    __shared__ uint32_t s_cnt[2];
+    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    if (h <= N) {
+        const GapText g = gap_of(src, off, n, st, N, h);
+        const uint8_t *s = src + (g.a - off);
+        const uint32_t len = g.b - g.a;
+        int list = -1;
+        ValueOut v;
+        v.v = 0;
+        v.row = (uint32_t)h;
+        v.text_off = g.a;
+        v.text_len = len;
+        v.type = kValNone;
+        v.flags = 0;
+        v.reserved = 0;
+        if (len == 0) {
+            // nothing between the two structural characters
+        } else if (s[0] == '"') {
+            list = 0;
+        } else if (literal(s, len, "true", 4)) {
+            v.type = kValTrue;
+        } else if (literal(s, len, "false", 5)) {
+            v.type = kValFalse;
+        } else if (literal(s, len, "null", 4)) {
+            v.type = kValNull;
+        } else {
+            list = 1;
+        }
+        if (list < 0) {
+            out[h] = v;
+        } else {
+            const uint32_t k = atomicAdd(&s_cnt[list], 1u);
+            s_list[list][k] = (uint8_t)threadIdx.x;
+            s_a[threadIdx.x] = g.a;
+            s_b[threadIdx.x] = g.b;
+            s_next[threadIdx.x] = g.next;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < s_cnt[0]) {  // a string
+        const uint32_t t = s_list[0][threadIdx.x];
+        const uint64_t hs = h0 + t;
+        const uint32_t a = s_a[t], len = s_b[t] - a;
+        const uint8_t *s = src + (a - off);
+        ValueOut v;
+        v.v = 0;
+        v.row = (uint32_t)hs;
+        v.type = kValInvalid;
+        v.flags = 0;
+        v.reserved = 0;
         uint32_t out_len = 0;
         uint8_t flags = 0;
         if (len >= 2 && s[len - 1] == '"' &&
-            unescape(s + 1, len - 2, strings ? strings + g.a + 1 : nullptr, out_len, flags)) {
-            v.type = g.next == ':' ? kValKey : kValString;
+            unescape(s + 1, len - 2, strings ? strings + a + 1 : nullptr, out_len, flags)) {
+            v.type = s_next[t] == ':' ? kValKey : kValString;
             v.flags = flags;
             v.v = out_len;
-        } else {
-            v.type = kValInvalid;
         }
-        v.text_off = g.a + 1;
+        v.text_off = a + 1;
         v.text_len = len >= 2 ? len - 2 : 0;
-    } else if (literal(s, len, "true", 4)) {
-        v.type = kValTrue;
-    } else if (literal(s, len, "false", 5)) {
-        v.type = kValFalse;
-    } else if (literal(s, len, "null", 4)) {
-        v.type = kValNull;
-    } else {
-        NumText t;
+        out[hs] = v;
+    }
+    if (threadIdx.x < s_cnt[1]) {  // a number (or invalid text)
+        const uint32_t t = s_list[1][threadIdx.x];
+        const uint64_t hn = h0 + t;
+        const uint32_t a = s_a[t], len = s_b[t] - a;
+        ValueOut v;
+        v.row = (uint32_t)hn;
+        v.text_off = a;
+        v.text_len = len;
+        v.reserved = 0;
+        NumText nt;
         uint64_t m_lo = 0;
         int64_t e_ulp = 0, iv = 0;
         uint8_t flags = 0;
-        v.type = parse_number(s, len, iv, flags, t, m_lo, e_ulp);
+        v.type = parse_number(src + (a - off), len, iv, flags, nt, m_lo, e_ulp);
         v.v = iv;
         if (flags & kValExact) {
             const uint32_t k = atomicAdd(exact_count, 1u);
-            exact_list[k] = (uint32_t)h;
+            exact_list[k] = (uint32_t)hn;
         }
         v.flags = flags & (uint8_t)~kValExact;
+        out[hn] = v;
     }
-    out[h] = v;
 }
 
 __global__ __launch_bounds__(64) void k_values_exact(const uint8_t *__restrict__ in, uint64_t n,
