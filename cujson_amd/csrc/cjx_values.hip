@@ -103,15 +103,16 @@ __global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, 
         }
     }
 #endif
-    // Classify every gap; empty gaps and literals are done at once.  Strings
-    // and numbers go to two block lists, then threads 0.. of the block take
-    // one list entry each: a wave runs one decoder for all its lanes instead
-    // of every decoder for some of them.
+    // Classify every gap; empty gaps and literals are done at once.  Keys,
+    // other strings and numbers go to three block lists, then threads 0.. of
+    // the block take one list entry each: a wave runs one decoder for all its
+    // lanes instead of every decoder for some of them, and short keys do not
+    // wait for long strings.
     __shared__ uint32_t s_a[256], s_b[256];
     __shared__ uint8_t s_next[256];
-    __shared__ uint8_t s_list[2][256];
-    __shared__ uint32_t s_cnt[2];
-    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+    __shared__ uint8_t s_list[3][256];  // keys, other strings, numbers
+    __shared__ uint32_t s_cnt[3];
+    if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     if (h <= N) {
         const GapText g = gap_of(src, off, n, st, N, h);
@@ -129,7 +130,7 @@ __global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, 
         if (len == 0) {
             // nothing between the two structural characters
         } else if (s[0] == '"') {
-            list = 0;
+            list = g.next == ':' ? 0 : 1;  // keys are short: their own waves
         } else if (literal(s, len, "true", 4)) {
             v.type = kValTrue;
         } else if (literal(s, len, "false", 5)) {
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, 
         } else if (literal(s, len, "null", 4)) {
             v.type = kValNull;
         } else {
-            list = 1;
+            list = 2;
         }
         if (list < 0) {
             out[h] = v;
@@ -150,8 +151,10 @@ __global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, 
         }
     }
     __syncthreads();
-    if (threadIdx.x < s_cnt[0]) {  // a string
-        const uint32_t t = s_list[0][threadIdx.x];
+    const uint32_t n_keys = s_cnt[0];
+    if (threadIdx.x < n_keys + s_cnt[1]) {  // keys on the first threads, then other strings
+        const uint32_t t = threadIdx.x < n_keys ? s_list[0][threadIdx.x]
+                                                : s_list[1][threadIdx.x - n_keys];
         const uint64_t hs = h0 + t;
         const uint32_t a = s_a[t], len = s_b[t] - a;
         const uint8_t *s = src + (a - off);
@@ -173,8 +176,8 @@ __global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, 
         v.text_len = len >= 2 ? len - 2 : 0;
         out[hs] = v;
     }
-    if (threadIdx.x < s_cnt[1]) {  // a number (or invalid text)
-        const uint32_t t = s_list[1][threadIdx.x];
+    if (threadIdx.x < s_cnt[2]) {  // a number (or invalid text)
+        const uint32_t t = s_list[2][threadIdx.x];
         const uint64_t hn = h0 + t;
         const uint32_t a = s_a[t], len = s_b[t] - a;
         ValueOut v;
