@@ -103,16 +103,18 @@ __global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, 
         }
     }
 #endif
-    // Classify every gap; empty gaps and literals are done at once.  Keys,
-    // other strings and numbers go to three block lists, then threads 0.. of
-    // the block take one list entry each: a wave runs one decoder for all its
-    // lanes instead of every decoder for some of them, and short keys do not
-    // wait for long strings.
+    // Classify every gap; empty gaps and literals are done at once.  Strings
+    // (in four lists by length) and numbers go to block lists, then threads
+    // 0.. of the block take one list entry each: a wave runs one decoder for
+    // all its lanes instead of every decoder for some of them, and strings of
+    // similar length share waves (a wave takes as long as its longest one).
     __shared__ uint32_t s_a[256], s_b[256];
     __shared__ uint8_t s_next[256];
-    __shared__ uint8_t s_list[3][256];  // keys, other strings, numbers
-    __shared__ uint32_t s_cnt[3];
-    if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
+    // lists 0-3: strings by length (under 16, 32, 48 bytes, longer); 4: numbers
+    constexpr int kStrLists = 4, kNumList = 4;
+    __shared__ uint8_t s_list[kStrLists + 1][256];
+    __shared__ uint32_t s_cnt[kStrLists + 1];
+    if (threadIdx.x < kStrLists + 1) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     if (h <= N) {
         const GapText g = gap_of(src, off, n, st, N, h);
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, 
         if (len == 0) {
             // nothing between the two structural characters
         } else if (s[0] == '"') {
-            list = g.next == ':' ? 0 : 1;  // keys are short: their own waves
+            list = len < 48 ? (int)(len >> 4) : 3;  // similar lengths share waves
         } else if (literal(s, len, "true", 4)) {
             v.type = kValTrue;
         } else if (literal(s, len, "false", 5)) {
@@ -138,7 +140,7 @@ __global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, 
         } else if (literal(s, len, "null", 4)) {
             v.type = kValNull;
         } else {
-            list = 2;
+            list = kNumList;
         }
         if (list < 0) {
             out[h] = v;
@@ -151,10 +153,13 @@ __global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, 
         }
     }
     __syncthreads();
-    const uint32_t n_keys = s_cnt[0];
-    if (threadIdx.x < n_keys + s_cnt[1]) {  // keys on the first threads, then other strings
-        const uint32_t t = threadIdx.x < n_keys ? s_list[0][threadIdx.x]
-                                                : s_list[1][threadIdx.x - n_keys];
+    const uint32_t c0 = s_cnt[0], c1 = c0 + s_cnt[1], c2 = c1 + s_cnt[2], c3 = c2 + s_cnt[3];
+    if (threadIdx.x < c3) {  // the string lists in order over the block's threads
+        const uint32_t i = threadIdx.x;
+        const uint32_t t = i < c0   ? s_list[0][i]
+                           : i < c1 ? s_list[1][i - c0]
+                           : i < c2 ? s_list[2][i - c1]
+                                    : s_list[3][i - c2];
         const uint64_t hs = h0 + t;
         const uint32_t a = s_a[t], len = s_b[t] - a;
         const uint8_t *s = src + (a - off);
@@ -176,8 +181,8 @@ __global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, 
         v.text_len = len >= 2 ? len - 2 : 0;
         out[hs] = v;
     }
-    if (threadIdx.x < s_cnt[2]) {  // a number (or invalid text)
-        const uint32_t t = s_list[2][threadIdx.x];
+    if (threadIdx.x < s_cnt[kNumList]) {  // a number (or invalid text)
+        const uint32_t t = s_list[kNumList][threadIdx.x];
         const uint64_t hn = h0 + t;
         const uint32_t a = s_a[t], len = s_b[t] - a;
         ValueOut v;
