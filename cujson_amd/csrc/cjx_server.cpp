@@ -444,6 +444,25 @@ hipError_t wait_event(Slot &s) {
     }
 }
 
+// Replaces a slot context's stream by one created with a CU mask of every
+// CU of the device: no CU is left out, and the runtime gives such a stream a
+// hardware queue of its own, so the slot never waits behind another stream's
+// copies on a shared queue.
+int own_queue_stream(int device, cjx_context *ctx) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return CJX_ERR_DEVICE;
+    const uint32_t cus = (uint32_t)prop.multiProcessorCount;
+    std::vector<uint32_t> mask((cus + 31) / 32, 0xFFFFFFFFu);
+    if (cus % 32) mask.back() = (1u << (cus % 32)) - 1u;
+    hipStream_t st = nullptr;
+    if (hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()) != hipSuccess)
+        return CJX_ERR_DEVICE;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    ctx->stream = st;
+    return CJX_OK;
+}
+
 // One host thread per slot: admit a batch, launch it, wait, lay it out,
 // hand the results out, repeat.  The threads share only the request queue,
 // the region pool and the statistics (under the server lock).
@@ -503,6 +522,7 @@ int cjx_server_create(int device, size_t device_budget, int n_slots, size_t max_
         int rc = cjx_context_create(device, &s.ctx);
         if (!rc && hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess)
             rc = CJX_ERR_DEVICE;
+        if (!rc) rc = own_queue_stream(device, s.ctx);
         if (rc) {
             for (Slot &t : srv->slots) {
                 if (t.ev) (void)hipEventDestroy(t.ev);
