@@ -447,16 +447,20 @@ hipError_t wait_event(Slot &s) {
 // Replaces a slot context's stream by one created with a CU mask of every
 // CU of the device: no CU is left out, and the runtime gives such a stream a
 // hardware queue of its own, so the slot never waits behind another stream's
-// copies on a shared queue.
+// copies on a shared queue.  Where that cannot be had, the slot keeps the
+// context's ordinary stream.
 int own_queue_stream(int device, cjx_context *ctx) {
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return CJX_ERR_DEVICE;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess || prop.multiProcessorCount <= 0)
+        return CJX_OK;
     const uint32_t cus = (uint32_t)prop.multiProcessorCount;
     std::vector<uint32_t> mask((cus + 31) / 32, 0xFFFFFFFFu);
     if (cus % 32) mask.back() = (1u << (cus % 32)) - 1u;
     hipStream_t st = nullptr;
-    if (hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()) != hipSuccess)
-        return CJX_ERR_DEVICE;
+    if (hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
+        (void)hipGetLastError();  // clear it: the ordinary stream stays
+        return CJX_OK;
+    }
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
     ctx->stream = st;
