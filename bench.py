@@ -114,31 +114,43 @@ def many_docs_bench(cj, gen, n_docs: int, reps: int, plain_compare: bool = True)
     stream = ctx.stream
     offs_l, sizes_l = offs.tolist(), sizes.tolist()
 
-    def batched():  # the whole batch as one captured HIP graph
-        rc = ctx.parse_batch_device(d_in.ptr, offs, sizes, d_st.ptr, d_pr.ptr, cap,
-                                    cj.CJX_FLAG_GRAPH)
+    def batched(flags):  # the whole batch as one launch sequence (or its HIP graph)
+        rc = ctx.parse_batch_device(d_in.ptr, offs, sizes, d_st.ptr, d_pr.ptr, cap, flags)
         if rc:
             raise RuntimeError(f"parse_batch_device: {cj.strerror(rc)}")
 
-    batched()
+    batched(cj.CJX_FLAG_GRAPH)
     rc, docs, N, M, over = ctx.batch_wait()
     if rc or over:
         raise RuntimeError(f"batch failed rc={rc} overflow={over}")
     e0, e1 = Event(), Event()
-    times = []
-    ctx.set_timing(True)
-    for _ in range(reps):  # per call: host planning + unit-table upload + device
-        synchronize()
-        e0.record(stream)
-        batched()
-        e1.record(stream)
-        e1.synchronize()
-        times.append(e0.elapsed_ms(e1))
-    dev = [t[2] for t in ctx.timings(reps)]  # device: the graph's start to end on the stream
-    ctx.set_timing(False)
-    ctx.batch_wait()
-    batched_ms = float(np.median(dev))
-    call_ms = float(np.median(times))
+
+    def timed(flags):
+        """(device ms, call ms) medians: device from the context's timing ring
+        (the sequence's start to end on the stream), call from events around
+        the call (host planning + unit-table upload + device)."""
+        ctx.set_timing(True)  # warm-up calls also fill the timing ring's events
+        for _ in range(2):  # the graph form captures on its first call
+            batched(flags)
+            ctx.batch_wait()
+        times = []
+        for _ in range(reps):
+            synchronize()
+            e0.record(stream)
+            batched(flags)
+            e1.record(stream)
+            e1.synchronize()
+            times.append(e0.elapsed_ms(e1))
+        dev = [t[2] for t in ctx.timings(reps)]
+        ctx.set_timing(False)
+        ctx.batch_wait()
+        return float(np.median(dev)), float(np.median(times))
+
+    graph_ms, graph_call_ms = timed(cj.CJX_FLAG_GRAPH)
+    launched_ms, launched_call_ms = timed(0)
+    batched_form = "graph" if graph_ms <= launched_ms else "launched"
+    batched_ms, call_ms = ((graph_ms, graph_call_ms) if batched_form == "graph"
+                           else (launched_ms, launched_call_ms))
     codes = np.bincount(docs[:, 3], minlength=4).tolist()
 
     # plain: one parse per document, enqueued back to back on the same stream
@@ -171,8 +183,11 @@ def many_docs_bench(cj, gen, n_docs: int, reps: int, plain_compare: bool = True)
         "doc_codes": {"ok": codes[0], "utf8": codes[2], "structure": codes[3]},
         "batched_ms": round(batched_ms, 4),
         "batched_call_ms": round(call_ms, 4),
-        "batched_form": "one HIP graph (captured launch sequence) per batch; batched_ms is "
-                        "device time, batched_call_ms adds the host's unit planning and upload",
+        "batched_form": f"one launch sequence for the whole batch, {batched_form} (the faster "
+                        f"of its HIP graph replay and its launched form); batched_ms is device "
+                        f"time, batched_call_ms adds the host's unit planning and upload",
+        "graph_ms": round(graph_ms, 4), "launched_ms": round(launched_ms, 4),
+        "graph_call_ms": round(graph_call_ms, 4), "launched_call_ms": round(launched_call_ms, 4),
         "batched_gbs": round(total / (batched_ms * 1e-3) / 1e9, 2),
         "plain_ms": _num(plain_ms, 3),
         "plain_gbs": _num(total / (plain_ms * 1e-3) / 1e9, 3),
