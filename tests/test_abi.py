@@ -47,6 +47,23 @@ def test_chunk_lines_bytes_mode_matches_loader_rule(twitter_small, nb):
     assert list(zip(offs, sizes)) == oracle.chunks_by_bytes(twitter_small, nb)
 
 
+def test_chunk_lines_threaded_scan_matches_loader_rule():
+    """From 64 MiB the host chunker scans the text on several threads: an
+    80 MB text of random line lengths (blank lines, an unterminated last
+    line) gives the loader rule's chunks in both modes."""
+    rng = np.random.default_rng(5)
+    lens = rng.integers(0, 2000, 80_000)
+    lens[rng.integers(0, lens.size, 2000)] = 0  # blank lines
+    a = np.full(int(lens.sum()) + lens.size + 17, ord("x"), np.uint8)
+    a[np.cumsum(lens + 1)[:-1] - 1] = 10
+    a[-17:] = ord("y")  # unterminated last line
+    assert a.size >= 64 << 20
+    for mode, param, want in ((0, 997, oracle.chunks_by_count(a, 997)),
+                              (1, 1 << 20, oracle.chunks_by_bytes(a, 1 << 20))):
+        offs, sizes = cj.chunk_lines(a, mode, param)
+        assert list(zip(offs, sizes)) == want, mode
+
+
 def test_chunk_lines_unterminated_last_line():
     data = b'{"a":1}\n{"b":2}'
     offs, sizes = cj.chunk_lines(data, 0, 2)
