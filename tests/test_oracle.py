@@ -132,7 +132,6 @@ def test_utf8_reference_rule_known_answers():
 
 def test_utf8_rule_bruteforce_and_strict_mode():
     rng = random.Random(99)
-    ref = oracle.ref_lib()
     bytes_pool = [0x41, 0x80, 0x9F, 0xBF, 0xC2, 0xC0, 0xDF, 0xE0, 0xED, 0xEF, 0xF0, 0xF4, 0xF5, 0xFF]
     for _ in range(20000):
         s = bytes(rng.choice(bytes_pool) for _ in range(rng.randint(1, 12)))
@@ -144,9 +143,6 @@ def test_utf8_rule_bruteforce_and_strict_mode():
         except UnicodeDecodeError:
             valid = False
         assert strict == (0 if valid else 1), s
-        if ref is not None:
-            a, p = oracle._u8(s)
-            assert ref.sjr_validate_utf8(p, a.size) == (1 if valid else 0)
 
 
 def test_valid_utf8_never_flagged_by_weak_rule():
@@ -166,11 +162,6 @@ def test_fixture_twitter_large(twitter_large):
     assert int((np.frombuffer(twitter_large, np.uint8) >= 0x80).sum()) == 0
     pos = S.lex(twitter_large)
     assert r.structural[1:r.N + 1].tolist() == pos
-    res = oracle.simdjson_op_indices(twitter_large)
-    if res is not None:
-        e, idx = res
-        assert e == 0
-        assert np.array_equal(idx, r.structural[1:r.N + 1].astype(np.uint32))
     ok, pair = S.stack_pairs(twitter_large, pos)
     assert ok
     for t, t2 in pair.items():
@@ -230,3 +221,52 @@ def test_sentinels_and_layout():
 def test_empty_input():
     assert oracle.parse_standard(b"").code == oracle.ERR_INPUT
     assert oracle.parse_unit(b"").code == oracle.ERR_INPUT
+
+
+# ---------------------------------------------------------------------------
+# Cross-checks against simdjson 3.2.1 built from the reference's vendored
+# sources (oracle/Makefile, into oracle/_ref/).  Without that build these
+# tests are skipped with the reason, never passed silently.
+# ---------------------------------------------------------------------------
+def _simdjson():
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref/libsimdjson_ref.so absent: simdjson 3.2.1 is built by "
+                    "oracle/Makefile from the reference tree, which this host lacks")
+    return ref
+
+
+def _config_docs():
+    from cujson_amd import gen
+    return {"config1": gen.config1(),
+            "config2 shape 24 MiB": gen.config2(target_bytes=24 << 20),
+            "config4 document": gen.standard_doc(target_bytes=3 << 20, seed=gen.SEED_BASE + 4)}
+
+
+def test_simdjson_stage1_on_fixture_and_configs(twitter_large):
+    """simdjson stage-1 structural indices filtered to { } [ ] : , equal the
+    oracle's structural positions (valid JSON: S4-S7 as a textbook lexer)."""
+    _simdjson()
+    docs = {"twitter_large": np.frombuffer(twitter_large, np.uint8)}
+    docs.update(_config_docs())
+    for label, d in docs.items():
+        r = oracle.parse_unit(d, 0)
+        assert r.code == 0, label
+        e, idx = oracle.simdjson_op_indices(d)
+        assert e == 0, (label, e)
+        assert idx.size == r.structural.size, (label, idx.size, r.structural.size)
+        assert np.array_equal(idx, r.structural), label
+
+
+def test_simdjson_validate_utf8_matches_strict_mode():
+    ref = _simdjson()
+    rng = random.Random(99)
+    bytes_pool = [0x41, 0x80, 0x9F, 0xBF, 0xC2, 0xC0, 0xDF, 0xE0, 0xED, 0xEF, 0xF0, 0xF4, 0xF5, 0xFF]
+    for _ in range(20000):
+        s = bytes(rng.choice(bytes_pool) for _ in range(rng.randint(1, 12)))
+        a, p = oracle._u8(s)
+        strict = oracle.utf8_check(s, oracle.FLAG_UTF8_STRICT)
+        assert ref.sjr_validate_utf8(p, a.size) == (1 - strict), s
+    for label, d in _config_docs().items():
+        a, p = oracle._u8(d)
+        assert ref.sjr_validate_utf8(p, a.size) == 1 - oracle.utf8_check(d, oracle.FLAG_UTF8_STRICT)
