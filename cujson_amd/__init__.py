@@ -43,6 +43,7 @@ CJX_ERR_STRUCTURE = 3
 CJX_ERR_ALLOC = 4
 CJX_ERR_DEVICE = 5
 CJX_ERR_CAPACITY = 6
+CJX_ERR_TOO_LARGE = 7
 
 CJX_FLAG_UTF8_STRICT = 0x1
 CJX_FLAG_DEPTH_U16 = 0x2
@@ -60,7 +61,7 @@ ABI_SYMBOLS = (
     "cjx_parse_lines_device", "cjx_server_create", "cjx_server_submit", "cjx_server_wait",
     "cjx_server_release", "cjx_server_reserve", "cjx_server_get_stats", "cjx_server_destroy",
     "cjx_scan_lines_device", "cjx_chunk_lines_device", "cjx_load_device",
-    "cjx_values_device", "cjx_values",
+    "cjx_values_device", "cjx_values", "cjx_noalloc_max_bytes", "cjx_parse_device_noalloc",
 )
 
 
@@ -222,12 +223,37 @@ def lib():
     L.cjx_server_get_stats.restype = ctypes.c_int
     L.cjx_server_destroy.argtypes = [vp]
     L.cjx_server_destroy.restype = None
+    L.cjx_noalloc_max_bytes.argtypes = []
+    L.cjx_noalloc_max_bytes.restype = sz
+    L.cjx_parse_device_noalloc.argtypes = [vp, sz, ctypes.c_uint, vp, vp, sz, vp, vp]
+    L.cjx_parse_device_noalloc.restype = ctypes.c_int
     _lib = L
     return L
 
 
 def strerror(code: int) -> str:
     return lib().cjx_strerror(code).decode()
+
+
+# cjx_device_status as a numpy record (device status of the no-allocation entry)
+DEVICE_STATUS_DTYPE = np.dtype([("n_structural", "<i8"), ("n_brackets", "<i8"), ("error", "<i4"),
+                                ("capacity_exceeded", "<i4"), ("n_records", "<i8")])
+
+
+def noalloc_max_bytes() -> int:
+    """Largest input cjx_parse_device_noalloc accepts."""
+    return int(lib().cjx_noalloc_max_bytes())
+
+
+def parse_device_noalloc(d_input: int, n: int, d_structural: int, d_pair: int, cap: int,
+                         d_status: int, flags: int = 0, stream: int = 0) -> int:
+    """Enqueue cjx_parse_device_noalloc on `stream` (0: the null stream): the
+    device-resident parse with no allocation; the status (DEVICE_STATUS_DTYPE)
+    lands in device memory at d_status once the stream reaches it."""
+    return lib().cjx_parse_device_noalloc(ctypes.c_void_p(d_input), n, flags,
+                                          ctypes.c_void_p(d_structural), ctypes.c_void_p(d_pair),
+                                          cap, ctypes.c_void_p(d_status),
+                                          ctypes.c_void_p(stream or None))
 
 
 class Context:

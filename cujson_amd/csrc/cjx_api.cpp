@@ -40,16 +40,6 @@ int set_device(cjx_context *ctx) {
     return hipSetDevice(ctx->device) == hipSuccess ? CJX_OK : CJX_ERR_DEVICE;
 }
 
-uint32_t phase3_blocks(uint64_t max_m) {
-    uint64_t g = (max_m + 65535) / 65536;
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 512));
-}
-
-uint32_t pair_blocks(uint64_t max_n) {
-    uint64_t g = (max_n + kPairChunk - 1) / kPairChunk;
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 2048));
-}
-
 // Pinned host buffers for results.  Allocating and pinning a large host
 // buffer per call costs about as much as the copy into it, so released
 // result buffers are kept for reuse: a request takes the smallest cached
@@ -874,6 +864,7 @@ const char *cjx_strerror(int code) {
         case CJX_ERR_ALLOC: return "allocation failed";
         case CJX_ERR_DEVICE: return "device error";
         case CJX_ERR_CAPACITY: return "output capacity exceeded";
+        case CJX_ERR_TOO_LARGE: return "input larger than the entry point's maximum";
         default: return "unknown error";
     }
 }

@@ -43,6 +43,18 @@ constexpr int kMatchKeys = 256;
 constexpr int kMatchMaxDepth = 256;  // deeper units take the sort path
 constexpr int kSmallBrackets = 64;   // units with at most this many brackets: k_match_small
 
+// Grid of the sort-path kernels of one unit (phase3_blocks: depth walk,
+// keys, histogram, scatter over 64 Ki brackets per block) and of its pair
+// writer (pair_blocks: kPairChunk rows per block), from the unit's row bound.
+inline uint32_t phase3_blocks(uint64_t max_m) {
+    const uint64_t g = (max_m + 65535) / 65536;
+    return (uint32_t)(g < 1 ? 1 : (g > 512 ? 512 : g));
+}
+inline uint32_t pair_blocks(uint64_t max_n) {
+    const uint64_t g = (max_n + kPairChunk - 1) / kPairChunk;
+    return (uint32_t)(g < 1 ? 1 : (g > 2048 ? 2048 : g));
+}
+
 // One parse unit inside a launch: a standard document, a JSON Lines chunk or
 // one document of a batch.  Escape/string state, depth and counts restart at
 // every unit.  The unit's outputs (structural row, pair row and bracket

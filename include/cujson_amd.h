@@ -42,7 +42,8 @@ enum {
     CJX_ERR_STRUCTURE = 3,
     CJX_ERR_ALLOC = 4,
     CJX_ERR_DEVICE = 5,
-    CJX_ERR_CAPACITY = 6
+    CJX_ERR_CAPACITY = 6,
+    CJX_ERR_TOO_LARGE = 7 /* input above an entry point's stated maximum */
 };
 
 /* Flags. */
@@ -162,6 +163,30 @@ typedef struct cjx_device_status {
 int cjx_parse_device(cjx_context *ctx, const uint8_t *d_input, size_t n, unsigned flags,
                      int32_t *d_structural, int32_t *d_pair, size_t cap);
 int cjx_device_wait(cjx_context *ctx, cjx_device_status *st);
+
+/* cjx_parse_device without allocation (no counterpart in the reference).
+ * For callers that cannot allocate: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  Same kernels, inputs,
+ * outputs and results as cjx_parse_device, but no context: every working
+ * buffer lives in one device array of the library (about 20 MiB per device,
+ * held until the process ends), so a call makes no hipMalloc / hipFree, no
+ * pinned or host allocation and creates no event.
+ *   n            at most cjx_noalloc_max_bytes() (1 MiB); larger inputs return
+ *                CJX_ERR_TOO_LARGE before anything is launched
+ *   flags        CJX_FLAG_UTF8_STRICT, CJX_FLAG_DEPTH_U16 (others: CJX_ERR_INPUT)
+ *   d_status     device-accessible cjx_device_status, written by the last kernel
+ *                of the call (what cjx_device_wait reports for cjx_parse_device)
+ *   stream       hipStream_t to launch on (NULL: the null stream)
+ * Returns after enqueueing; the caller synchronizes the stream before
+ * reading d_status or the rows.  Calls on one device share that device's
+ * array (the last call's contents stay in it): calls that overlap on one
+ * device, from any streams or threads, race and must be serialized by the
+ * caller.  The first call on a device resolves the array's device address
+ * (with that device current). */
+size_t cjx_noalloc_max_bytes(void);
+int cjx_parse_device_noalloc(const uint8_t *d_input, size_t n, unsigned flags,
+                             int32_t *d_structural, int32_t *d_pair, size_t cap,
+                             cjx_device_status *d_status, void *stream);
 
 /* Per-phase device times of recent parses on a context, from hipEvents
  * recorded on the context stream around each phase while timing is enabled

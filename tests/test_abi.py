@@ -192,3 +192,25 @@ def test_byte_class_tables():
             if b == 0x22:
                 want |= 0x80
             assert f(b, jsonl) == want, (jsonl, hex(b), hex(f(b, jsonl)), hex(want))
+
+
+def test_noalloc_entry_rejects_before_launching():
+    """cjx_parse_device_noalloc checks its arguments on the host before any
+    HIP call: missing pointers or an unaligned input are CJX_ERR_INPUT, flags
+    other than the UTF-8 and depth flags are CJX_ERR_INPUT, and an input
+    above the stated maximum is CJX_ERR_TOO_LARGE."""
+    L = cj.lib()
+    mx = cj.noalloc_max_bytes()
+    assert mx == 1 << 20
+    fake = 1 << 40  # never dereferenced: every case below returns first
+    f = cj.parse_device_noalloc
+    assert f(0, 100, fake, fake, 100, fake) == cj.CJX_ERR_INPUT
+    assert f(fake, 0, fake, fake, 100, fake) == cj.CJX_ERR_INPUT
+    assert f(fake, 100, fake, fake, 100, 0) == cj.CJX_ERR_INPUT
+    assert f(fake + 4, 100, fake, fake, 100, fake) == cj.CJX_ERR_INPUT
+    for flags in (cj.CJX_FLAG_GRAPH, cj.CJX_FLAG_JSONL, cj.CJX_FLAG_RECORDS, 0x100):
+        assert f(fake, 100, fake, fake, 100, fake, flags) == cj.CJX_ERR_INPUT, flags
+    assert f(fake, mx + 1, fake, fake, mx + 1, fake) == cj.CJX_ERR_TOO_LARGE
+    assert "maximum" in cj.strerror(cj.CJX_ERR_TOO_LARGE)
+    assert L.cjx_noalloc_max_bytes() == mx
+
