@@ -21,10 +21,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# The request server drives one stream per slot (8); with HIP's default of 4
-# hardware queues per process, streams share queues and a copy wait in one
-# blocks the kernels of another.  Set before the HIP runtime loads.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 METRIC = "input GB/s and % HBM roofline, batched vs plain at 1 GPU; p50/p99 parse latency"
 HBM_PEAK_GBS = 8000.0
@@ -431,6 +427,10 @@ def serving_bench(seconds: float, rate: float, plain_seconds: float, slots: int 
     rc = L.cjx_loadgen_run(a.encode(), buf, len(buf))
     out = json.loads(buf.value.decode() or "{}")
     out["rc"] = int(rc)
+    # the server's slot streams are CU-masked streams with a hardware queue
+    # each; other streams share the process's GPU_MAX_HW_QUEUES queues
+    out["hw_queues"] = {"slot_streams": slots,
+                        "GPU_MAX_HW_QUEUES": os.environ.get("GPU_MAX_HW_QUEUES", "unset (4)")}
     out["payloads"] = "pinned host memory (every request crosses the host link both ways)"
     if out.get("plain_GBps"):
         out["sustained_vs_plain"] = round(out["sustained_GBps"] / out["plain_GBps"], 2)

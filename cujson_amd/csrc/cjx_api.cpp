@@ -14,6 +14,8 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
+#include <system_error>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -547,21 +549,23 @@ void stream_rows(cjx_context *ctx, RowStream &rs, uint64_t n, uint64_t part, uin
 }
 
 // Whether the units' bytes lie densely in the source: the span from the
-// first unit's start (rounded down to a 16-byte address) to the last unit's
-// end holds at most an eighth more than the units themselves, plus 1 MiB.
-// Then one copy of the span and a placement on the device replace the
-// host's gathering of the units into a staging buffer.
+// first unit's start to the last unit's end holds at most an eighth more
+// than the units themselves, plus 1 MiB.  Then one copy of the span and a
+// placement on the device replace the host's gathering of the units into a
+// staging buffer.  The copy reads exactly [lo, lo + bytes) of the caller's
+// memory and lands at device offset skew = lo's address mod 16, so that
+// device and host addresses agree mod 16.
 bool dense_span(const uint8_t *data, const std::vector<uint64_t> &offs,
                 const std::vector<uint64_t> &sizes, uint64_t total, const uint8_t *&lo,
-                uint64_t &bytes) {
+                uint64_t &bytes, uint64_t &skew) {
     uint64_t a = UINT64_MAX, b = 0;
     for (size_t i = 0; i < offs.size(); i++) {
         a = std::min(a, offs[i]);
         b = std::max(b, offs[i] + sizes[i]);
     }
-    lo = reinterpret_cast<const uint8_t *>(reinterpret_cast<uintptr_t>(data + a) &
-                                           ~(uintptr_t)15);
-    bytes = (uint64_t)((data + b) - lo);
+    lo = data + a;
+    bytes = b - a;
+    skew = reinterpret_cast<uintptr_t>(lo) & 15u;
     return bytes <= total + total / 8 + (1u << 20);
 }
 
@@ -583,20 +587,22 @@ int run_host_units(cjx_context *ctx, const uint8_t *data, const std::vector<uint
     // attempt only; a re-run with exact regions finds the input in place).
     const uint64_t part = chunked_part(units);
     const uint8_t *span_lo = nullptr;
-    uint64_t span_bytes = 0;
+    uint64_t span_bytes = 0, span_skew = 0;
     std::vector<uint64_t> unit_src;  // units' offsets in the copied span
     if (part) {
-    } else if (units.size() > 16 && dense_span(data, offs, sizes, total, span_lo, span_bytes)) {
-        // One copy of the source span from its 16-byte aligned start, then
-        // k_place_units moves every unit to its aligned offset (the JSON
-        // Lines chunks of a file lie back to back).
-        if (!ctx->span.ensure(span_bytes + 32)) return CJX_ERR_ALLOC;
-        if (hipMemcpyAsync(ctx->span.p, span_lo, span_bytes, hipMemcpyHostToDevice, s) !=
-            hipSuccess)
+    } else if (units.size() > 16 &&
+               dense_span(data, offs, sizes, total, span_lo, span_bytes, span_skew)) {
+        // One copy of the source span, then k_place_units moves every unit
+        // to its aligned offset (the JSON Lines chunks of a file lie back to
+        // back).
+        if (!ctx->span.ensure(span_skew + span_bytes + 32)) return CJX_ERR_ALLOC;
+        if (hipMemcpyAsync(ctx->span.as<uint8_t>() + span_skew, span_lo, span_bytes,
+                           hipMemcpyHostToDevice, s) != hipSuccess)
             return CJX_ERR_DEVICE;
+        span_bytes += span_skew;  // the device span: [0, skew) unused, then the bytes
         unit_src.resize(units.size());
         for (size_t i = 0; i < units.size(); i++)
-            unit_src[i] = (uint64_t)((data + offs[i]) - span_lo);
+            unit_src[i] = span_skew + (uint64_t)((data + offs[i]) - span_lo);
     } else if (units.size() > 16) {  // one pinned staging copy and one H2D instead of one per unit
         if (!ctx->h_stage.ensure(dev_bytes)) return CJX_ERR_ALLOC;
         uint8_t *st = static_cast<uint8_t *>(ctx->h_stage.p);
@@ -1063,10 +1069,17 @@ int emit_chunks(const ChunkList &out, size_t *chunk_off, size_t *chunk_size, siz
     return out.size() <= cap ? CJX_OK : CJX_ERR_CAPACITY;
 }
 
+// Test hook: when positive, the threaded line scan behaves as if only this
+// many of its threads could be created (cjx_debug_fail_threads).
+std::atomic<int> g_thread_limit{0};
+
 // Appends the offset after every '\n' of data[0..n) to offs, in order.  From
 // 64 MiB the text is cut into segments of at least 16 MiB scanned by up to 16
 // threads, each collecting its own offsets; the lists are joined in order.
-void host_line_starts(const uint8_t *data, size_t n, std::vector<uint64_t> &offs) {
+// If a thread cannot be created, the threads already started are joined and
+// the whole text is scanned on the calling thread.  Returns false when
+// memory runs out (nothing escapes into the C ABI).
+bool host_line_starts(const uint8_t *data, size_t n, std::vector<uint64_t> &offs) {
     auto scan = [data](size_t a, size_t b, std::vector<uint64_t> &v) {
         const uint8_t *p = data + a, *end = data + b;
         while (p < end) {
@@ -1078,21 +1091,46 @@ void host_line_starts(const uint8_t *data, size_t n, std::vector<uint64_t> &offs
     };
     const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
     const size_t T = n >= (64u << 20) ? std::min<size_t>({hw, 16, n / (16u << 20)}) : 1;
-    if (T <= 1) {
-        scan(0, n, offs);
-        return;
+    try {
+        if (T > 1) {
+            std::vector<std::vector<uint64_t>> part(T);
+            std::vector<char> ok(T, 0);
+            std::vector<std::thread> th;
+            th.reserve(T);
+            bool started = true;
+            try {
+                for (size_t t = 0; t < T; t++) {
+                    const int lim = g_thread_limit.load();
+                    if (lim > 0 && t >= (size_t)lim)
+                        throw std::system_error(std::make_error_code(std::errc::resource_unavailable_try_again));
+                    th.emplace_back([&, t] {
+                        try {
+                            scan(n * t / T, n * (t + 1) / T, part[t]);
+                            ok[t] = 1;
+                        } catch (...) {
+                        }
+                    });
+                }
+            } catch (const std::system_error &) {
+                started = false;
+            }
+            for (std::thread &x : th) x.join();
+            bool all = started;
+            for (size_t t = 0; t < T && all; t++) all = ok[t] != 0;
+            if (all) {
+                size_t total = offs.size();
+                for (const auto &v : part) total += v.size();
+                offs.reserve(total);
+                for (const auto &v : part) offs.insert(offs.end(), v.begin(), v.end());
+                return true;
+            }
+            if (started) return false;  // a thread ran out of memory
+        }
+        scan(0, n, offs);  // one thread (small texts, or threads unavailable)
+        return true;
+    } catch (const std::bad_alloc &) {
+        return false;
     }
-    std::vector<std::vector<uint64_t>> part(T);
-    std::vector<std::thread> th;
-    for (size_t t = 0; t < T; t++)
-        th.emplace_back(scan, n * t / T, n * (t + 1) / T, std::ref(part[t]));
-    size_t total = offs.size();
-    for (size_t t = 0; t < T; t++) {
-        th[t].join();
-        total += part[t].size();
-    }
-    offs.reserve(total);
-    for (const auto &v : part) offs.insert(offs.end(), v.begin(), v.end());
 }
 
 }  // namespace
@@ -1107,20 +1145,30 @@ int cjx_chunk_lines(const uint8_t *data, size_t n, int mode, size_t param, size_
     if (!chunk_rule_ok(mode, param)) return CJX_ERR_INPUT;
     // Line starts (load_file.cu:72-81): 0, every byte after '\n', and n if the
     // last line is unterminated.
-    std::vector<uint64_t> offs;
-    offs.push_back(0);
-    host_line_starts(data, n, offs);
-    if (offs.back() < n) offs.push_back(n);
-    ChunkList out;
-    if (mode == 0) {
-        std::vector<uint64_t> li;
-        count_rule(offs.size() - 1, param, li);
-        for (size_t i = 0; i < param; i++) out.emplace_back(offs[li[i]], offs[li[i + 1]] - offs[li[i]]);
-    } else {
-        bytes_rule([&](size_t j) { return offs[j]; }, offs.size(), n, param, out);
+    try {
+        std::vector<uint64_t> offs;
+        offs.push_back(0);
+        if (!host_line_starts(data, n, offs)) return CJX_ERR_ALLOC;
+        if (offs.back() < n) offs.push_back(n);
+        ChunkList out;
+        if (mode == 0) {
+            std::vector<uint64_t> li;
+            count_rule(offs.size() - 1, param, li);
+            for (size_t i = 0; i < param; i++)
+                out.emplace_back(offs[li[i]], offs[li[i + 1]] - offs[li[i]]);
+        } else {
+            bytes_rule([&](size_t j) { return offs[j]; }, offs.size(), n, param, out);
+        }
+        return emit_chunks(out, chunk_off, chunk_size, cap, n_chunks);
+    } catch (const std::bad_alloc &) {
+        return CJX_ERR_ALLOC;
     }
-    return emit_chunks(out, chunk_off, chunk_size, cap, n_chunks);
 }
+
+// Test hook (not in the public header): the threaded line scan of
+// cjx_chunk_lines acts as if only `k` threads could be created (k <= 0:
+// no limit), so tests can force its single-thread fallback.
+void cjx_debug_fail_threads(int k) { g_thread_limit.store(k); }
 
 namespace {
 

@@ -74,6 +74,10 @@ __global__ __launch_bounds__(kRecThreads) void k_rec_scan(uint32_t *__restrict__
                                                           LaunchState *__restrict__ ls) {
     __shared__ uint32_t wsum[kRecThreads / 64];
     __shared__ uint32_t carry;
+    if (ls->overflow) {  // k_rec_count wrote no counts: no records
+        if (threadIdx.x == 0) ls->n_records = 0;
+        return;
+    }
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (uint32_t base = 0; base < n_units; base += kRecThreads) {

@@ -86,7 +86,17 @@ __global__ __launch_bounds__(256) void k_values(const uint8_t *__restrict__ in, 
         uint64_t B = hl == N ? n : (uint64_t)(uint32_t)st[hl];  // through the next structural
         if (B > n) B = n;
         const uint64_t a0 = (A < B ? A : B) & ~15ull;
-        if (((uintptr_t)in & 15u) == 0 && B - a0 <= kValStage) {  // uniform over the block
+        // Every row the block's gaps use (h0 - 1 .. h0 + 255), clamped to the
+        // text as gap_of clamps it, must lie in [a0, B]: rows that are not
+        // ascending (caller-built rows, gapped regions) read the text in place.
+        bool outside = false;
+        if (h0 + threadIdx.x >= 1 && h0 + threadIdx.x - 1 < N) {
+            uint64_t v = (uint64_t)(uint32_t)st[h0 + threadIdx.x - 1];
+            if (v > n) v = n;
+            outside = v < a0 || v > B;
+        }
+        const bool any_outside = __syncthreads_or(outside) != 0;
+        if (!any_outside && ((uintptr_t)in & 15u) == 0 && B - a0 <= kValStage) {  // uniform
             const uint32_t words = (uint32_t)((B - a0 + 15) / 16);
             for (uint32_t w = threadIdx.x; w < words; w += 256) {
                 const uint64_t p = a0 + 16ull * w;

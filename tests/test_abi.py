@@ -214,3 +214,27 @@ def test_noalloc_entry_rejects_before_launching():
     assert "maximum" in cj.strerror(cj.CJX_ERR_TOO_LARGE)
     assert L.cjx_noalloc_max_bytes() == mx
 
+
+def test_chunk_lines_thread_fallback():
+    """The threaded line scan of cjx_chunk_lines, forced to find that only
+    one or three of its threads can be created: it joins the threads it
+    started, scans on the calling thread instead, and gives the loader rule's
+    chunks; nothing escapes into the C ABI."""
+    f = cj.lib().cjx_debug_fail_threads
+    f.argtypes = [ctypes.c_int]
+    f.restype = None
+    rng = np.random.default_rng(9)
+    lens = rng.integers(0, 3000, 50_000)
+    a = np.full(int(lens.sum()) + lens.size, ord("q"), np.uint8)
+    a[np.cumsum(lens + 1) - 1] = 10
+    assert a.size >= 64 << 20
+    want = oracle.chunks_by_bytes(a, 1 << 20)
+    try:
+        for k in (1, 3):
+            f(k)
+            offs, sizes = cj.chunk_lines(a, 1, 1 << 20)
+            assert list(zip(offs, sizes)) == want, k
+    finally:
+        f(0)
+    offs, sizes = cj.chunk_lines(a, 0, 333)
+    assert list(zip(offs, sizes)) == oracle.chunks_by_count(a, 333)

@@ -220,7 +220,7 @@ def test_chunked_host_input_fault(ctx, doc96):
     check_standard(ctx, d, label="chunked with a fault")
 
 
-@pytest.mark.parametrize("shift", [0, 3])
+@pytest.mark.parametrize("shift", [0, 3, 9, 15])
 def test_lines_host_span_placement(ctx, shift):
     """More than 16 chunks from host memory: the source span goes in as one
     copy and the device places every chunk at its aligned offset (source at

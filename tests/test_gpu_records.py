@@ -149,5 +149,26 @@ def test_records_over_dirty_device_memory(twitter_small):
         for cc in (3, 7, 51):
             check_records(c, twitter_small, oracle.chunks_by_count(twitter_small, cc),
                           f"dirty cc={cc}")
+        # rows that outgrow their guessed regions on the device: the launch
+        # reports CJX_ERR_CAPACITY and no records (not the previous counts)
+        dense = (b"[" + b"1," * 3000 + b"1]\n") * 40
+        ch = oracle.chunks_by_count(dense, 40)
+        offs, pos = [], 0
+        for _, size in ch:
+            offs.append(pos)
+            pos += -(-size // 16) * 16
+        buf = np.zeros(pos + 64, np.uint8)
+        for (a, size), off in zip(ch, offs):
+            buf[off:off + size] = np.frombuffer(dense[a:a + size], np.uint8)
+        dev = DeviceBuffer(buf.size)
+        dev.upload(buf)
+        cap = len(dense) - 1000  # below the exact bound: regions are the n/3 + 256 guess
+        st, pr = DeviceBuffer(cap * 4), DeviceBuffer(cap * 4)
+        rec = DeviceBuffer(10000 * cj.RECORD_DTYPE.itemsize)
+        assert c.parse_lines_device(dev.ptr, offs, [z for _, z in ch], st.ptr, pr.ptr, cap,
+                                    rec.ptr, 10000) == 0
+        rc, docs, N, M, over = c.batch_wait()
+        assert rc == cj.CJX_ERR_CAPACITY and over
+        assert c.last_records == 0
     finally:
         c.close()

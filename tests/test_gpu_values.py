@@ -235,3 +235,50 @@ def test_long_strings_and_unaligned_text(ctx):
 def test_bad_arguments(ctx):
     rc, _ = ctx.values_device(0, 10, 0, 0, 0)
     assert rc == cj.CJX_ERR_INPUT
+
+
+def test_rows_out_of_order(ctx):
+    """Caller-built rows that are not ascending (a block of 256 gaps with
+    rows that go back, and rows past the end of the text): the kernel reads
+    the text in place for such blocks and treats each gap as gap_of defines
+    it (a gap whose start lies after its end is empty at its end; rows past
+    the text are clamped to it), never the LDS stage of another span."""
+    data = bytes(gen.config1()[:200000])
+    data = data[:data.rindex(b",")] + b"]"
+    o = oracle.parse_standard(data)
+    n = len(data)
+    st = o.structural[1:o.N + 1].astype(np.int64).copy()
+    rng = np.random.default_rng(4)
+    for blk in (3, 17, 40):  # reverse runs inside some 256-gap blocks
+        a = blk * 256 + 10
+        st[a:a + 90] = st[a:a + 90][::-1]
+    idx = rng.integers(0, st.size, 60)
+    st[idx[:30]] = rng.integers(0, n, 30)         # rows anywhere in the text
+    st[idx[30:]] = n + rng.integers(1, 1 << 20, 30)  # rows past the end
+    rows = st.astype(np.uint32).view(np.int32)
+    N = rows.size
+    dev = DeviceBuffer(n + 64)
+    dev.upload(np.frombuffer(data, np.uint8))
+    d_st = DeviceBuffer(N * 4)
+    d_st.upload(rows)
+    vals = DeviceBuffer((N + 1) * cj.VALUE_DTYPE.itemsize)
+    strs = DeviceBuffer(n + 64)
+    rc, _ = ctx.values_device(dev.ptr, n, d_st.ptr, N, vals.ptr, strs.ptr)
+    assert rc == 0
+    got = np.frombuffer(vals.download(np.uint8, (N + 1) * cj.VALUE_DTYPE.itemsize).tobytes(),
+                        dtype=cj.VALUE_DTYPE)
+    bad = []
+    for h in range(N + 1):
+        b = n if h == N else min(int(st[h]) - 1, n)
+        a = 0 if h == 0 else min(int(st[h - 1]), b)
+        nxt = data[b] if h < N and b < n else None
+        t, off, ln, fl, v, u = expected_entry(data, a, b, nxt)
+        e = got[h]
+        if (int(e["type"]), int(e["text_off"]), int(e["text_len"]), int(e["row"])) != (t, off, ln, h):
+            bad.append((h, a, b, tuple(e.tolist()), (t, off, ln)))
+        elif t in (cj.CJX_VALUE_INT, cj.CJX_VALUE_FLOAT, cj.CJX_VALUE_KEY, cj.CJX_VALUE_STRING):
+            if int(e["v"]) != v or int(e["flags"]) != fl:
+                bad.append((h, a, b, tuple(e.tolist()), (t, off, ln, fl, v)))
+        if len(bad) >= 5:
+            break
+    assert not bad, bad
