@@ -40,6 +40,11 @@ __device__ __forceinline__ uint32_t wave_shr1(uint32_t v, uint32_t carry) {
     return dpp<kDppWaveShr1>(carry, v);
 }
 
+// Set bits of a wave-wide mask in the lanes below this one (v_mbcnt).
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 __device__ __forceinline__ uint32_t wave_read(uint32_t x, int lane) {
     return (uint32_t)__builtin_amdgcn_readlane((int)x, lane);
 }

@@ -370,6 +370,8 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
     uint32_t &utf8_bad = ws.utf8_bad;
     uint32_t &lane_cnt0 = ws.lane_cnt0, &lane_all = ws.lane_all;
     c_run = s_run = seen = dflag = utf8_bad = lane_cnt0 = lane_all = 0;
+    // per-lane counts over the rounds (each popcount accumulates in one op)
+    uint32_t cnt_op0 = 0, cnt_br0 = 0, cnt_op = 0, cnt_br = 0;
 
 #pragma unroll
     for (int i = 0; i < kTokRounds; i++) {
@@ -430,16 +432,26 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
         const uint32_t run = (uint32_t)__clz(~bs);  // trailing backslash run of the lane
         const uint64_t NB = __ballot(!all_bs);
         const uint64_t TP = __ballot(run & 1u);
-        const uint64_t mm = NB & lt;
-        const uint32_t ov = mm ? (uint32_t)((TP >> (63 - __clzll(mm))) & 1ull) : c_run;
+        // Parity of the backslash run that ends just before the lane: the
+        // trailing run of the nearest lower lane that is not all backslashes.
+        // When no lane of the round is all backslashes (a uniform test) that
+        // is the previous lane's, one lane shift.
+        uint32_t ov;
+        if (NB == ~0ull) {
+            ov = wave_shr1(run & 1u, c_run);
+        } else {
+            const uint64_t mm = NB & lt;
+            ov = mm ? (uint32_t)((TP >> (63 - __clzll(mm))) & 1ull) : c_run;
+        }
         const uint32_t rq = unescaped32(bs, q, ov);
         const uint32_t px = prefix_xor32(rq);
         const uint64_t QP = __ballot(__popc(rq) & 1u);
-        const uint32_t s_lane = s_run ^ (uint32_t)(__popcll(QP & lt) & 1);
+        const uint32_t s_lane = s_run ^ (mbcnt64(QP) & 1u);
         instr_r[i] = px ^ (s_lane ? ~0u : 0u);
-        lane_cnt0 += (uint32_t)__popc(op_r[i] & ~instr_r[i]) |
-                     ((uint32_t)__popc(br_r[i] & ~instr_r[i]) << 16);
-        lane_all += (uint32_t)__popc(op_r[i]) | ((uint32_t)__popc(br_r[i]) << 16);
+        cnt_op0 += (uint32_t)__popc(op_r[i] & ~instr_r[i]);
+        cnt_br0 += (uint32_t)__popc(br_r[i] & ~instr_r[i]);
+        cnt_op += (uint32_t)__popc(op_r[i]);
+        cnt_br += (uint32_t)__popc(br_r[i]);
 
         if (!seen && NB) {  // first non-backslash byte of the wave: is it a quote?
             const uint32_t k0 = (uint32_t)__ffsll((long long)NB) - 1;
@@ -452,7 +464,8 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
         if (NB) c_run = (uint32_t)((TP >> (63 - __clzll(NB))) & 1ull);
         s_run ^= (uint32_t)(__popcll(QP) & 1);
     }
-
+    lane_cnt0 = cnt_op0 | (cnt_br0 << 16);
+    lane_all = cnt_op | (cnt_br << 16);
 }
 
 // Large units (SMALL = false): one workgroup per 64 KiB tile; tiles are taken
@@ -637,20 +650,32 @@ __global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
         const uint32_t rn = rtot & 0xFFFFu;
         const uint32_t nlm = want_nl ? (s_b5[w][i][lane] & ~br_r[i]) : 0u;
         if (rn <= (uint32_t)kStageRows && (uint64_t)nb + rn <= cap_end) {
-            uint32_t k = excl & 0xFFFFu;
+            // The lane's structurals go to slots [k, kh] of the stage, taken
+            // from both ends of its mask at once (lowest bit to slot k, highest
+            // to slot kh), so a wave runs half as many steps as its densest
+            // lane has structurals.  A last single bit is both ends: the same
+            // value to the same slot twice.
+            uint32_t k = excl & 0xFFFFu, kh = k + (cnt & 0xFFFFu) - 1u;
+            const uint32_t posh = pos0 + 31u;
             if (want_nl) {  // and the row's newline flag (in s_bchar, free until the brackets)
                 while (m) {
-                    const uint32_t j = (uint32_t)__builtin_ctz(m);
-                    m &= m - 1;
-                    s_stage[w][k] = pos0 + j;
-                    s_bchar[w][k] = (uint8_t)((nlm >> j) & 1u);
+                    const uint32_t jl = (uint32_t)__builtin_ctz(m), jz = (uint32_t)__builtin_clz(m);
+                    s_stage[w][k] = pos0 + jl;
+                    s_stage[w][kh] = posh - jz;
+                    s_bchar[w][k] = (uint8_t)((nlm >> jl) & 1u);
+                    s_bchar[w][kh] = (uint8_t)((nlm >> (31u - jz)) & 1u);
+                    m = (m & (m - 1u)) & ~(0x80000000u >> jz);
                     k++;
+                    kh--;
                 }
             } else {
                 while (m) {
-                    const uint32_t j = (uint32_t)__builtin_ctz(m);
-                    m &= m - 1;
-                    s_stage[w][k++] = pos0 + j;
+                    const uint32_t jl = (uint32_t)__builtin_ctz(m), jz = (uint32_t)__builtin_clz(m);
+                    s_stage[w][k] = pos0 + jl;
+                    s_stage[w][kh] = posh - jz;
+                    m = (m & (m - 1u)) & ~(0x80000000u >> jz);
+                    k++;
+                    kh--;
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -691,21 +716,34 @@ __global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
         const uint32_t b1 = s_b1[w][i][lane], b5 = s_b5[w][i][lane];
         const uint32_t rm = rtot >> 16;
         if (rn <= (uint32_t)kStageRows && (uint64_t)mb + rm <= cap_end) {
-            uint32_t k = excl >> 16;
+            // Both ends of the bracket mask at once, as for the structurals.
+            // A stage entry holds the bracket's structural row within the
+            // round (below 2^11) and its (open, curly) code in bits 16-17; the
+            // row index and the character are formed when the run is stored.
+            uint32_t k = excl >> 16, kh = k + (cnt >> 16) - 1u;
+            const uint32_t r0 = excl & 0xFFFFu;  // the lane's first row in the round
             while (m) {
-                const uint32_t j = (uint32_t)__builtin_ctz(m);
-                m &= m - 1;
-                const uint32_t sel = ((b1 >> j) & 1u) | (((b5 >> j) & 1u) << 1);
-                s_stage[w][k] = idx0 + (uint32_t)__popc(sti & ((1u << j) - 1u));
-                s_bchar[w][k] = (uint8_t)perm(0u, 0x7B7D5B5Du, sel);
+                const uint32_t jl = (uint32_t)__builtin_ctz(m), jz = (uint32_t)__builtin_clz(m);
+                const uint32_t jh = 31u - jz;
+                const uint32_t el = ((uint32_t)__popc(sti & ((1u << jl) - 1u)) + r0) +
+                                    (__builtin_amdgcn_ubfe(b1, jl, 1u) << 16) +
+                                    (__builtin_amdgcn_ubfe(b5, jl, 1u) << 17);
+                const uint32_t eh = ((uint32_t)__popc(sti & ((1u << jh) - 1u)) + r0) +
+                                    (__builtin_amdgcn_ubfe(b1, jh, 1u) << 16) +
+                                    (__builtin_amdgcn_ubfe(b5, jh, 1u) << 17);
+                s_stage[w][k] = el;
+                s_stage[w][kh] = eh;
+                m = (m & (m - 1u)) & ~(0x80000000u >> jz);
                 k++;
+                kh--;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             for (uint32_t t = lane; t < rm; t += 64) {
-                oc_idx[mb + t] = s_stage[w][t];
-                oc_char[mb + t] = s_bchar[w][t];
+                const uint32_t e = s_stage[w][t];
+                oc_idx[mb + t] = nb + (e & 0xFFFFu);
+                oc_char[mb + t] = (uint8_t)perm(0u, 0x7B7D5B5Du, e >> 16);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
