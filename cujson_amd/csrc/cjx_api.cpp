@@ -293,7 +293,9 @@ int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsign
     if (u16 && !ctx->tmp.ensure(sh.rows * 4 + 16)) return CJX_ERR_ALLOC;
     LaunchBufs b = bufs_of(ctx);
     hipStream_t s = ctx->stream;
-    if (launch_reset(s, b, sh.ntiles, sh.n_units) != hipSuccess) return CJX_ERR_DEVICE;
+    if ((ctx->one_unit_dev ? launch_reset_one(s, b, ctx->one_unit, sh.ntiles, sh.n_small)
+                           : launch_reset(s, b, sh.ntiles, sh.n_units)) != hipSuccess)
+        return CJX_ERR_DEVICE;
     hipEvent_t *tev = ctx->tev[ctx->timing_count % cjx_context::kTimingRing];
     if (ctx->timing) (void)hipEventRecord(tev[0], s);
     const bool strict = (flags & CJX_FLAG_UTF8_STRICT) != 0;
@@ -411,7 +413,8 @@ int enqueue_maybe_graph(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, 
                         uint32_t *d_struct, int32_t *d_pair) {
     if (!(flags & CJX_FLAG_GRAPH)) return enqueue_parse(ctx, d_in, sh, flags, d_struct, d_pair);
     if ((flags & CJX_FLAG_DEPTH_U16) && !ctx->tmp.ensure(sh.rows * 4 + 16)) return CJX_ERR_ALLOC;
-    cjx_context::GraphKey key{};
+    cjx_context::GraphKey key;
+    std::memset(&key, 0, sizeof(key));  // padding too: keys are compared bytewise
     key.b = bufs_of(ctx);
     key.d_in = d_in;
     key.d_struct = d_struct;
@@ -424,6 +427,8 @@ int enqueue_maybe_graph(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, 
     key.chunks = sh.chunks;
     key.flags = flags;
     key.concat = sh.concat;
+    key.one_unit_dev = ctx->one_unit_dev;
+    if (ctx->one_unit_dev) key.one_unit = ctx->one_unit;
     if (!ctx->graph || std::memcmp(&key, &ctx->graph_key, sizeof(key)) != 0) {
         if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
         ctx->graph = nullptr;
@@ -1445,11 +1450,19 @@ int cjx_parse_device(cjx_context *ctx, const uint8_t *d_input, size_t n, unsigne
     Shape sh{ntiles, 1, n_small, 0, 0, 0, 0, false};
     shape_rows(sh, units);
     if ((rc = ensure_work(ctx, sh))) return rc;
-    if ((rc = upload_units(ctx, units, ntiles, n_small))) return rc;
+    // The reset kernel writes the one-unit table from its arguments: no
+    // upload, so no copy-engine round trip before the launch.
+    if (!ctx->units.ensure(sizeof(UnitDesc) + ((size_t)ntiles + 1) * 4 + 16)) return CJX_ERR_ALLOC;
+    ctx->tile_unit = reinterpret_cast<uint32_t *>(ctx->units.as<uint8_t>() + sizeof(UnitDesc));
+    ctx->small_units = ctx->tile_unit + ntiles;
+    ctx->unit_src = nullptr;
+    ctx->one_unit = units[0];
+    ctx->one_unit_dev = true;
     ctx->n_units = 1;
     (void)hipEventRecord(ctx->ev0, ctx->stream);
     rc = enqueue_maybe_graph(ctx, d_input, sh, flags & ~CJX_FLAG_JSONL,
                        reinterpret_cast<uint32_t *>(d_structural), d_pair);
+    ctx->one_unit_dev = false;
     (void)hipEventRecord(ctx->ev1, ctx->stream);
     ctx->events_recorded = true;
     return rc;

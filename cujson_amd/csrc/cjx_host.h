@@ -119,6 +119,10 @@ struct cjx_context {
     PinnedBuf load_buf[kLoadBufs];
     hipEvent_t load_ev[kLoadBufs] = {};
     std::vector<uint64_t> batch_off;  // byte offsets of the documents of the batch in flight
+    // One-unit device launches (cjx_parse_device): the unit table is written
+    // by the reset kernel from this descriptor instead of being uploaded.
+    bool one_unit_dev = false;
+    cjx::UnitDesc one_unit{};
     // Shape of the launch in flight (device form).
     uint32_t n_units = 0;
     bool events_recorded = false;
@@ -133,6 +137,8 @@ struct cjx_context {
         uint64_t rows, max_rows, chunks;
         unsigned flags;
         bool concat;
+        bool one_unit_dev;
+        cjx::UnitDesc one_unit;
     } graph_key{};
 };
 

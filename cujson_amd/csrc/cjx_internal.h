@@ -41,7 +41,7 @@ constexpr int kMatchChunk = 4096;
 constexpr int kMatchThreads = 256;
 constexpr int kMatchKeys = 256;
 constexpr int kMatchMaxDepth = 256;  // deeper units take the sort path
-constexpr int kSmallBrackets = 64;   // units with at most this many brackets: k_match_small
+constexpr int kSmallBrackets = 1024;  // units with at most this many brackets: one wave matches them
 
 // Grid of the sort-path kernels of one unit (phase3_blocks: depth walk,
 // keys, histogram, scatter over 64 Ki brackets per block) and of its pair
@@ -158,6 +158,10 @@ struct LaunchBufs {
 };
 
 hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uint32_t n_units);
+// launch_reset for one unit, whose table (descriptor, tile map, small-unit
+// entry) the kernel writes from its argument.
+hipError_t launch_reset_one(hipStream_t s, const LaunchBufs &b, const UnitDesc &u, uint32_t ntiles,
+                            uint32_t n_small);
 hipError_t launch_rows_out(hipStream_t s, const uint32_t *src, uint32_t *host_dst, uint64_t count);
 // Units of at most kWaveBytes bytes are "small": no tiles, one wave each
 // (listed in small_units); the others are cut into kTileBytes tiles.

@@ -26,6 +26,24 @@ __global__ void k_reset(unsigned long long *tile_st, uint32_t ntiles, LaunchStat
     if (i0 == 0) *ls = LaunchState{};
 }
 
+// k_reset_one: k_reset for a launch of one unit whose descriptor comes as a
+// kernel argument: also writes the unit table (the descriptor, its tile ->
+// unit map and small-unit entry, all unit 0), so the launch needs no
+// host-to-device copy before it.
+__global__ void k_reset_one(UnitDesc u, UnitDesc *units, uint32_t *tile_unit, uint32_t ntiles,
+                            uint32_t n_small, unsigned long long *tile_st, LaunchState *ls,
+                            UnitOut *uo) {
+    const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = i0; i < 2 * ntiles; i += stride) tile_st[i] = 0;
+    for (uint32_t i = i0; i < ntiles + n_small; i += stride) tile_unit[i] = 0;
+    if (i0 == 0) {
+        units[0] = u;
+        uo[0] = UnitOut{};
+        *ls = LaunchState{};
+    }
+}
+
 // Per-unit bases: pair_base offsets unit u's pair values by the structural
 // count of all earlier units when results are concatenated (the JSON Lines
 // lastStructuralIndex carry, parse_json_lines.cu:1222); row_base is that
@@ -747,6 +765,16 @@ hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uin
     if (grid > 4096) grid = 4096;
     hipLaunchKernelGGL(k_reset, dim3(grid), dim3(256), 0, s, b.tile_st, ntiles, b.ls, b.uout,
                        n_units);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_one(hipStream_t s, const LaunchBufs &b, const UnitDesc &u, uint32_t ntiles,
+                           uint32_t n_small) {
+    uint32_t grid = (2 * ntiles + 255) / 256;
+    if (grid < 1) grid = 1;
+    if (grid > 4096) grid = 4096;
+    hipLaunchKernelGGL(k_reset_one, dim3(grid), dim3(256), 0, s, u, b.units, b.tile_unit, ntiles,
+                       n_small, b.tile_st, b.ls, b.uout);
     return hipGetLastError();
 }
 

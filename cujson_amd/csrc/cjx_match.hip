@@ -487,17 +487,25 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_local(
     }
 }
 
-// Units with at most kSmallBrackets brackets, one wave per unit, lane l
-// holding bracket l: the depth walk by a wave scan, the well-nested test, and
-// the matching by ballots (an opener's closer is the next bracket of its
-// key).  The unit's pair row is written densely through a 64-cell LDS window.
-// A unit that is not well nested keeps fast = 0 and goes to the sort path.
+// Units with at most kSmallBrackets (1024) brackets, one wave per unit.
+// Lane l of group g holds bracket 64g + l.  The depth walk is a wave scan
+// per group with a carry; a unit whose walk leaves [0, 256] or does not end
+// at 0 is not well nested and keeps fast = 0 (the sort path takes it).
+// Otherwise the keys are exact depths and an opener's closer is the next
+// bracket of its key.  The groups are visited from the last to the first:
+// within a group, the lanes of one key are found with eight ballots (one per
+// key bit); the first later bracket of a key outside the group is kept per
+// key in LDS (its row and character), updated by each group's first lane of
+// that key.  The unit's pair-row cells are filled with -1 (16-byte stores)
+// and the openers' values are stored over them.
+constexpr int kWaveGroups = kSmallBrackets / 64;
 __device__ __forceinline__ void match_small_block(uint32_t bid, const uint8_t *__restrict__ oc_char,
                                                      const uint32_t *__restrict__ oc_idx,
                                                      UnitOut *__restrict__ uo, uint32_t n_units,
                                                      int32_t *__restrict__ pair_row,
                                                      const LaunchState *__restrict__ ls) {
-    __shared__ int32_t s_win[4][64];
+    __shared__ uint32_t s_nrow[4][kMatchKeys];  // per key: row of the next bracket after the group
+    __shared__ uint8_t s_nch[4][kMatchKeys];    // and its character
     if (ls->overflow) return;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t u = bid * 4 + w;
@@ -505,40 +513,79 @@ __device__ __forceinline__ void match_small_block(uint32_t bid, const uint8_t *_
     const uint32_t mb = uo[u].m_base, M = uo[u].m_end - mb;
     if (M > (uint32_t)kSmallBrackets) return;  // the chunk kernels
     const uint32_t nb = uo[u].n_base, ne = uo[u].n_end, pbase = uo[u].pair_base + 1u;
-    const bool has = lane < M;
-    const uint32_t c = has ? oc_char[mb + lane] : 0u;
-    const uint32_t row = has ? oc_idx[mb + lane] : 0u;
-    const int32_t d = has ? delta_of((uint8_t)c) : 0;
-    const int32_t incl = (int32_t)wave_incl_scan((uint32_t)d);  // depth after the bracket
-    const bool bad = has && (incl < 0 || incl > kMatchMaxDepth);
-    if (__ballot(bad) || __shfl(incl, 63, 64) != 0) return;  // not well nested
-    const uint32_t key = (uint32_t)(d > 0 ? incl - 1 : incl);  // exact depth, in [0, 255]
-    uint64_t peers = __ballot(has);
+    const uint32_t G = (M + 63) / 64;
+    uint32_t ch[kWaveGroups], row[kWaveGroups], key[kWaveGroups];
+    int32_t carry = 0;
+    bool bad = false;
 #pragma unroll
-    for (int bit = 0; bit < 8; bit++) {
-        const uint32_t bv = (key >> bit) & 1u;
-        const uint64_t bm = __ballot(bv);
-        peers &= bv ? bm : ~bm;
+    for (int g = 0; g < kWaveGroups; g++) {
+        ch[g] = row[g] = key[g] = 0;
+        if ((uint32_t)g >= G) continue;
+        const uint32_t e = 64u * g + lane;
+        const bool has = e < M;
+        ch[g] = has ? oc_char[mb + e] : 0u;
+        row[g] = has ? oc_idx[mb + e] : 0u;
+        const int32_t d = has ? delta_of((uint8_t)ch[g]) : 0;
+        const int32_t incl = carry + (int32_t)wave_incl_scan((uint32_t)d);  // depth after it
+        carry = __shfl(incl, 63, 64);
+        bad = bad || (has && (incl < 0 || incl > kMatchMaxDepth));
+        key[g] = (uint32_t)(d > 0 ? incl - 1 : incl) & 0xFFu;  // exact depth, in [0, 255]
     }
-    const uint64_t after = lane == 63 ? 0ull : peers & ~((2ull << lane) - 1ull);
-    const uint32_t partner = after ? (uint32_t)__ffsll((long long)after) - 1u : lane;
-    const uint32_t c2 = (uint32_t)__shfl((int)c, (int)partner, 64);
-    const uint32_t row2 = (uint32_t)__shfl((int)row, (int)partner, 64);
-    const bool opener = has && d > 0;
-    const bool err = opener && (c ^ c2) != 6u;
-    for (uint32_t base = nb; base < ne; base += 64) {
-        s_win[w][lane] = -1;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (opener && row >= base && row < base + 64)
-            s_win[w][row - base] = (int32_t)(row2 - nb + pbase);
+    if (__ballot(bad) || carry != 0) return;  // not well nested
+    for (uint32_t k = lane; k < (uint32_t)kMatchKeys; k += 64) s_nrow[w][k] = 0;
+    int32_t pval[kWaveGroups];
+    bool err = false;
+#pragma unroll
+    for (int g = kWaveGroups - 1; g >= 0; g--) {
+        pval[g] = -1;
+        if ((uint32_t)g >= G) continue;
+        const uint32_t e = 64u * g + lane;
+        const bool has = e < M;
+        uint64_t peers = __ballot(has);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            const uint32_t bv = (key[g] >> bit) & 1u;
+            const uint64_t bm = __ballot(bv);
+            peers &= bv ? bm : ~bm;
+        }
+        const uint64_t after = lane == 63 ? 0ull : peers & ~((2ull << lane) - 1ull);
+        const uint32_t partner = after ? (uint32_t)__ffsll((long long)after) - 1u : lane;
+        uint32_t c2 = (uint32_t)__shfl((int)ch[g], (int)partner, 64);
+        uint32_t row2 = (uint32_t)__shfl((int)row[g], (int)partner, 64);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (base + lane < ne) pair_row[base + lane] = s_win[w][lane];
+        if (!after) {  // the next bracket of the key lies in a later group
+            c2 = s_nch[w][key[g]];
+            row2 = s_nrow[w][key[g]];
+        }
+        const bool opener = has && (ch[g] & 2u);
+        if (opener) {
+            pval[g] = (int32_t)(row2 - nb + pbase);
+            if ((ch[g] ^ c2) != 6u) err = true;
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // the group's first lane of each key is the next bracket of that key
+        // for the groups before it
+        if (has && (peers & ((1ull << lane) - 1ull)) == 0) {
+            s_nrow[w][key[g]] = row[g];
+            s_nch[w][key[g]] = (uint8_t)ch[g];
+        }
     }
+    // Pair-row cells [nb, ne): -1, then the openers' values.
+    const bool vec = (reinterpret_cast<uintptr_t>(pair_row) & 15u) == 0;
+    const uint32_t a0 = vec ? min(ne, (nb + 3u) & ~3u) : ne;
+    const uint32_t a1 = vec ? max(a0, ne & ~3u) : a0;
+    for (uint32_t t = nb + lane; t < a0; t += 64) pair_row[t] = -1;
+    for (uint32_t t = a0 + 4u * lane; t < a1; t += 256)
+        *reinterpret_cast<int4 *>(pair_row + t) = make_int4(-1, -1, -1, -1);
+    for (uint32_t t = a1 + lane; t < ne; t += 64) pair_row[t] = -1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // the fill before the values
+#pragma unroll
+    for (int g = 0; g < kWaveGroups; g++)
+        if (pval[g] >= 0) pair_row[row[g]] = pval[g];
     const uint64_t any_err = __ballot(err);
     if (lane == 0) {
         if (any_err) atomicOr(&uo[u].struct_err, 1u);

@@ -314,6 +314,34 @@ def test_nested_depth_limits(ctx, max_depth):
             check_standard(ctx, doc, oracle.FLAG_DEPTH_U16, label=f"u16 d<={max_depth} n={n}")
 
 
+@pytest.mark.parametrize("max_depth", [2, 40, 255, 256, 257])
+def test_nested_wave_units(ctx, max_depth):
+    """Units of 1 to 1100 brackets (one wave matches those of at most 1024,
+    in groups of 64; deeper or larger ones go to the sort or chunk path):
+    random well-nested documents at the depth limits, with and without
+    planted type mismatches, alone and in a batch."""
+    rng = random.Random(100 + max_depth)
+    docs = []
+    for n in (1, 2, 63, 64, 65, 100, 127, 128, 129, 500, 1000, 1023, 1024, 1025, 1100):
+        for bad in (0, 1):
+            d = nested_doc(rng, n, max_depth, bad if n > 2 else 0)
+            docs.append(d)
+            check_standard(ctx, d, label=f"wave unit n={n} d<={max_depth} bad={bad}")
+            if max_depth >= 255:
+                check_standard(ctx, d, oracle.FLAG_DEPTH_U16, label=f"wave u16 n={n}")
+    offs, pos = [], 0
+    for d in docs:
+        offs.append(pos)
+        pos += len(d)
+    data = np.frombuffer(b"".join(docs), np.uint8)
+    r = cj.parse_batch(data, offs, [len(d) for d in docs], 0, ctx)
+    assert r.rc == 0
+    for i, d in enumerate(docs):
+        want = oracle.parse_standard(d)
+        err, buf = r.doc(i)
+        assert err == want.code and np.array_equal(buf, want.buf), (max_depth, i)
+
+
 def test_many_match_chunks(ctx):
     """More than 4096 chunks of 4096 brackets under one top-level array."""
     rng = random.Random(7)
