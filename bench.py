@@ -666,6 +666,29 @@ def main():
         form, ms_per_step, step_ms = "graph", g_ms_per_step, g_step_ms
     value = n / (ms_per_step * 1e-3) / 1e9
 
+    # The same parse in strict UTF-8 mode (RFC 3629, CJX_FLAG_UTF8_STRICT),
+    # launched: the strict check replaces the reference's weaker rule.
+    def strict_step():
+        rc = ctx.parse_device(d_in.ptr, n, d_st.ptr, d_pr.ptr, cap, cj.CJX_FLAG_UTF8_STRICT)
+        if rc != 0:
+            raise RuntimeError(f"parse_device (strict): {cj.strerror(rc)}")
+    strict_step()
+    rc, N4, M4, over = ctx.wait()
+    assert rc == 0 and N4 == N and M4 == M and not over
+    ctx.set_timing(True)
+    s_steps = max(3, min(args.steps, 10))
+    sevs = [Event() for _ in range(2)]
+    synchronize()
+    sevs[0].record(stream)
+    for _ in range(s_steps):
+        strict_step()
+    sevs[1].record(stream)
+    synchronize()
+    s_tim = ctx.timings(s_steps)
+    ctx.set_timing(False)
+    strict_ms = sevs[0].elapsed_ms(sevs[1]) / s_steps
+    strict_front_ms = float(np.mean([t[0] for t in s_tim]))  # strict check + tokenizer
+
     tok_bytes = n + 4 * N + 5 * M          # k_tokenize: read input, write rows it owns
     path_bytes = n + 8 * (N + 2)            # the whole path's algorithmic bytes
     achieved = tok_bytes / (tok_ms * 1e-3) / 1e9
@@ -713,6 +736,11 @@ def main():
             "wall_ms_per_step": round((wall1 - wall0) * 1e3 / args.steps, 4),
             "launched_ms_per_step": round(total_ms / args.steps, 4),
             "graph_ms_per_step": round(g_ms_per_step, 4),
+            "strict_utf8_ms_per_step": round(strict_ms, 4),
+            "strict_utf8_check_and_tokenize_ms": round(strict_front_ms, 4),
+            "strict_utf8_form": "CJX_FLAG_UTF8_STRICT (RFC 3629) instead of the reference's "
+                                "per-word rule, launched; the second figure is the strict "
+                                "check plus the tokenizer",
         },
     }
     if not args.no_host_path:

@@ -63,6 +63,71 @@ __device__ __forceinline__ void agent_store64(unsigned long long *p, unsigned lo
 }
 
 // ---------------------------------------------------------------------------
+// Byte classes by three v_perm table lookups over a word's four bytes (the
+// tokenizer and the strict UTF-8 check): lookups indexed by byte bits 0-2,
+// 3-5 and 6-7 are ANDed; every class bit is a product set in those three
+// coordinates, so the AND is exact.  Class bits of the tokenizer's tables:
+//   bit 0 ':' (and '\n' for JSON Lines)   bit 4 '\\'
+//   bit 1 CONT (0x80-0xBF)                 bit 5 E    (>= 0xE0)
+//   bit 2 '[' ']' '{' '}'                  bit 6 F    (>= 0xF0)
+//   bit 3 ','                              bit 7 '"'
+// ---------------------------------------------------------------------------
+// Table entries, byte i = value for index i.
+constexpr uint64_t kLutBits02 = 0x6262667A66E36262ull;        // by bits 0-2
+constexpr uint64_t kLutBits35Std = 0x67622AA216020202ull;     // by bits 3-5
+constexpr uint64_t kLutBits35Lines = 0x67622AA216020302ull;   // '\n' joins ':'
+constexpr uint64_t kLutBits67 = 0x0000000060021489ull;        // by bits 6-7
+
+// v_perm: byte i of the result is byte sel_i of hi:lo (sel_i < 8); the host
+// pass models it for the host runs of the classifiers in the tests (only
+// table selectors 0-7 and 12 (zero) occur in these uses).
+__host__ __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_perm(hi, lo, sel);
+#else
+    const uint64_t data = ((uint64_t)hi << 32) | lo;
+    uint32_t out = 0;
+    for (int i = 0; i < 4; i++) {
+        const uint32_t k = (sel >> (8 * i)) & 0xFFu;
+        const uint32_t v = k < 8 ? (uint32_t)(data >> (8 * k)) & 0xFFu : (k == 12 ? 0u : 0xFFu);
+        out |= v << (8 * i);
+    }
+    return out;
+#endif
+}
+
+template <bool JSONL>
+__host__ __device__ __forceinline__ uint32_t classify(uint32_t r) {
+    constexpr uint64_t t35 = JSONL ? kLutBits35Lines : kLutBits35Std;
+    const uint32_t a = perm((uint32_t)(kLutBits02 >> 32), (uint32_t)kLutBits02, r & 0x07070707u);
+    const uint32_t b = perm((uint32_t)(t35 >> 32), (uint32_t)t35, (r >> 3) & 0x07070707u);
+    const uint32_t c = perm(0u, (uint32_t)kLutBits67, (r >> 6) & 0x03030303u);
+    return a & b & c;
+}
+
+// 4x4 byte transpose: o_k = [a.k, b.k, c.k, e.k].
+__host__ __device__ __forceinline__ void transpose4(uint32_t a, uint32_t b, uint32_t c, uint32_t e,
+                                           uint32_t &o0, uint32_t &o1, uint32_t &o2,
+                                           uint32_t &o3) {
+    const uint32_t ab_lo = perm(b, a, 0x05010400u), ab_hi = perm(b, a, 0x07030602u);
+    const uint32_t ce_lo = perm(e, c, 0x05010400u), ce_hi = perm(e, c, 0x07030602u);
+    o0 = perm(ce_lo, ab_lo, 0x05040100u);
+    o1 = perm(ce_lo, ab_lo, 0x07060302u);
+    o2 = perm(ce_hi, ab_hi, 0x05040100u);
+    o3 = perm(ce_hi, ab_hi, 0x07060302u);
+}
+
+// Bytes 3-6 of hi:lo, i.e. (hi << 8) | (lo >> 24): the previous word's top
+// byte shifted in below.
+__host__ __device__ __forceinline__ uint32_t alignbyte3(uint32_t hi, uint32_t lo) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_alignbyte(hi, lo, 3u);
+#else
+    return (hi << 8) | (lo >> 24);
+#endif
+}
+
+// ---------------------------------------------------------------------------
 // Escape / string carry as an affine map over GF(2).
 // A span of input maps the state (c = parity of the backslash run ending at
 // its start, s = in-string parity at its start) to the state at its end:

@@ -31,38 +31,6 @@
 
 namespace cjx {
 
-// Table entries, byte i = value for index i.
-constexpr uint64_t kLutBits02 = 0x6262667A66E36262ull;        // by bits 0-2
-constexpr uint64_t kLutBits35Std = 0x67622AA216020202ull;     // by bits 3-5
-constexpr uint64_t kLutBits35Lines = 0x67622AA216020302ull;   // '\n' joins ':'
-constexpr uint64_t kLutBits67 = 0x0000000060021489ull;        // by bits 6-7
-
-
-__device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) {
-    return __builtin_amdgcn_perm(hi, lo, sel);
-}
-
-template <bool JSONL>
-__device__ __forceinline__ uint32_t classify(uint32_t r) {
-    constexpr uint64_t t35 = JSONL ? kLutBits35Lines : kLutBits35Std;
-    const uint32_t a = perm((uint32_t)(kLutBits02 >> 32), (uint32_t)kLutBits02, r & 0x07070707u);
-    const uint32_t b = perm((uint32_t)(t35 >> 32), (uint32_t)t35, (r >> 3) & 0x07070707u);
-    const uint32_t c = perm(0u, (uint32_t)kLutBits67, (r >> 6) & 0x03030303u);
-    return a & b & c;
-}
-
-// 4x4 byte transpose: o_k = [a.k, b.k, c.k, e.k].
-__device__ __forceinline__ void transpose4(uint32_t a, uint32_t b, uint32_t c, uint32_t e,
-                                           uint32_t &o0, uint32_t &o1, uint32_t &o2,
-                                           uint32_t &o3) {
-    const uint32_t ab_lo = perm(b, a, 0x05010400u), ab_hi = perm(b, a, 0x07030602u);
-    const uint32_t ce_lo = perm(e, c, 0x05010400u), ce_hi = perm(e, c, 0x07030602u);
-    o0 = perm(ce_lo, ab_lo, 0x05040100u);
-    o1 = perm(ce_lo, ab_lo, 0x07060302u);
-    o2 = perm(ce_hi, ab_hi, 0x05040100u);
-    o3 = perm(ce_hi, ab_hi, 0x07060302u);
-}
-
 // 8x8 bit-matrix transpose in every byte slot: afterwards t[K] holds, in
 // byte b, bit d = bit K of byte b of the input t[d].  With transposed input
 // words (byte b of word d = input byte 8b+d) t[K] is the linear 32-bit mask
@@ -111,9 +79,6 @@ __device__ __forceinline__ uint32_t prefix_xor32(uint32_t x) {
     return x;
 }
 
-__device__ __forceinline__ uint32_t alignbyte3(uint32_t hi, uint32_t lo) {
-    return __builtin_amdgcn_alignbyte(hi, lo, 3u);
-}
 
 // `inside`: the caller knows the wave's whole range lies inside the unit
 // (a uniform condition), so no bound checks are needed.

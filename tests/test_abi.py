@@ -238,3 +238,49 @@ def test_chunk_lines_thread_fallback():
         f(0)
     offs, sizes = cj.chunk_lines(a, 0, 333)
     assert list(zip(offs, sizes)) == oracle.chunks_by_count(a, 333)
+
+
+def test_strict_utf8_lead_table():
+    """The strict UTF-8 check's lead table (host model of the same v_perm
+    lookups) against its class definitions, for all 256 bytes."""
+    f = cj.lib().cjx_debug_lead_class
+    f.argtypes = [ctypes.c_uint32]
+    f.restype = ctypes.c_uint32
+    for b in range(256):
+        want = ((b in (0xC0, 0xC1)) | ((0xF5 <= b <= 0xF7) << 1) | ((b >= 0xF8) << 2) |
+                ((b == 0xF4) << 3) | ((b == 0xF0) << 4) | ((b == 0xE0) << 5) |
+                ((b == 0xED) << 6) | ((b >= 0xC0) << 7))
+        assert f(b) == want, (hex(b), hex(f(b)), hex(want))
+
+
+def test_strict_utf8_word_steps_against_oracle():
+    """Host run of the strict check's word-level steps (transposed words,
+    both classifications, the carry between 16-byte lanes) against the
+    oracle's RFC 3629 validation: every short sequence at every offset of a
+    48-byte window, and random byte strings."""
+    f = cj.lib().cjx_debug_strict_utf8
+    f.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    f.restype = ctypes.c_int
+    seqs = [b"\xc3\xa9", b"\xe2\x82\xac", b"\xf0\x9f\x98\x80", b"\xf4\x8f\xbf\xbf", b"\xc3",
+            b"\xe2\x82", b"\xf0\x9f\x98", b"\xc0\xaf", b"\xc1\xbf", b"\xe0\x80\xaf",
+            b"\xe0\xa0\x80", b"\xed\xa0\x80", b"\xed\x9f\xbf", b"\xf0\x80\x80\x80",
+            b"\xf0\x90\x80\x80", b"\xf4\x90\x80\x80", b"\xf5\x80", b"\xff", b"\x80", b"\xbf\xbf",
+            b"\xc3\xa9\xa9", b"\xe2\x82\xac\x80", b"\xdf\xbf", b"\xef\xbf\xbf", b"\xf8\x88\x80\x80"]
+    for sq in seqs:
+        for off in range(0, 48 - len(sq)):
+            for tail in (0, 1, 5):
+                d = b"a" * off + sq + b"b" * tail
+                assert f(d, len(d)) == oracle.utf8_check(d, oracle.FLAG_UTF8_STRICT), (sq, off, tail)
+    rng = np.random.default_rng(17)
+    pool = np.array([0x41, 0x80, 0x8F, 0x90, 0x9F, 0xA0, 0xBF, 0xC0, 0xC2, 0xDF, 0xE0, 0xE1,
+                     0xED, 0xEF, 0xF0, 0xF1, 0xF4, 0xF5, 0xFF], np.uint8)
+    for _ in range(20000):
+        d = rng.choice(pool, rng.integers(1, 70)).tobytes()
+        assert f(d, len(d)) == oracle.utf8_check(d, oracle.FLAG_UTF8_STRICT), d
+    for _ in range(300):  # mostly valid text with some invalid bytes
+        s = "".join(chr(int(c)) for c in rng.choice([0x41, 0xE9, 0x20AC, 0x1F600, 0x7F], 300))
+        d = bytearray(s.encode())
+        if rng.random() < 0.5:
+            d[int(rng.integers(0, len(d)))] = int(rng.choice(pool))
+        d = bytes(d)
+        assert f(d, len(d)) == oracle.utf8_check(d, oracle.FLAG_UTF8_STRICT), d
