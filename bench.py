@@ -503,6 +503,9 @@ def cpu_baseline(sample_bytes: int):
     else:
         out["kind"] = "port"
         out["sample"] = "simdjson build absent; oracle only"
+    out["host"] = host_cpu()
+    ref_extra = simdjson_batch_baselines(ref) if ref is not None else {}
+    out.update(ref_extra)
     # the oracle (a scalar transcription of the reference) on 32 MB
     small = sample[: 32 << 20].copy()
     small[-1] = ord("]")
@@ -513,6 +516,56 @@ def cpu_baseline(sample_bytes: int):
     if out["value"] is None:
         out["value"] = out["oracle_port_gbs"]
     out["pison"] = pison_baseline()
+    return out
+
+
+def host_cpu() -> dict:
+    """The host the CPU baselines ran on: CPU model, nproc, and the threads
+    this process may use (its affinity mask)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def simdjson_batch_baselines(ref, lines_records: int = 250_000, docs: int = 50_000) -> dict:
+    """simdjson 3.2.1 (the reference's vendored copy) on the two batched
+    workloads, one core: parse_many (ondemand iterate_many, the reference's
+    quickstart-original-parse-many.cpp) over a JSON Lines sample from the
+    8M-record generator, and ondemand iterate of each of the 50,000
+    documents in turn (the one-call-per-document form of main.cu:26)."""
+    import ctypes
+    import oracle
+    from cujson_amd import gen
+    out = {}
+    lines = gen.lines_parallel(lines_records)
+    a, p = oracle._u8(lines)
+    nd = ctypes.c_size_t()
+    ms = min(ref.sjr_parse_many_ms(p, a.size, 1 << 20, ctypes.byref(nd)) for _ in range(3))
+    out["jsonl_parse_many"] = {"value": round(a.size / (ms * 1e-3) / 1e9, 3), "unit": "GB/s",
+                               "cores": 1, "documents": int(nd.value),
+                               "sample": f"{lines_records} records ({a.size / 1e6:.0f} MB) of the "
+                                         f"JSON Lines generator, 1 MiB batches, best of 3"}
+    data, offs, sizes, kinds = gen.many_docs(n_docs=docs)
+    a, p = oracle._u8(data)
+    szp = ctypes.POINTER(ctypes.c_size_t)
+    o = np.ascontiguousarray(offs, np.uint64)
+    z = np.ascontiguousarray(sizes, np.uint64)
+    errs = ctypes.c_size_t()
+    ms = ref.sjr_iterate_docs_ms(p, o.ctypes.data_as(szp), z.ctypes.data_as(szp), docs, 3,
+                                 ctypes.byref(errs))
+    total = int(sizes.sum())
+    out["batch_iterate"] = {"value": round(total / (ms * 1e-3) / 1e9, 3), "unit": "GB/s",
+                            "cores": 1, "ms": round(ms, 2), "documents": docs,
+                            "error_documents": int(errs.value),
+                            "sample": f"the whole {docs}-document batch ({total / 1e6:.0f} MB), "
+                                      f"one ondemand iterate per document, best of 3"}
     return out
 
 
@@ -532,8 +585,10 @@ def pison_baseline(sample_bytes: int = 256 << 20):
                                      ctypes.c_int, ctypes.c_int]
     L.pison_construct_ms.restype = ctypes.c_double
     doc = gen.standard_doc(target_bytes=sample_bytes, seed=gen.SEED_BASE + 103).tobytes()
+    # every CPU this process may use, at most the 16 a one-GPU box shares
+    # out (nproc may count the whole machine)
     cores = min(16, len(os.sched_getaffinity(0)))
-    out = {"unit": "GB/s", "levels": 8, "cores_all": cores,
+    out = {"unit": "GB/s", "levels": 8, "cores_all": cores, "nproc": os.cpu_count(),
            "sample": f"{len(doc) / 1e6:.0f} MB document from the same generator without the "
                      f"deep branch, BitmapConstructor::construct + getIterator, best of 3"}
     for name, th in (("one_core", 1), ("all_cores", cores)):

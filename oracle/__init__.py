@@ -84,6 +84,9 @@ def ref_lib():
         L.sjr_parse_many_ms.argtypes = [u8p, ctypes.c_size_t, ctypes.c_size_t,
                                         ctypes.POINTER(ctypes.c_size_t)]
         L.sjr_parse_many_ms.restype = ctypes.c_double
+        szp = ctypes.POINTER(ctypes.c_size_t)
+        L.sjr_iterate_docs_ms.argtypes = [u8p, szp, szp, ctypes.c_size_t, ctypes.c_int, szp]
+        L.sjr_iterate_docs_ms.restype = ctypes.c_double
         _ref = L
     return _ref
 

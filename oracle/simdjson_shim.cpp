@@ -80,4 +80,44 @@ double sjr_parse_many_ms(const uint8_t *buf, size_t len, size_t batch, size_t *n
     return std::chrono::duration<double, std::milli>(t1 - t0).count();
 }
 
+// Many independent documents (data + off[i], size[i] bytes each): ondemand
+// iterate of every document in turn with one parser, the reference's
+// one-call-per-document form (main.cu:26) on the CPU; the documents are
+// copied with simdjson's padding before the timed loop.  Best of `reps`
+// passes in milliseconds; *err_docs receives how many documents failed.
+double sjr_iterate_docs_ms(const uint8_t *data, const size_t *off, const size_t *size,
+                           size_t count, int reps, size_t *err_docs) {
+    size_t total = 0;
+    for (size_t i = 0; i < count; i++) total += size[i] + SIMDJSON_PADDING;
+    char *buf = new char[total + 64];
+    size_t *pos = new size_t[count];
+    size_t at = 0;
+    for (size_t i = 0; i < count; i++) {
+        pos[i] = at;
+        std::memcpy(buf + at, data + off[i], size[i]);
+        std::memset(buf + at + size[i], 0, SIMDJSON_PADDING);
+        at += size[i] + SIMDJSON_PADDING;
+    }
+    ondemand::parser parser;
+    double best = 1e30;
+    size_t errs = 0;
+    for (int r = 0; r < reps; r++) {
+        size_t e = 0;
+        auto t0 = std::chrono::high_resolution_clock::now();
+        for (size_t i = 0; i < count; i++) {
+            auto doc = parser.iterate(padded_string_view(buf + pos[i], size[i],
+                                                         size[i] + SIMDJSON_PADDING));
+            if (doc.error()) e++;
+        }
+        auto t1 = std::chrono::high_resolution_clock::now();
+        errs = e;
+        double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        if (ms < best) best = ms;
+    }
+    delete[] pos;
+    delete[] buf;
+    if (err_docs) *err_docs = errs;
+    return best;
+}
+
 }  // extern "C"
