@@ -98,6 +98,7 @@ def many_docs_bench(cj, gen, n_docs: int, reps: int, plain_compare: bool = True)
     device-resident: batched (one launch sequence for all documents) against
     plain (one device-resident parse per document, the reference's
     one-document-per-launch form), same GPU, same inputs."""
+    from cujson_amd import autotune
     from cujson_amd.hip import DeviceBuffer, Event, synchronize
     data, offs, sizes, kinds = gen.many_docs(n_docs=n_docs)
     total = int(sizes.sum())
@@ -119,6 +120,11 @@ def many_docs_bench(cj, gen, n_docs: int, reps: int, plain_compare: bool = True)
     rc, docs, N, M, over = ctx.batch_wait()
     if rc or over:
         raise RuntimeError(f"batch failed rc={rc} overflow={over}")
+    # tokenizer launch shape for the batch (untimed): tile shapes for its
+    # large documents, then workgroup sizes for its one-wave documents
+    shape, shape_table = autotune.tune_tokenizer_shape(
+        ctx, lambda: batched(cj.CJX_FLAG_GRAPH), signature=f"batch:{n_docs}:{total}",
+        tiles=True, small=True, reps=5, use_cache=False)
     e0, e1 = Event(), Event()
 
     def timed(flags):
@@ -177,6 +183,8 @@ def many_docs_bench(cj, gen, n_docs: int, reps: int, plain_compare: bool = True)
                     f"device-resident",
         "docs": n_docs, "bytes": total, "structurals": int(N), "brackets": int(M),
         "doc_codes": {"ok": codes[0], "utf8": codes[2], "structure": codes[3]},
+        "tokenizer_shape": {"tile_waves": shape[0], "small_waves": shape[1],
+                            "autotune_ms": shape_table},
         "batched_ms": round(batched_ms, 4),
         "batched_call_ms": round(call_ms, 4),
         "batched_form": f"one launch sequence for the whole batch, {batched_form} (the faster "
@@ -242,6 +250,12 @@ def lines_bench(cj, gen, n_records: int, reps: int, plain_compare: bool = True):
 
     batched()
     synchronize()
+    L0 = launches[0]
+    shape, shape_table = autotune.tune_tokenizer_shape(
+        ctx, lambda: ctx.parse_lines_device(L0["dev"].ptr, L0["offs"], L0["sizes"], L0["st"].ptr,
+                                            L0["pr"].ptr, L0["cap"], L0["rec"].ptr,
+                                            L0["max_rec"]),
+        signature=f"lines:{best}:{int(L0['sizes'].sum())}", reps=3, use_cache=False)
     nrec = 0
     for L in launches:  # each launch's status (the last launch is the one a wait reads)
         ctx.parse_lines_device(L["dev"].ptr, L["offs"], L["sizes"], L["st"].ptr, L["pr"].ptr,
@@ -288,6 +302,8 @@ def lines_bench(cj, gen, n_records: int, reps: int, plain_compare: bool = True):
                     f"clipped to 64 KiB), depth 1-6; device-resident; launches of < 2^31 bytes",
         "records": int(nrec), "bytes": total, "launches": len(launches), "chunks": n_chunks,
         "chunk_bytes": int(best), "autotune_ms": {str(k): round(v, 3) for k, v in table.items()},
+        "tokenizer_shape": {"tile_waves": shape[0], "small_waves": shape[1],
+                            "autotune_ms": shape_table},
         "batched_ms": round(batched_ms, 3),
         "batched_gbs": round(total / (batched_ms * 1e-3) / 1e9, 2),
         "batched_call_ms": round(call_ms, 3),
@@ -640,7 +656,7 @@ def main():
         return
 
     import cujson_amd as cj
-    from cujson_amd import gen
+    from cujson_amd import autotune, gen
     from cujson_amd.hip import DeviceBuffer, Event, synchronize
 
     t0 = time.time()
@@ -666,6 +682,11 @@ def main():
     if over or rc != 0:
         raise RuntimeError(f"first parse failed: rc={rc} overflow={over}")
     log(f"N={N} M={M} rho={N / n:.4f}")
+    # tokenizer launch shape for this workload (untimed): every tile shape
+    # timed on the parse itself, the fastest left set on the context
+    shape, shape_table = autotune.tune_tokenizer_shape(ctx, step, signature=f"plain:{n}",
+                                                       reps=5, use_cache=False)
+    log(f"tokenizer shape {shape} from {shape_table}")
     for _ in range(args.warmup):
         step()
     synchronize()
@@ -768,6 +789,8 @@ def main():
                         "device-resident, one parse per step",
             "bytes": n, "structurals": int(N), "brackets": int(M),
             "rho": round(N / n, 5), "path": "plain", "form": form,
+            "tokenizer_shape": {"tile_waves": shape[0], "small_waves": shape[1],
+                                "autotune_ms": shape_table},
         },
         "roofline": {
             "bound": "hbm",

@@ -57,6 +57,7 @@ ABI_SYMBOLS = (
     "cjx_chunk_lines", "cjx_parse_standard", "cjx_parse_lines", "cjx_free_result",
     "cjx_strerror", "cjx_parse_device", "cjx_device_wait", "cjx_context_stream",
     "cjx_version", "cjx_context_set_timing", "cjx_context_timings",
+    "cjx_context_set_tokenizer_shape", "cjx_context_tokenizer_shape",
     "cjx_parse_batch", "cjx_free_batch_result", "cjx_parse_batch_device", "cjx_batch_wait",
     "cjx_parse_lines_device", "cjx_server_create", "cjx_server_submit", "cjx_server_wait",
     "cjx_server_release", "cjx_server_reserve", "cjx_server_get_stats", "cjx_server_destroy",
@@ -184,6 +185,11 @@ def lib():
     L.cjx_context_set_timing.restype = ctypes.c_int
     L.cjx_context_timings.argtypes = [vp, ctypes.POINTER(_Timings), ctypes.c_int]
     L.cjx_context_timings.restype = ctypes.c_int
+    L.cjx_context_set_tokenizer_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.cjx_context_set_tokenizer_shape.restype = ctypes.c_int
+    L.cjx_context_tokenizer_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_int)]
+    L.cjx_context_tokenizer_shape.restype = ctypes.c_int
     L.cjx_version.argtypes = []
     L.cjx_version.restype = ctypes.c_char_p
     L.cjx_parse_batch.argtypes = [vp, u8p, szp, szp, sz, ctypes.c_uint,
@@ -293,6 +299,20 @@ class Context:
 
     def set_timing(self, on: bool = True):
         lib().cjx_context_set_timing(self._h, 1 if on else 0)
+
+    def set_tokenizer_shape(self, tile_waves: int = 4, small_waves: int = 4):
+        """Tokenizer launch shape of later parses: wave64s per 64 KiB tile
+        (2, 4, 8, 16) and one-wave documents per workgroup (1, 2, 4, 8)."""
+        rc = lib().cjx_context_set_tokenizer_shape(self._h, int(tile_waves), int(small_waves))
+        if rc:
+            raise ValueError(f"invalid tokenizer shape ({tile_waves}, {small_waves})")
+
+    def tokenizer_shape(self):
+        """(tile_waves, small_waves) of later parses."""
+        t, sm = ctypes.c_int(), ctypes.c_int()
+        if lib().cjx_context_tokenizer_shape(self._h, ctypes.byref(t), ctypes.byref(sm)):
+            raise RuntimeError("no context")
+        return t.value, sm.value
 
     def timings(self, max_entries: int = 1):
         """[(tokenize_ms, phase3_ms, total_ms)] of the most recent parses, oldest

@@ -1,22 +1,34 @@
-"""Autotuner for the batched JSON Lines path.
+"""Autotuner: JSON Lines chunk size and the tokenizer's launch shape.
 
 The launch shape of a JSON Lines parse is fixed by the chunk size: every
 chunk is one parse unit.  Small chunks mean many units (and more tail waste
 per unit); large chunks mean fewer, larger units whose phase-3 work runs per
 unit.  `tune_lines_chunk` times the candidate chunk sizes on a sample of the
-workload, device-resident, and returns the fastest.  Results are cached in a
-JSON file (``$CJX_TUNE_CACHE`` or ``~/.cache/cujson_amd/tune.json``) keyed by
-the device and the workload signature, so a later run reuses them.
+workload, device-resident, and returns the fastest.
+
+The tokenizer's per-thread work and workgroup size are picked at run time
+from the library's built instantiations (`Context.set_tokenizer_shape`):
+a 64 KiB tile is 2, 4, 8 or 16 wave64s (1024, 512, 256 or 128 input bytes
+per lane), and documents of at most 16 KiB take one wave each, 1, 2, 4 or 8
+per workgroup.  `tune_tokenizer_shape` times the candidates on the caller's
+own parse call and leaves the fastest set on the context.
+
+Results are cached in a JSON file (``$CJX_TUNE_CACHE`` or
+``~/.cache/cujson_amd/tune.json``) keyed by the workload signature the
+caller gives (name it for the device and the input shape), so a later run
+reuses them.
 """
 from __future__ import annotations
 
 import json
 import os
-from typing import Dict, Sequence, Tuple
+from typing import Callable, Dict, Sequence, Tuple
 
 import numpy as np
 
 CANDIDATES = (256 << 10, 1 << 20, 4 << 20, 16 << 20, 64 << 20)
+TILE_WAVES = (2, 4, 8, 16)   # wave64s per 64 KiB tokenizer tile
+SMALL_WAVES = (1, 2, 4, 8)   # one-wave documents per workgroup
 
 
 def _cache_path() -> str:
@@ -106,3 +118,66 @@ def tune_lines_chunk(ctx, sample: np.ndarray, candidates: Sequence[int] = CANDID
         cache[key] = {"best": best, "table": {str(k): v for k, v in table.items()}}
         _store_cache(cache)
     return best, table
+
+
+def time_device(ctx, run: Callable[[], None], reps: int = 5) -> float:
+    """Median device time (ms, from the parse's reset kernel to its last
+    phase-3 kernel) of `reps` calls of run(), a parse on `ctx` that waits for
+    nothing.  One untimed call first (a HIP graph is captured there)."""
+    reps = max(2, min(int(reps), 256))
+    run()
+    ctx.set_timing(True)
+    for _ in range(reps):
+        run()
+    t = [x[2] for x in ctx.timings(reps)]
+    ctx.set_timing(False)
+    return float(np.median(t))
+
+
+def tune_tokenizer_shape(ctx, run: Callable[[], None], signature: str,
+                         tiles: bool = True, small: bool = False,
+                         tile_candidates: Sequence[int] = TILE_WAVES,
+                         small_candidates: Sequence[int] = SMALL_WAVES,
+                         reps: int = 5, use_cache: bool = True
+                         ) -> Tuple[Tuple[int, int], Dict[str, float]]:
+    """Fastest tokenizer shape (tile_waves, small_waves) for the parse that
+    run() performs, and the measured table {"tile_waves x small_waves": ms}.
+    The tile shape is searched first with the small shape at its default,
+    then the small shape with the best tile shape (`tiles` / `small` say
+    which kinds of unit the workload has).  The winner is left set on ctx."""
+    key = f"tok-shape:{signature}"
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        e = cache[key]
+        best = (int(e["best"][0]), int(e["best"][1]))
+        ctx.set_tokenizer_shape(*best)
+        return best, dict(e["table"])
+    table: Dict[str, float] = {}
+
+    def measure(tw: int, sw: int) -> float:
+        ctx.set_tokenizer_shape(tw, sw)
+        ms = time_device(ctx, run, reps)
+        table[f"{tw}x{sw}"] = round(ms, 4)
+        return ms
+
+    best_t, best_s = ctx.tokenizer_shape()
+    best_ms = measure(best_t, best_s)
+    if tiles:
+        for tw in tile_candidates:
+            if f"{tw}x{best_s}" not in table:
+                ms = measure(tw, best_s)
+                if ms < best_ms:
+                    best_ms, best_t = ms, tw
+    if small:
+        s0 = best_s
+        for sw in small_candidates:
+            if f"{best_t}x{sw}" not in table:
+                ms = measure(best_t, sw)
+                if ms < best_ms:
+                    best_ms, s0 = ms, sw
+        best_s = s0
+    ctx.set_tokenizer_shape(best_t, best_s)
+    if use_cache:
+        cache[key] = {"best": [best_t, best_s], "table": table}
+        _store_cache(cache)
+    return (best_t, best_s), table

@@ -302,7 +302,7 @@ int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsign
     if (strict && launch_utf8_strict(s, b, d_in, sh.ntiles, sh.n_small) != hipSuccess)
         return CJX_ERR_DEVICE;
     if (launch_tokenize(s, b, d_in, sh.ntiles, sh.n_small, (flags & CJX_FLAG_JSONL) != 0,
-                        !strict, d_struct) != hipSuccess)
+                        !strict, d_struct, ctx->tok_shape) != hipSuccess)
         return CJX_ERR_DEVICE;
     if (ctx->timing) (void)hipEventRecord(tev[1], s);
     return enqueue_back(ctx, sh, flags, d_pair);
@@ -389,8 +389,8 @@ int enqueue_parse_chunked(cjx_context *ctx, const uint8_t *host, uint64_t n, con
         if (k == 0 && ctx->timing) (void)hipEventRecord(tev[0], s);
         const uint32_t upto =
             k + 1 == parts ? sh.ntiles : (uint32_t)std::min<uint64_t>(sh.ntiles, (k + 1) * part / kTileBytes);
-        if (launch_tokenize_tiles(s, b, d_in, upto - launched, jsonl, !strict, d_struct) !=
-            hipSuccess)
+        if (launch_tokenize_tiles(s, b, d_in, upto - launched, jsonl, !strict, d_struct,
+                                  ctx->tok_shape) != hipSuccess)
             return CJX_ERR_DEVICE;
         if (counts && (hipMemcpyAsync(static_cast<uint64_t *>(ctx->h_part.p) + k,
                                       b.tile_st + 2 * (uint64_t)(upto - 1), 8,
@@ -429,6 +429,7 @@ int enqueue_maybe_graph(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, 
     key.concat = sh.concat;
     key.one_unit_dev = ctx->one_unit_dev;
     if (ctx->one_unit_dev) key.one_unit = ctx->one_unit;
+    key.tok_shape = ctx->tok_shape;
     if (!ctx->graph || std::memcmp(&key, &ctx->graph_key, sizeof(key)) != 0) {
         if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
         ctx->graph = nullptr;
@@ -963,6 +964,22 @@ int cjx_context_set_timing(cjx_context *ctx, int enable) {
     }
     ctx->timing = enable != 0;
     ctx->timing_count = 0;
+    return CJX_OK;
+}
+
+int cjx_context_set_tokenizer_shape(cjx_context *ctx, int tile_waves, int small_waves) {
+    if (!ctx || tile_waves < 0 || small_waves < 0 ||
+        !cjx::tok_shape_valid((uint32_t)tile_waves, (uint32_t)small_waves))
+        return CJX_ERR_INPUT;
+    ctx->tok_shape.tile_waves = (uint32_t)tile_waves;
+    ctx->tok_shape.small_waves = (uint32_t)small_waves;
+    return CJX_OK;
+}
+
+int cjx_context_tokenizer_shape(cjx_context *ctx, int *tile_waves, int *small_waves) {
+    if (!ctx || !tile_waves || !small_waves) return CJX_ERR_INPUT;
+    *tile_waves = (int)ctx->tok_shape.tile_waves;
+    *small_waves = (int)ctx->tok_shape.small_waves;
     return CJX_OK;
 }
 

@@ -77,6 +77,7 @@ struct cjx_context {
     hipEvent_t tev[kTimingRing][3] = {};
     uint64_t timing_count = 0;
     bool timing = false;
+    cjx::TokShape tok_shape;  // tokenizer launch shape (cjx_context_set_tokenizer_shape)
     DevBuf in, structural, pair;
     DevBuf span;  // host units copied as one span of their source, before placement
     DevBuf units, uout, ls, tile_st, layout, chunk_unit, big_units, unit_part, rec_count, recs;
@@ -139,6 +140,7 @@ struct cjx_context {
         bool concat;
         bool one_unit_dev;
         cjx::UnitDesc one_unit;
+        cjx::TokShape tok_shape;
     } graph_key{};
 };
 

@@ -163,16 +163,31 @@ hipError_t launch_reset(hipStream_t s, const LaunchBufs &b, uint32_t ntiles, uin
 hipError_t launch_reset_one(hipStream_t s, const LaunchBufs &b, const UnitDesc &u, uint32_t ntiles,
                             uint32_t n_small);
 hipError_t launch_rows_out(hipStream_t s, const uint32_t *src, uint32_t *host_dst, uint64_t count);
+// Launch shape of the tokenizer, chosen at run time from the built
+// instantiations.  A 64 KiB tile is tile_waves waves of 64 KiB / tile_waves
+// bytes each (2 x 32 KiB, 4 x 16 KiB, 8 x 8 KiB or 16 x 4 KiB: 1024, 512, 256
+// or 128 bytes per lane); small units take one wave of 16 KiB each,
+// small_waves of them per workgroup (1, 2, 4 or 8).
+struct TokShape {
+    uint32_t tile_waves = 4;
+    uint32_t small_waves = 4;
+};
+inline bool tok_shape_valid(uint32_t tile_waves, uint32_t small_waves) {
+    const bool t = tile_waves == 2 || tile_waves == 4 || tile_waves == 8 || tile_waves == 16;
+    const bool sm = small_waves == 1 || small_waves == 2 || small_waves == 4 || small_waves == 8;
+    return t && sm;
+}
 // Units of at most kWaveBytes bytes are "small": no tiles, one wave each
 // (listed in small_units); the others are cut into kTileBytes tiles.
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
                            uint32_t ntiles, uint32_t n_small, bool jsonl, bool weak_utf8,
-                           uint32_t *out_struct);
+                           uint32_t *out_struct, TokShape shape = TokShape{});
 // `count` more big-tile workgroups (tickets continue from the launches before
 // on the same stream), no small units: the tokenizer of a launch split into
 // parts that each wait for their input bytes.
 hipError_t launch_tokenize_tiles(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                                 uint32_t count, bool jsonl, bool weak_utf8, uint32_t *out_struct);
+                                 uint32_t count, bool jsonl, bool weak_utf8, uint32_t *out_struct,
+                                 TokShape shape = TokShape{});
 hipError_t launch_utf8_strict(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
                               uint32_t ntiles, uint32_t n_small);
 hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool concat);

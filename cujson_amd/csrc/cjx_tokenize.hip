@@ -293,8 +293,9 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
 }
 
 // State of a wave after phases 1 and 2 (everything before the look-back).
+template <int ROUNDS>
 struct WaveScan {
-    uint32_t instr_r[kTokRounds], op_r[kTokRounds], br_r[kTokRounds];
+    uint32_t instr_r[ROUNDS], op_r[ROUNDS], br_r[ROUNDS];
     uint32_t c_run, s_run, seen, dflag, utf8_bad, lane_cnt0, lane_all;
 };
 
@@ -303,16 +304,16 @@ struct WaveScan {
 // waves of a unit), so the rounds need no end checks and the loads no bounds
 // checks; a separate instantiation keeps that path free of the branches
 // (and the register copies at their joins) that the edge waves need.
-template <bool JSONL, bool INSIDE>
+template <bool JSONL, bool INSIDE, int ROUNDS>
 __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_off, uint64_t n,
                                             uint32_t lane, uint64_t lt, uint32_t weak_utf8,
-                                            uint32_t (&b1)[kTokRounds][64],
-                                            uint32_t (&b5)[kTokRounds][64], WaveScan &ws) {
+                                            uint32_t (&b1)[ROUNDS][64],
+                                            uint32_t (&b5)[ROUNDS][64], WaveScan<ROUNDS> &ws) {
     // Rounds stream through a double buffer: round i+2 is loaded as soon as
     // round i has been transposed.
     uint32_t x[2][8];
     load_lane(ubase, wave_off + (uint64_t)lane * kLaneBytes, n, INSIDE, x[0]);
-    if (kTokRounds > 1)
+    if (ROUNDS > 1)
         load_lane(ubase, wave_off + kRoundBytes + (uint64_t)lane * kLaneBytes, n, INSIDE, x[1]);
     uint32_t prev_dw = 0;
     if (wave_off > 0 && wave_off < n)
@@ -328,9 +329,9 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
         c_r7 = prev_dw;
     }
 
-    uint32_t(&instr_r)[kTokRounds] = ws.instr_r;
-    uint32_t(&op_r)[kTokRounds] = ws.op_r;
-    uint32_t(&br_r)[kTokRounds] = ws.br_r;
+    uint32_t(&instr_r)[ROUNDS] = ws.instr_r;
+    uint32_t(&op_r)[ROUNDS] = ws.op_r;
+    uint32_t(&br_r)[ROUNDS] = ws.br_r;
     uint32_t &c_run = ws.c_run, &s_run = ws.s_run, &seen = ws.seen, &dflag = ws.dflag;
     uint32_t &utf8_bad = ws.utf8_bad;
     uint32_t &lane_cnt0 = ws.lane_cnt0, &lane_all = ws.lane_all;
@@ -339,7 +340,7 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
     uint32_t cnt_op0 = 0, cnt_br0 = 0, cnt_op = 0, cnt_br = 0;
 
 #pragma unroll
-    for (int i = 0; i < kTokRounds; i++) {
+    for (int i = 0; i < ROUNDS; i++) {
         // Rounds past the end of the unit hold no bytes: no masks, carries
         // unchanged (nothing after the unit's end depends on them).
         if (!INSIDE && wave_off + (uint64_t)i * kRoundBytes >= n) {
@@ -350,7 +351,7 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
         uint32_t r[8], k[8];
         transpose4(x[i & 1][0], x[i & 1][2], x[i & 1][4], x[i & 1][6], r[0], r[1], r[2], r[3]);
         transpose4(x[i & 1][1], x[i & 1][3], x[i & 1][5], x[i & 1][7], r[4], r[5], r[6], r[7]);
-        if (i + 2 < kTokRounds)
+        if (i + 2 < ROUNDS)
             load_lane(ubase, lane_off + 2 * kRoundBytes, n, INSIDE, x[i & 1]);
 #pragma unroll
         for (int d = 0; d < 8; d++) k[d] = classify<JSONL>(r[d]);
@@ -438,37 +439,36 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
 // running workgroups.  Small units (SMALL = true, at most one wave's 16 KiB):
 // one wave per unit, which starts in state (0, 0) and owns the unit's whole
 // output region, so there is no look-back and no workgroup barrier.
-// Register budget: 4 waves per SIMD (103 VGPRs, no spills).  With the
-// interior and edge rounds as separate instantiations, the 5-wave budget
-// (96 VGPRs) spills and measured 4 % slower.
-#ifndef CJX_TOK_WPE
-#define CJX_TOK_WPE 4
-#endif
-#if CJX_TOK_WPE > 0
-#define CJX_TOK_ATTR __attribute__((amdgpu_waves_per_eu(CJX_TOK_WPE, CJX_TOK_WPE)))
-#else
-#define CJX_TOK_ATTR
-#endif
-template <bool JSONL, bool SMALL>
-__global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
+// A tile is WAVES waves of ROUNDS rounds (WAVES x ROUNDS x 2 KiB = 64 KiB);
+// a small-unit workgroup is WAVES waves of kTokRounds rounds (one wave covers
+// the largest small unit).  WPE is the register budget in waves per SIMD:
+// 4 x 8 at 4 waves per SIMD uses 103 VGPRs without spills; with the interior
+// and edge rounds as separate instantiations, the 5-wave budget (96 VGPRs)
+// spills and measured 4 % slower.
+template <bool JSONL, bool SMALL, int WAVES, int WPE>
+__global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 * WAVES) void k_tokenize(
     const uint8_t *__restrict__ in, const UnitDesc *__restrict__ units,
     const uint32_t *__restrict__ tile_unit, const uint32_t *__restrict__ small_units,
     uint32_t n_small, unsigned long long *__restrict__ tile_st, LaunchState *__restrict__ ls,
     UnitOut *__restrict__ uo, uint32_t *__restrict__ out_struct, uint8_t *__restrict__ oc_char,
     uint32_t *__restrict__ oc_idx, uint8_t *__restrict__ row_nl, uint32_t weak_utf8) {
+    constexpr int ROUNDS = SMALL ? kTokRounds : kTileBytes / (WAVES * kRoundBytes);
+    constexpr uint64_t WAVE_BYTES = (uint64_t)ROUNDS * kRoundBytes;
+    static_assert(SMALL || WAVES * ROUNDS * kRoundBytes == kTileBytes, "a tile is 64 KiB");
+    static_assert(!SMALL || WAVE_BYTES == kWaveBytes, "one wave covers a small unit");
     __shared__ uint32_t s_tile, s_unit;
-    __shared__ uint32_t s_map[kWaves];
-    __shared__ uint32_t s_cnt0[kWaves], s_cnt_all[kWaves];
-    __shared__ uint32_t s_fix[kWaves], s_nbase[kWaves], s_mbase[kWaves];
-    __shared__ uint32_t s_stage[kWaves][kStageRows];  // one round's positions per wave
-    __shared__ uint8_t s_bchar[kWaves][kStageRows];   // and its bracket characters
+    __shared__ uint32_t s_map[WAVES];
+    __shared__ uint32_t s_cnt0[WAVES], s_cnt_all[WAVES];
+    __shared__ uint32_t s_fix[WAVES], s_nbase[WAVES], s_mbase[WAVES];
+    __shared__ uint32_t s_stage[WAVES][kStageRows];  // one round's positions per wave
+    __shared__ uint8_t s_bchar[WAVES][kStageRows];   // and its bracket characters
     // Bracket characters as two bit masks per lane-round, kept in LDS until
     // compaction: byte bit 1 ('[' / '{' have it, ']' / '}' do not) and byte
     // bit 5 (curly), both restricted to bracket bytes.
     // For JSON Lines s_b5 also holds the '\n' bytes, which are never brackets:
     // class 0 without byte bit 5 (':' has it, '\n' not); they give the
     // per-row newline flags of record tables (row_nl).
-    __shared__ uint32_t s_b1[kWaves][kTokRounds][64], s_b5[kWaves][kTokRounds][64];
+    __shared__ uint32_t s_b1[WAVES][ROUNDS][64], s_b5[WAVES][ROUNDS][64];
     const bool want_nl = JSONL && row_nl != nullptr;
 
     const uint32_t tid = threadIdx.x;
@@ -477,7 +477,7 @@ __global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
 
     uint32_t tile = 0, u;
     if constexpr (SMALL) {
-        const uint32_t idx = blockIdx.x * kWaves + w;
+        const uint32_t idx = blockIdx.x * WAVES + w;
         if (idx >= n_small) return;  // the whole wave; SMALL code has no workgroup barrier
         u = small_units[idx];
     } else {
@@ -494,17 +494,17 @@ __global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
     const uint32_t tile_in_unit = SMALL ? 0u : tile - U.tile0;
     const uint8_t *ubase = in + U.in_off;
     const uint64_t wave_off =
-        SMALL ? 0ull : (uint64_t)tile_in_unit * kTileBytes + (uint64_t)w * kWaveBytes;
+        SMALL ? 0ull : (uint64_t)tile_in_unit * kTileBytes + (uint64_t)w * WAVE_BYTES;
 
-    const bool inside = wave_off + (uint64_t)kWaveBytes <= U.n;
-    WaveScan ws;
+    const bool inside = wave_off + (uint64_t)WAVE_BYTES <= U.n;
+    WaveScan<ROUNDS> ws;
     if (inside)
-        scan_rounds<JSONL, true>(ubase, wave_off, U.n, lane, lt, weak_utf8, s_b1[w], s_b5[w], ws);
+        scan_rounds<JSONL, true, ROUNDS>(ubase, wave_off, U.n, lane, lt, weak_utf8, s_b1[w], s_b5[w], ws);
     else
-        scan_rounds<JSONL, false>(ubase, wave_off, U.n, lane, lt, weak_utf8, s_b1[w], s_b5[w], ws);
-    const uint32_t(&instr_r)[kTokRounds] = ws.instr_r;
-    const uint32_t(&op_r)[kTokRounds] = ws.op_r;
-    const uint32_t(&br_r)[kTokRounds] = ws.br_r;
+        scan_rounds<JSONL, false, ROUNDS>(ubase, wave_off, U.n, lane, lt, weak_utf8, s_b1[w], s_b5[w], ws);
+    const uint32_t(&instr_r)[ROUNDS] = ws.instr_r;
+    const uint32_t(&op_r)[ROUNDS] = ws.op_r;
+    const uint32_t(&br_r)[ROUNDS] = ws.br_r;
     const uint32_t c_run = ws.c_run, s_run = ws.s_run, seen = ws.seen, dflag = ws.dflag;
     const uint32_t utf8_bad = ws.utf8_bad, lane_cnt0 = ws.lane_cnt0, lane_all = ws.lane_all;
 
@@ -540,9 +540,9 @@ __global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
         if (w == 0) {
             // The tile entered at (c, s) = (0, 0): wave string fixes and counts.
             uint32_t T = kMapIdentity, state = 0, n0 = 0, m0 = 0, opc = 0, brc = 0;
-            uint32_t fix0[kWaves];
+            uint32_t fix0[WAVES];
 #pragma unroll
-            for (int kk = 0; kk < kWaves; kk++) {
+            for (int kk = 0; kk < WAVES; kk++) {
                 const uint32_t m = s_map[kk];
                 fix0[kk] = ((state >> 1) & 1u) ^ ((state & 1u) & ((m >> 2) & 1u));
                 state = map_apply(m, state);
@@ -564,7 +564,7 @@ __global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
                 publish_inc(tile_st, tile, map_apply(T, entry), n_excl + nt, m_excl + mt);
                 uint32_t nb = U.out_off + n_excl, mb = U.out_off + m_excl;
 #pragma unroll
-                for (int kk = 0; kk < kWaves; kk++) {
+                for (int kk = 0; kk < WAVES; kk++) {
                     const uint32_t f = fix0[kk] ^ e;
                     const uint32_t c0 = s_cnt0[kk], ca = s_cnt_all[kk];
                     s_fix[kk] = f;
@@ -595,7 +595,7 @@ __global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
     const uint32_t cap_end = U.out_off + U.out_cap;
     uint32_t overflow = 0;
 #pragma unroll
-    for (int i = 0; i < kTokRounds; i++) {
+    for (int i = 0; i < ROUNDS; i++) {
         if (wave_off + (uint64_t)i * kRoundBytes >= U.n) break;  // no bytes, no rows
         const uint64_t lane_off = wave_off + (uint64_t)i * kRoundBytes + (uint64_t)lane * kLaneBytes;
         const uint32_t sti = op_r[i] & ~(instr_r[i] ^ fix);
@@ -734,45 +734,66 @@ __global__ CJX_TOK_ATTR __launch_bounds__(kTokThreads) void k_tokenize(
     if (__ballot(overflow) && lane == 0) atomicOr(&ls->overflow, 1u);
 }
 
+namespace {
+
+template <bool JSONL, bool SMALL, int WAVES, int WPE>
+void launch_variant(hipStream_t s, uint32_t grid, const LaunchBufs &b, const uint8_t *d_in,
+                    uint32_t n_small, uint32_t *out_struct, uint32_t weak) {
+    hipLaunchKernelGGL((k_tokenize<JSONL, SMALL, WAVES, WPE>), dim3(grid), dim3(64 * WAVES), 0, s,
+                       d_in, b.units, b.tile_unit, b.small_units, n_small, b.tile_st, b.ls, b.uout,
+                       out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
+}
+
+// Register budgets: 4 waves per SIMD for 4, 8 and 16 waves of 16, 8 and
+// 4 KiB; the 32 KiB waves of the 2-wave tile carry twice the per-round
+// counts and take a 3-wave budget.
+template <bool JSONL>
+void launch_tiles(hipStream_t s, uint32_t grid, uint32_t tile_waves, const LaunchBufs &b,
+                  const uint8_t *d_in, uint32_t *out_struct, uint32_t weak) {
+    switch (tile_waves) {
+    case 2: launch_variant<JSONL, false, 2, 3>(s, grid, b, d_in, 0u, out_struct, weak); break;
+    case 8: launch_variant<JSONL, false, 8, 4>(s, grid, b, d_in, 0u, out_struct, weak); break;
+    case 16: launch_variant<JSONL, false, 16, 4>(s, grid, b, d_in, 0u, out_struct, weak); break;
+    default: launch_variant<JSONL, false, 4, 4>(s, grid, b, d_in, 0u, out_struct, weak); break;
+    }
+}
+
+template <bool JSONL>
+void launch_small(hipStream_t s, uint32_t n_small, uint32_t small_waves, const LaunchBufs &b,
+                  const uint8_t *d_in, uint32_t *out_struct, uint32_t weak) {
+    const uint32_t sw = small_waves;
+    const uint32_t grid = (n_small + sw - 1) / sw;
+    switch (sw) {
+    case 1: launch_variant<JSONL, true, 1, 4>(s, grid, b, d_in, n_small, out_struct, weak); break;
+    case 2: launch_variant<JSONL, true, 2, 4>(s, grid, b, d_in, n_small, out_struct, weak); break;
+    case 8: launch_variant<JSONL, true, 8, 4>(s, grid, b, d_in, n_small, out_struct, weak); break;
+    default: launch_variant<JSONL, true, 4, 4>(s, grid, b, d_in, n_small, out_struct, weak); break;
+    }
+}
+
+}  // namespace
+
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
                            uint32_t ntiles, uint32_t n_small, bool jsonl, bool weak_utf8,
-                           uint32_t *out_struct) {
+                           uint32_t *out_struct, TokShape shape) {
     const uint32_t weak = weak_utf8 ? 1u : 0u;
-    const uint32_t gs = (n_small + kWaves - 1) / kWaves;
     if (jsonl) {
-        if (ntiles)
-            hipLaunchKernelGGL((k_tokenize<true, false>), dim3(ntiles), dim3(kTokThreads), 0, s,
-                               d_in, b.units, b.tile_unit, b.small_units, n_small, b.tile_st, b.ls,
-                               b.uout, out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
-        if (n_small)
-            hipLaunchKernelGGL((k_tokenize<true, true>), dim3(gs), dim3(kTokThreads), 0, s, d_in,
-                               b.units, b.tile_unit, b.small_units, n_small, b.tile_st, b.ls,
-                               b.uout, out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
+        if (ntiles) launch_tiles<true>(s, ntiles, shape.tile_waves, b, d_in, out_struct, weak);
+        if (n_small) launch_small<true>(s, n_small, shape.small_waves, b, d_in, out_struct, weak);
     } else {
-        if (ntiles)
-            hipLaunchKernelGGL((k_tokenize<false, false>), dim3(ntiles), dim3(kTokThreads), 0, s,
-                               d_in, b.units, b.tile_unit, b.small_units, n_small, b.tile_st, b.ls,
-                               b.uout, out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
-        if (n_small)
-            hipLaunchKernelGGL((k_tokenize<false, true>), dim3(gs), dim3(kTokThreads), 0, s, d_in,
-                               b.units, b.tile_unit, b.small_units, n_small, b.tile_st, b.ls,
-                               b.uout, out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
+        if (ntiles) launch_tiles<false>(s, ntiles, shape.tile_waves, b, d_in, out_struct, weak);
+        if (n_small) launch_small<false>(s, n_small, shape.small_waves, b, d_in, out_struct, weak);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_tokenize_tiles(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
-                                 uint32_t count, bool jsonl, bool weak_utf8, uint32_t *out_struct) {
+                                 uint32_t count, bool jsonl, bool weak_utf8, uint32_t *out_struct,
+                                 TokShape shape) {
     if (count == 0) return hipSuccess;
     const uint32_t weak = weak_utf8 ? 1u : 0u;
-    if (jsonl)
-        hipLaunchKernelGGL((k_tokenize<true, false>), dim3(count), dim3(kTokThreads), 0, s, d_in,
-                           b.units, b.tile_unit, b.small_units, 0u, b.tile_st, b.ls, b.uout,
-                           out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
-    else
-        hipLaunchKernelGGL((k_tokenize<false, false>), dim3(count), dim3(kTokThreads), 0, s, d_in,
-                           b.units, b.tile_unit, b.small_units, 0u, b.tile_st, b.ls, b.uout,
-                           out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
+    if (jsonl) launch_tiles<true>(s, count, shape.tile_waves, b, d_in, out_struct, weak);
+    else launch_tiles<false>(s, count, shape.tile_waves, b, d_in, out_struct, weak);
     return hipGetLastError();
 }
 

@@ -203,6 +203,18 @@ typedef struct cjx_timings {
 int cjx_context_set_timing(cjx_context *ctx, int enable);
 int cjx_context_timings(cjx_context *ctx, cjx_timings *out, int max);
 
+/* Tokenizer launch shape of the context's later parses, chosen at run time
+ * from the library's built instantiations (the autotuner's knob).  A 64 KiB
+ * tile is one workgroup of tile_waves wave64s (2, 4, 8 or 16: 1024, 512, 256
+ * or 128 input bytes per lane); documents of at most 16 KiB take one wave
+ * each, small_waves per workgroup (1, 2, 4 or 8).  Default 4 and 4.  Results
+ * do not depend on the shape.  CJX_ERR_INPUT for other values.  No
+ * counterpart in the reference, whose block size and per-thread work are
+ * compile-time constants (BLOCKSIZE, cujson/utils.cu:41; WORDS,
+ * cujson/parse_standard_json.cu:273, :1071, :1524). */
+int cjx_context_set_tokenizer_shape(cjx_context *ctx, int tile_waves, int small_waves);
+int cjx_context_tokenizer_shape(cjx_context *ctx, int *tile_waves, int *small_waves);
+
 /* HIP stream of the context (hipStream_t), for callers that order their own
  * work against the parse. */
 void *cjx_context_stream(cjx_context *ctx);

@@ -78,6 +78,13 @@ def test_entry_points_reject_bad_arguments_without_device():
                                     ctypes.byref(r)) == cj.CJX_ERR_INPUT
 
 
+def test_tokenizer_shape_entries_reject_bad_arguments_without_device():
+    L = cj.lib()
+    assert L.cjx_context_set_tokenizer_shape(None, 4, 4) == cj.CJX_ERR_INPUT
+    t, s = ctypes.c_int(), ctypes.c_int()
+    assert L.cjx_context_tokenizer_shape(None, ctypes.byref(t), ctypes.byref(s)) == cj.CJX_ERR_INPUT
+
+
 def test_loader_entries_reject_bad_arguments_without_device():
     L = cj.lib()
     n = ctypes.c_size_t(7)
