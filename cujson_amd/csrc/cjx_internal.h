@@ -107,6 +107,7 @@ struct LaunchState {
     uint32_t n_big;     // units with more than 64 matching chunks (listed in big_units)
     uint32_t n_records;     // JSON Lines records (cjx_records.hip)
     uint32_t rec_overflow;  // more records than the caller's table holds
+    uint32_t chunk_ticket;  // dynamic matching-chunk ticket (k_match_local)
 };
 
 // One JSON Lines record (cjx_records.hip); same layout as cjx_record.

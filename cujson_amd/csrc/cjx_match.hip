@@ -287,20 +287,35 @@ __device__ __forceinline__ void chunk_scan_small_block(uint32_t bid, UnitOut *__
 // its depth; one that no lane of the chunk closes goes to highs, and a low
 // closer that no earlier lane of the chunk opened goes to lows.  Partners are
 // kept in LDS (s_part) and the chunk's pair-row cells are then written as
-// dense 4096-row tiles.
+// dense tiles, each wave over the rows of its quarter of the brackets.
 constexpr int kLaneBrackets = kMatchChunk / kMatchThreads;  // 16
 constexpr int kRowTile = 4096;
+constexpr uint32_t kWaveElems = kMatchChunk / (kMatchThreads / 64);  // a wave's brackets
+constexpr uint32_t kWaveTile = kRowTile / (kMatchThreads / 64);      // a wave's tile rows
+static_assert(kWaveElems == 64 * kLaneBrackets, "lane l of wave w: elements 1024 w + l + 64 k");
 constexpr uint16_t kNoPart = 0xFFFF;
-__global__ __launch_bounds__(kMatchThreads) void k_match_local(
+
+// LDS writes of the wave are visible to all its lanes after this.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+#ifndef CJX_MATCH_WPE
+#define CJX_MATCH_WPE 5
+#endif
+__global__ __attribute__((amdgpu_waves_per_eu(CJX_MATCH_WPE, CJX_MATCH_WPE)))
+__launch_bounds__(kMatchThreads) void k_match_local(
     const uint8_t *__restrict__ oc_char, const uint32_t *__restrict__ oc_idx,
     UnitOut *__restrict__ uo, const uint32_t *__restrict__ chunk_unit,
-    const ChunkStat *__restrict__ ck, uint32_t *__restrict__ lows, uint32_t *__restrict__ highs, int32_t *__restrict__ pair_row,
-    const LaunchState *__restrict__ ls) {
+    const ChunkStat *__restrict__ ck, uint32_t *__restrict__ lows, uint32_t *__restrict__ highs,
+    int32_t *__restrict__ pair_row, LaunchState *__restrict__ ls,
+    const UnitOut *__restrict__ uor) {
     constexpr int kW = kMatchThreads / 64;
     constexpr int kGroups = kMatchThreads / 16;
-    // The chunk's bracket rows stay in registers (vi) and in L2: a partner's
-    // row is read back from oc_idx, so no LDS copy of them limits the blocks
-    // per CU.
+    // The chunk's bracket rows stay in registers (vi); for the partners'
+    // rows they are copied into the matching state's LDS once matching is
+    // done, so no LDS of their own limits the blocks per CU.
     __shared__ __align__(16) uint8_t s_ch[kMatchChunk];
     __shared__ uint16_t s_part[kMatchChunk];
     // Stack columns and low closers during matching; the row tile afterwards
@@ -316,177 +331,214 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_local(
     __shared__ int32_t s_lst[kMatchThreads];   // lane start depth
     __shared__ int32_t s_gmin[kGroups];        // minimum over 16 lanes
     __shared__ int32_t s_wsum[kW], s_wmin[kW];
+    __shared__ uint32_t s_wrow[kW];  // row of each wave's first bracket
+    __shared__ uint32_t s_next;
 
     if (ls->overflow) return;
     const uint32_t tid = threadIdx.x, w = tid >> 6, lane = lane_id();
     const uint32_t n_chunks = ls->n_chunks;
-    for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    // Chunks are taken from a ticket counter, so blocks that start late or
+    // meet cheap chunks take more of them.  The next ticket is drawn when a
+    // chunk starts and handed over once the chunk's loads are in, so its
+    // round trip overlaps theirs.  (Fetch-and-increment: the compiler gives
+    // it no wave-wide rewrite that would wait for the result at once.)
+    if (tid == 0) s_next = atomicInc(&ls->chunk_ticket, 0xFFFFFFFFu);
+    __syncthreads();
+    for (uint32_t c = s_next; c < n_chunks;) {
+        uint32_t next = 0;
+        if (tid == 0) next = atomicInc(&ls->chunk_ticket, 0xFFFFFFFFu);
         const uint32_t u = chunk_unit[c];
-        if (!uo[u].fast) continue;
-        const uint32_t xc = c - uo[u].chunk0;
-        const uint32_t first = uo[u].m_base + xc * kMatchChunk;
-        const uint32_t cnt =
-            uo[u].m_end > first ? min((uint32_t)kMatchChunk, uo[u].m_end - first) : 0u;
-        const uint32_t nb = uo[u].n_base, pbase = uo[u].pair_base + 1u;
-        const bool last_chunk = xc + 1 == uo[u].nchunks;
-        const uint32_t r1 = last_chunk ? uo[u].n_end : oc_idx[first + kMatchChunk];
-        const int32_t d0 = ck[c].d0;
-        // All loads in flight before the first LDS store.
-        uint8_t vc[kLaneBrackets];
-        uint32_t vi[kLaneBrackets];
-#pragma unroll
-        for (int k = 0; k < kLaneBrackets; k++) {
-            const uint32_t e = tid + k * kMatchThreads;
-            vc[k] = e < cnt ? oc_char[first + e] : 0;
-            vi[k] = e < cnt ? oc_idx[first + e] : 0;
-        }
-        __syncthreads();  // LDS of the previous chunk is no longer read
-#pragma unroll
-        for (int k = 0; k < kLaneBrackets; k++) {
-            const uint32_t e = tid + k * kMatchThreads;
-            s_ch[e] = vc[k];
-            s_part[e] = kNoPart;
-        }
-        __syncthreads();
-        const uint32_t r0 = xc == 0 ? nb : oc_idx[first];
-
-        const uint32_t e0 = tid * kLaneBrackets;
-        const uint32_t ne = cnt > e0 ? min((uint32_t)kLaneBrackets, cnt - e0) : 0u;
-        const uint4 cw = *reinterpret_cast<const uint4 *>(&s_ch[e0]);
-        const uint32_t cv[4] = {cw.x, cw.y, cw.z, cw.w};
-        int32_t mine = 0;
-#pragma unroll
-        for (int k = 0; k < kLaneBrackets; k++)
-            if ((uint32_t)k < ne) mine += delta_of((uint8_t)(cv[k >> 2] >> (8 * (k & 3))));
-        // Lane start depth: exclusive scan of the lanes' net changes.
-        const int32_t incl = (int32_t)wave_incl_scan((uint32_t)mine);
-        if (lane == 63) s_wsum[w] = incl;
-        __syncthreads();
-        int32_t ds = d0 + incl - mine;
-        for (uint32_t k = 0; k < w; k++) ds += s_wsum[k];
-
-        // Stack walk of the lane.
-        uint32_t sp = 0, nlow = 0;
-        int32_t err = 0;
-#pragma unroll
-        for (int k = 0; k < kLaneBrackets; k++) {
-            if ((uint32_t)k >= ne) break;
-            const uint8_t ch = (uint8_t)(cv[k >> 2] >> (8 * (k & 3)));
-            const uint32_t e = e0 + k;
-            if (ch & 2u) {
-                s_u.m.stk[sp][tid] = (uint16_t)e;
-                sp++;
-            } else if (sp) {
-                sp--;
-                const uint32_t i = s_u.m.stk[sp][tid];
-                if ((s_ch[i] ^ ch) != 6) err = 1;
-                s_part[i] = (uint16_t)e;
-            } else {
-                s_u.m.low[nlow][tid] = (uint16_t)e;
-                nlow++;
-            }
-        }
-        const int32_t dm = ds - (int32_t)nlow;
-        s_lmin[tid] = ne ? dm : INT32_MAX;
-        s_lst[tid] = ds;
-        // Lowest depth before this lane within the chunk (chunk start included).
-        int32_t pm = ne ? dm : INT32_MAX;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int32_t o = __shfl_up(pm, off, 64);
-            if (lane >= (uint32_t)off) pm = min(pm, o);
-        }
-        if (lane == 63) s_wmin[w] = pm;
-        {
-            const int32_t o = __shfl_up(pm, 1, 64);
-            pm = lane ? o : INT32_MAX;
-        }
-        __syncthreads();
-        pm = min(pm, d0);
-        for (uint32_t k = 0; k < w; k++) pm = min(pm, s_wmin[k]);
-        if (tid < kGroups) {
-            int32_t g = INT32_MAX;
-            for (int k = 0; k < 16; k++) g = min(g, s_lmin[16 * tid + k]);
-            s_gmin[tid] = g;
-        }
-        __syncthreads();
-
-        // Openers left open by the lane.
-        for (uint32_t j = 0; j < sp; j++) {
-            const int32_t d = dm + (int32_t)j;
-            const uint32_t i = s_u.m.stk[j][tid];
-            uint32_t L = tid + 1;
-            while (L < kMatchThreads) {
-                if ((L & 15u) == 0 && s_gmin[L >> 4] > d) {
-                    L += 16;
-                    continue;
-                }
-                if (s_lmin[L] <= d) break;
-                L++;
-            }
-            if (L < kMatchThreads) {
-                const int32_t k = s_lst[L] - 1 - d;
-                const uint32_t e2 = s_u.m.low[k & (kLaneBrackets - 1)][L];
-                if ((s_ch[i] ^ s_ch[e2]) != 6) err = 1;
-                s_part[i] = (uint16_t)e2;
-            } else if ((uint32_t)d < (uint32_t)kMatchKeys) {
-                highs[(uint64_t)c * kMatchKeys + d] = first + i;
-            }
-        }
-        // Low closers that no earlier lane of the chunk opened.
-        for (uint32_t k = 0; k < nlow; k++) {
-            const int32_t d = ds - 1 - (int32_t)k;
-            if (d < pm && (uint32_t)d < (uint32_t)kMatchKeys)
-                lows[(uint64_t)c * kMatchKeys + d] = first + s_u.m.low[k][tid];
-        }
-        if (err) atomicOr(&uo[u].struct_err, 1u);
-
-        // Dense pair-row tiles over the chunk's rows [r0, r1).  Each thread
-        // keeps the rows and values of its openers (elements tid + 256k) in
-        // registers and drops those that fall in the current tile.
-        __syncthreads();  // s_part complete
-        int32_t pv[kLaneBrackets];
-#pragma unroll
-        for (int k = 0; k < kLaneBrackets; k++) {
-            const uint32_t e = tid + k * kMatchThreads;
-            const uint32_t p = s_part[e];
-            pv[k] = p != kNoPart ? (int32_t)(oc_idx[first + p] - nb + pbase) : -1;
-        }
-        // Tiles start at a multiple of 4 rows so that the fill and the store
-        // move 16 bytes per access; rows before r0 or from r1 on belong to the
-        // neighbouring chunks and are skipped (edge groups are stored by row).
-        // (Filling the cells in HBM and storing the values over them instead
-        // measured slower: 0.51 against 0.35 ms of phase 3 per 2 GB.)
-        const bool vec = (reinterpret_cast<uintptr_t>(pair_row) & 15u) == 0;
-        int4 *const rw = reinterpret_cast<int4 *>(s_u.row);
-        for (uint32_t t0 = vec ? (r0 & ~3u) : r0; t0 < r1; t0 += kRowTile) {
-            __syncthreads();  // matching state / previous tile no longer read
-            for (uint32_t i = tid; i < (uint32_t)kRowTile / 4; i += kMatchThreads)
-                rw[i] = make_int4(-1, -1, -1, -1);
-            __syncthreads();
+        const UnitOut U = uor[u];  // uniform: scalar loads, not queued behind vector stores
+        if (!U.fast) {
+            __syncthreads();  // every thread has read this chunk's ticket
+            if (tid == 0) s_next = next;
+        } else {
+            const uint32_t xc = c - U.chunk0;
+            const uint32_t first = U.m_base + xc * kMatchChunk;
+            const uint32_t cnt = U.m_end > first ? min((uint32_t)kMatchChunk, U.m_end - first) : 0u;
+            const uint32_t nb = U.n_base, pbase = U.pair_base + 1u;
+            const bool last_chunk = xc + 1 == U.nchunks;
+            const uint32_t r1 = last_chunk ? U.n_end : oc_idx[first + kMatchChunk];
+            const int32_t d0 = ck[c].d0;
+            // All loads in flight before the first LDS store.
+            uint8_t vc[kLaneBrackets];
+            uint32_t vi[kLaneBrackets];
+            // Wave w holds elements [1024 w, 1024 w + 1024), lane l elements
+            // 1024 w + l + 64 k, for the pair-row tiles at the end.
 #pragma unroll
             for (int k = 0; k < kLaneBrackets; k++) {
-                const uint32_t off = vi[k] - t0;
-                if (pv[k] >= 0 && off < (uint32_t)kRowTile) s_u.row[off] = pv[k];
+                const uint32_t e = w * kWaveElems + lane + 64u * k;
+                vc[k] = e < cnt ? oc_char[first + e] : 0;
+                vi[k] = e < cnt ? oc_idx[first + e] : 0;
+            }
+            __syncthreads();  // LDS of the previous chunk is no longer read
+            if (tid == 0) s_next = next;
+#pragma unroll
+            for (int k = 0; k < kLaneBrackets; k++) {
+                const uint32_t e = w * kWaveElems + lane + 64u * k;
+                s_ch[e] = vc[k];
+                s_part[e] = kNoPart;
+            }
+            if (lane == 0) s_wrow[w] = vi[0];  // the wave's first bracket row
+            __syncthreads();
+            const uint32_t r0 = xc == 0 ? nb : s_wrow[0];  // row of the chunk's first bracket
+
+            const uint32_t e0 = tid * kLaneBrackets;
+            const uint32_t ne = cnt > e0 ? min((uint32_t)kLaneBrackets, cnt - e0) : 0u;
+            const uint4 cw = *reinterpret_cast<const uint4 *>(&s_ch[e0]);
+            const uint32_t cv[4] = {cw.x, cw.y, cw.z, cw.w};
+            int32_t mine = 0;
+#pragma unroll
+            for (int k = 0; k < kLaneBrackets; k++)
+                if ((uint32_t)k < ne) mine += delta_of((uint8_t)(cv[k >> 2] >> (8 * (k & 3))));
+            // Lane start depth: exclusive scan of the lanes' net changes.
+            const int32_t incl = (int32_t)wave_incl_scan((uint32_t)mine);
+            if (lane == 63) s_wsum[w] = incl;
+            __syncthreads();
+            int32_t ds = d0 + incl - mine;
+            for (uint32_t k = 0; k < w; k++) ds += s_wsum[k];
+
+            // Stack walk of the lane.
+            uint32_t sp = 0, nlow = 0;
+            int32_t err = 0;
+#pragma unroll
+            for (int k = 0; k < kLaneBrackets; k++) {
+                if ((uint32_t)k >= ne) break;
+                const uint8_t ch = (uint8_t)(cv[k >> 2] >> (8 * (k & 3)));
+                const uint32_t e = e0 + k;
+                if (ch & 2u) {
+                    s_u.m.stk[sp][tid] = (uint16_t)e;
+                    sp++;
+                } else if (sp) {
+                    sp--;
+                    const uint32_t i = s_u.m.stk[sp][tid];
+                    if ((s_ch[i] ^ ch) != 6) err = 1;
+                    s_part[i] = (uint16_t)e;
+                } else {
+                    s_u.m.low[nlow][tid] = (uint16_t)e;
+                    nlow++;
+                }
+            }
+            const int32_t dm = ds - (int32_t)nlow;
+            s_lmin[tid] = ne ? dm : INT32_MAX;
+            s_lst[tid] = ds;
+            // Lowest depth before this lane within the chunk (chunk start included).
+            int32_t pm = ne ? dm : INT32_MAX;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int32_t o = __shfl_up(pm, off, 64);
+                if (lane >= (uint32_t)off) pm = min(pm, o);
+            }
+            if (lane == 63) s_wmin[w] = pm;
+            {
+                const int32_t o = __shfl_up(pm, 1, 64);
+                pm = lane ? o : INT32_MAX;
             }
             __syncthreads();
-            const uint32_t hi = min(r1, t0 + (uint32_t)kRowTile);
-            if (vec) {
-                for (uint32_t g = tid; g < (uint32_t)kRowTile / 4; g += kMatchThreads) {
-                    const uint32_t a = t0 + 4u * g;
-                    if (a >= hi) break;
-                    if (a >= r0 && a + 4u <= hi) {
-                        *reinterpret_cast<int4 *>(pair_row + a) = rw[g];
-                    } else {
-#pragma unroll
-                        for (uint32_t j = 0; j < 4; j++)
-                            if (a + j >= r0 && a + j < hi) pair_row[a + j] = s_u.row[4u * g + j];
+            pm = min(pm, d0);
+            for (uint32_t k = 0; k < w; k++) pm = min(pm, s_wmin[k]);
+            if (tid < kGroups) {
+                int32_t g = INT32_MAX;
+                for (int k = 0; k < 16; k++) g = min(g, s_lmin[16 * tid + k]);
+                s_gmin[tid] = g;
+            }
+            __syncthreads();
+
+            // Openers left open by the lane.
+            for (uint32_t j = 0; j < sp; j++) {
+                const int32_t d = dm + (int32_t)j;
+                const uint32_t i = s_u.m.stk[j][tid];
+                uint32_t L = tid + 1;
+                while (L < kMatchThreads) {
+                    if ((L & 15u) == 0 && s_gmin[L >> 4] > d) {
+                        L += 16;
+                        continue;
                     }
+                    if (s_lmin[L] <= d) break;
+                    L++;
                 }
-            } else {
-                for (uint32_t i = tid; t0 + i < hi; i += kMatchThreads) pair_row[t0 + i] = s_u.row[i];
+                if (L < kMatchThreads) {
+                    const int32_t k = s_lst[L] - 1 - d;
+                    const uint32_t e2 = s_u.m.low[k & (kLaneBrackets - 1)][L];
+                    if ((s_ch[i] ^ s_ch[e2]) != 6) err = 1;
+                    s_part[i] = (uint16_t)e2;
+                } else if ((uint32_t)d < (uint32_t)kMatchKeys) {
+                    highs[(uint64_t)c * kMatchKeys + d] = first + i;
+                }
+            }
+            // Low closers that no earlier lane of the chunk opened.
+            for (uint32_t k = 0; k < nlow; k++) {
+                const int32_t d = ds - 1 - (int32_t)k;
+                if (d < pm && (uint32_t)d < (uint32_t)kMatchKeys)
+                    lows[(uint64_t)c * kMatchKeys + d] = first + s_u.m.low[k][tid];
+            }
+            if (err) atomicOr(&uo[u].struct_err, 1u);
+
+            // Dense pair-row tiles over the chunk's rows [r0, r1): wave w
+            // writes the rows from its first bracket's row to the next wave's
+            // (from r0 for wave 0, to r1 for the last wave with brackets) in
+            // tiles of 1024 rows of its own, so the tiles need no workgroup
+            // barrier.  The lanes keep the rows and values of their openers in
+            // registers and drop those that fall in the current tile.
+            __syncthreads();  // s_part complete, the matching state no longer read
+            // The chunk's bracket rows go to LDS over the matching state, for
+            // the partners' rows.
+#pragma unroll
+            for (int k = 0; k < kLaneBrackets; k++)
+                s_u.row[w * kWaveElems + lane + 64u * k] = (int32_t)vi[k];
+            __syncthreads();
+            int32_t pv[kLaneBrackets];
+#pragma unroll
+            for (int k = 0; k < kLaneBrackets; k++) {
+                const uint32_t e = w * kWaveElems + lane + 64u * k;
+                const uint32_t p = s_part[e];
+                pv[k] = p != kNoPart ? (int32_t)((uint32_t)s_u.row[p] - nb + pbase) : -1;
+            }
+            __syncthreads();  // the rows are read; the tiles reuse their space
+            const uint32_t wbeg = w * kWaveElems;
+            const uint32_t wr0 = w == 0 ? r0 : (wbeg < cnt ? s_wrow[w] : r1);
+            const uint32_t wr1 = wbeg + kWaveElems < cnt ? s_wrow[w + 1] : r1;
+            // Tiles start at a multiple of 4 rows so that the fill and the store
+            // move 16 bytes per access; rows before wr0 or from wr1 on belong to
+            // the neighbouring waves or chunks and are skipped (edge groups are
+            // stored by row).  (Filling the cells in HBM and storing the values
+            // over them instead measured slower: 0.51 against 0.35 ms of phase 3
+            // per 2 GB.)
+            const bool vec = (reinterpret_cast<uintptr_t>(pair_row) & 15u) == 0;
+            int32_t *const tile = s_u.row + w * kWaveTile;
+            int4 *const tw = reinterpret_cast<int4 *>(tile);
+            for (uint32_t t0 = vec ? (wr0 & ~3u) : wr0; t0 < wr1; t0 += kWaveTile) {
+                wave_lds_sync();  // the previous tile is no longer read
+#pragma unroll
+                for (uint32_t i = lane; i < (uint32_t)kWaveTile / 4; i += 64)
+                    tw[i] = make_int4(-1, -1, -1, -1);
+                wave_lds_sync();
+#pragma unroll
+                for (int k = 0; k < kLaneBrackets; k++) {
+                    const uint32_t off = vi[k] - t0;
+                    if (pv[k] >= 0 && off < (uint32_t)kWaveTile) tile[off] = pv[k];
+                }
+                wave_lds_sync();
+                const uint32_t hi = min(wr1, t0 + (uint32_t)kWaveTile);
+                if (vec) {
+                    for (uint32_t g = lane; g < (uint32_t)kWaveTile / 4; g += 64) {
+                        const uint32_t a = t0 + 4u * g;
+                        if (a >= hi) break;
+                        if (a >= wr0 && a + 4u <= hi) {
+                            *reinterpret_cast<int4 *>(pair_row + a) = tw[g];
+                        } else {
+#pragma unroll
+                            for (uint32_t j = 0; j < 4; j++)
+                                if (a + j >= wr0 && a + j < hi) pair_row[a + j] = tile[4u * g + j];
+                        }
+                    }
+                } else {
+                    for (uint32_t i = lane; t0 + i < hi; i += 64) pair_row[t0 + i] = tile[i];
+                }
             }
         }
+        __syncthreads();  // the next ticket is in s_next
+        c = s_next;
     }
 }
 
@@ -747,8 +799,11 @@ hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, ui
     hipLaunchKernelGGL(k_chunk_scans, dim3(gb + (n_units + kScanThreads / 64 - 1) / (kScanThreads / 64)),
                        dim3(kScanThreads), 0, s, b.uout, n_units, b.ck, b.gmin, b.gmin1, b.gmin2,
                        b.gagg, b.gd0, b.big_units, b.ls, gb);
+    // uo twice: the kernel's fields as read-only (scalar loads) and the
+    // error flag it sets; it writes no field that it reads.
     hipLaunchKernelGGL(k_match_local, dim3(g), dim3(kMatchThreads), 0, s, b.oc_char, b.oc_idx,
-                       b.uout, b.chunk_unit, b.ck, b.lows, b.highs, pair_row, b.ls);
+                       b.uout, b.chunk_unit, b.ck, b.lows, b.highs, pair_row, b.ls,
+                       (const UnitOut *)b.uout);
     hipLaunchKernelGGL(k_match_cross, dim3(gc), dim3(64), 0, s, b.oc_char, b.oc_idx, b.uout,
                        b.chunk_unit, b.ck, b.gmin, b.gmin1, b.gmin2, b.lows, b.highs, pair_row,
                        b.ls);
