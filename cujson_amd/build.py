@@ -15,7 +15,8 @@ LIB = os.path.join(HERE, "libcujson_amd.so")
 ARCH = os.environ.get("CJX_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["cjx_tokenize.hip", "cjx_utf8.hip", "cjx_kernels.hip", "cjx_match.hip", "cjx_records.hip",
-           "cjx_lines.hip", "cjx_values.hip", "cjx_noalloc.hip", "cjx_api.cpp", "cjx_server.cpp"]
+           "cjx_lines.hip", "cjx_values.hip", "cjx_noalloc.hip", "cjx_api.cpp", "cjx_chunk.cpp",
+           "cjx_server.cpp"]
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     # packed fp32 VALU ops are disabled for every HIP object of this library

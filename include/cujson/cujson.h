@@ -13,7 +13,11 @@ inline cuJSONResult parse_standard_json(cuJSONInput input) {
         return cuJSONResult{};
     }
     cjx_context *ctx = cujson_amd_detail::context();
-    if (!ctx) return cuJSONResult{};
+    if (!ctx) {  // no device: the input is released all the same
+        cjx_input in{input.data, input.size};
+        cjx_free_input(&in);
+        return cuJSONResult{};
+    }
     cjx_result r{};
     const int rc = cjx_parse_standard(ctx, input.data, input.size, 0, &r);
     cjx_input in{input.data, input.size};

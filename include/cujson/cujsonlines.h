@@ -16,7 +16,11 @@ inline cuJSONResult parse_json_lines(cuJSONLinesInput input) {
         return cuJSONResult{};
     }
     cjx_context *ctx = cujson_amd_detail::context();
-    if (!ctx) return cuJSONResult{};
+    if (!ctx) {  // no device: the input is released all the same
+        cjx_input in{input.data, input.size};
+        cjx_free_input(&in);
+        return cuJSONResult{};
+    }
     std::vector<size_t> off(input.chunkCount);
     for (size_t i = 0; i < input.chunkCount; i++) off[i] = (size_t)(input.chunks[i] - input.data);
     cjx_result r{};
