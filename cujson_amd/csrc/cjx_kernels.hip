@@ -49,8 +49,8 @@ __global__ void k_reset_one(UnitDesc u, UnitDesc *units, uint32_t *tile_unit, ui
 // lastStructuralIndex carry, parse_json_lines.cu:1222); row_base is that
 // count always; chunk0, grp0 and sgrp0 number the units' matching chunks and
 // their 64- and 4096-chunk groups consecutively.  Units with at most
-// kSmallBrackets brackets have no chunks (k_match_small pairs them in one
-// wave).  Three launches: block sums, a scan of the block sums, and a final
+// kSmallBrackets brackets have no chunks (k_match_front's wave matcher
+// pairs them in one wave).  Three launches: block sums, a scan of the block sums, and a final
 // pass that writes the fields, the chunk -> unit map of units with up to 64
 // chunks and the list of larger units (whose maps k_chunk_map writes).
 constexpr int kBaseThreads = 256;

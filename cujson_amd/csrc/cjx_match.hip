@@ -255,8 +255,8 @@ __device__ __forceinline__ void chunk_scan_small_block(uint32_t bid, UnitOut *__
     const uint32_t u = bid * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (u >= n_units) return;
     const uint32_t nch = uo[u].nchunks;
-    if (nch > 64) return;  // k_chunk_scan
-    if (nch == 0) return;  // k_match_small decides these units
+    if (nch > 64) return;  // chunk_scan_blocks
+    if (nch == 0) return;  // match_small_block decides these units
     const uint32_t c0 = uo[u].chunk0;
     ChunkStat st{0, 0, 0, 0};
     if (lane < nch) st = ck[c0 + lane];
