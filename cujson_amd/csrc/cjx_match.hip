@@ -444,11 +444,14 @@ __launch_bounds__(kMatchThreads) void k_match_local(
             }
             __syncthreads();
 
-            // Openers left open by the lane.
-            for (uint32_t j = 0; j < sp; j++) {
-                const int32_t d = dm + (int32_t)j;
+            // Openers left open by the lane, the deepest first.  The first later
+            // lane that reaches depth d never comes before the first that
+            // reaches d + 1, so one forward walk over the lanes serves them all
+            // (a nested run of openers costs one walk, not one per opener).
+            uint32_t L = tid + 1;
+            for (int32_t j = (int32_t)sp - 1; j >= 0; j--) {
+                const int32_t d = dm + j;
                 const uint32_t i = s_u.m.stk[j][tid];
-                uint32_t L = tid + 1;
                 while (L < kMatchThreads) {
                     if ((L & 15u) == 0 && s_gmin[L >> 4] > d) {
                         L += 16;
