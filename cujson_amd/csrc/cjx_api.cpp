@@ -191,6 +191,7 @@ int ensure_work(cjx_context *ctx, const Shape &sh) {
               ctx->highs.ensure(sh.chunks * kMatchKeys * 4 + 16) &&
               ctx->chunk_unit.ensure(sh.chunks * 4 + 16) &&
               ctx->big_units.ensure((size_t)sh.n_units * 4 + 16) &&
+              ctx->slow_units.ensure((size_t)sh.n_units * 4 + 16) &&
               ctx->unit_part.ensure(((size_t)sh.n_units / 256 + 2) * 32) &&
               ctx->h_units.ensure(sh.n_units * sizeof(UnitDesc)) &&
               ctx->h_uout.ensure(sh.n_units * sizeof(UnitOut)) &&
@@ -240,6 +241,7 @@ LaunchBufs bufs_of(cjx_context *ctx) {
     b.small_units = ctx->small_units;
     b.chunk_unit = ctx->chunk_unit.as<uint32_t>();
     b.big_units = ctx->big_units.as<uint32_t>();
+    b.slow_units = ctx->slow_units.as<uint32_t>();
     b.unit_part = ctx->unit_part.as<uint32_t>();
     b.uout = ctx->uout.as<UnitOut>();
     b.ls = ctx->ls.as<LaunchState>();
@@ -264,6 +266,36 @@ LaunchBufs bufs_of(cjx_context *ctx) {
     return b;
 }
 
+// The second stream and its events (created on first use); false when the
+// context runs everything on its own stream.
+bool ensure_aux(cjx_context *ctx) {
+    if (!ctx->fork_ok) return false;
+    if (ctx->aux) return true;
+    for (int i = 0; i < 2; i++)
+        if ((!ctx->fork_ev[i] &&
+             hipEventCreateWithFlags(&ctx->fork_ev[i], hipEventDisableTiming) != hipSuccess) ||
+            (!ctx->join_ev[i] &&
+             hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming) != hipSuccess))
+            return false;
+    if (hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess) {
+        ctx->aux = nullptr;
+        return false;
+    }
+    return true;
+}
+
+// aux waits for everything enqueued on the context stream so far.
+bool fork_aux(cjx_context *ctx, int k) {
+    return hipEventRecord(ctx->fork_ev[k], ctx->stream) == hipSuccess &&
+           hipStreamWaitEvent(ctx->aux, ctx->fork_ev[k], 0) == hipSuccess;
+}
+
+// The context stream waits for everything enqueued on aux so far.
+bool join_aux(cjx_context *ctx, int k) {
+    return hipEventRecord(ctx->join_ev[k], ctx->aux) == hipSuccess &&
+           hipStreamWaitEvent(ctx->stream, ctx->join_ev[k], 0) == hipSuccess;
+}
+
 // Phase 3 of a launch (unit bases, matching, sort path) on the context
 // stream, after the tokenizer.
 int enqueue_back(cjx_context *ctx, const Shape &sh, unsigned flags, int32_t *d_pair) {
@@ -273,16 +305,26 @@ int enqueue_back(cjx_context *ctx, const Shape &sh, unsigned flags, int32_t *d_p
     // Concatenated results number pairs across units (JSON Lines).
     if (launch_unit_bases(s, b, sh.n_units, sh.concat) != hipSuccess) return CJX_ERR_DEVICE;
     // Well-nested units are matched directly; the rest take the sort path.
-    if (launch_match(s, b, sh.n_units, (uint32_t)sh.chunks, (uint32_t)sh.groups, d_pair) !=
-        hipSuccess)
+    // Once the walks have told them apart, the chunk matcher and the sort
+    // path touch different units and can run side by side.  That pays for
+    // batches of independent documents, whose faulty and deep members give
+    // the sort path real work; a single unit or JSON Lines chunks, whose sort
+    // path finds nothing to do, keep one stream (the fork and join cost more
+    // than the overlap: 10-20 us of launch gaps measured).
+    if (launch_match_front(s, b, sh.n_units, (uint32_t)sh.chunks, d_pair) != hipSuccess)
         return CJX_ERR_DEVICE;
+    const bool fork = sh.n_units > 1 && !sh.concat && ensure_aux(ctx);
+    if (fork && !fork_aux(ctx, 1)) return CJX_ERR_DEVICE;
+    hipStream_t ss = fork ? ctx->aux : s;
     if (sh.n_units == 1) {  // one unit: the multi-block sort path
         const uint32_t G = phase3_blocks(sh.max_rows);
         const uint32_t PG = pair_blocks(sh.max_rows);
-        if (launch_phase3(s, b, 0, G, PG, u16, d_pair) != hipSuccess) return CJX_ERR_DEVICE;
-    } else if (launch_sort_units(s, b, sh.n_units, u16, d_pair) != hipSuccess) {
+        if (launch_phase3(ss, b, 0, G, PG, u16, d_pair) != hipSuccess) return CJX_ERR_DEVICE;
+    } else if (launch_sort_units(ss, b, sh.n_units, u16, d_pair) != hipSuccess) {
         return CJX_ERR_DEVICE;
     }
+    if (launch_match_local(s, b, (uint32_t)sh.chunks, d_pair) != hipSuccess) return CJX_ERR_DEVICE;
+    if (fork && !join_aux(ctx, 1)) return CJX_ERR_DEVICE;
     if (ctx->timing) {
         hipEvent_t *tev = ctx->tev[ctx->timing_count % cjx_context::kTimingRing];
         (void)hipEventRecord(tev[2], s);
@@ -305,11 +347,30 @@ int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsign
     hipEvent_t *tev = ctx->tev[ctx->timing_count % cjx_context::kTimingRing];
     if (ctx->timing) (void)hipEventRecord(tev[0], s);
     const bool strict = (flags & CJX_FLAG_UTF8_STRICT) != 0;
-    if (strict && launch_utf8_strict(s, b, d_in, sh.ntiles, sh.n_small) != hipSuccess)
+    const bool jsonl = (flags & CJX_FLAG_JSONL) != 0;
+    // Beside the tile tokenizer on the second stream: the strict check, which
+    // shares no output with it (it only sets the units' UTF-8 flags, which the
+    // tokenizer leaves alone in strict mode; the tokenizer is bound by its
+    // instruction issue, the check by HBM), and in document batches the small
+    // units' waves (their units are not the tiles' units; measured 4-8 us
+    // faster per 50,000-document batch than after the tiles).
+    const bool two = sh.ntiles && sh.n_small && sh.n_units > 1 && !sh.concat;
+    const bool fork = (strict || two) && ensure_aux(ctx);
+    if (fork && !fork_aux(ctx, 0)) return CJX_ERR_DEVICE;
+    if (strict && launch_utf8_strict(fork ? ctx->aux : s, b, d_in, sh.ntiles, sh.n_small) !=
+                      hipSuccess)
         return CJX_ERR_DEVICE;
-    if (launch_tokenize(s, b, d_in, sh.ntiles, sh.n_small, (flags & CJX_FLAG_JSONL) != 0,
-                        !strict, d_struct, ctx->tok_shape) != hipSuccess)
+    if (fork && two) {
+        if (launch_tokenize(ctx->aux, b, d_in, 0, sh.n_small, jsonl, !strict, d_struct,
+                            ctx->tok_shape) != hipSuccess ||
+            launch_tokenize(s, b, d_in, sh.ntiles, 0, jsonl, !strict, d_struct,
+                            ctx->tok_shape) != hipSuccess)
+            return CJX_ERR_DEVICE;
+    } else if (launch_tokenize(s, b, d_in, sh.ntiles, sh.n_small, jsonl, !strict, d_struct,
+                               ctx->tok_shape) != hipSuccess) {
         return CJX_ERR_DEVICE;
+    }
+    if (fork && !join_aux(ctx, 0)) return CJX_ERR_DEVICE;
     if (ctx->timing) (void)hipEventRecord(tev[1], s);
     return enqueue_back(ctx, sh, flags, d_pair);
 }
@@ -914,7 +975,8 @@ void cjx_context_destroy(cjx_context *ctx) {
                       &ctx->tmp,     &ctx->spos,       &ctx->block_delta, &ctx->hist,
                       &ctx->ck,      &ctx->gmin,       &ctx->gmin1,  &ctx->gmin2,
                       &ctx->gagg,    &ctx->gd0,        &ctx->lows,   &ctx->highs,
-                      &ctx->chunk_unit, &ctx->big_units, &ctx->unit_part, &ctx->rec_count,
+                      &ctx->chunk_unit, &ctx->big_units, &ctx->slow_units, &ctx->unit_part,
+                      &ctx->rec_count,
                       &ctx->recs,       &ctx->row_nl,   &ctx->nl_seg, &ctx->nl_off,
                       &ctx->nl_idx,     &ctx->val_exact, &ctx->span};
     for (DevBuf *b : bufs) b->release();
@@ -941,6 +1003,14 @@ void cjx_context_destroy(cjx_context *ctx) {
         (void)hipStreamSynchronize(ctx->out_stream);
         (void)hipStreamDestroy(ctx->out_stream);
     }
+    if (ctx->aux) {
+        (void)hipStreamSynchronize(ctx->aux);
+        (void)hipStreamDestroy(ctx->aux);
+    }
+    for (hipEvent_t e : ctx->fork_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->join_ev)
+        if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->copy_ev)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->part_ev)

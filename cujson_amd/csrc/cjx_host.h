@@ -80,7 +80,8 @@ struct cjx_context {
     cjx::TokShape tok_shape;  // tokenizer launch shape (cjx_context_set_tokenizer_shape)
     DevBuf in, structural, pair;
     DevBuf span;  // host units copied as one span of their source, before placement
-    DevBuf units, uout, ls, tile_st, layout, chunk_unit, big_units, unit_part, rec_count, recs;
+    DevBuf units, uout, ls, tile_st, layout, chunk_unit, big_units, slow_units, unit_part,
+        rec_count, recs;
     uint32_t *tile_unit = nullptr;    // inside `units`, after the unit descriptors
     uint32_t *small_units = nullptr;  // after the tile -> unit map
     uint64_t *unit_src = nullptr;     // after those: source offsets of relocated units
@@ -107,6 +108,15 @@ struct cjx_context {
     // lands in h_part, and the rows go out on out_stream.
     hipStream_t out_stream = nullptr;
     hipEvent_t part_ev[kCopyEvents] = {};
+    // Work of a launch that does not depend on the kernels beside it runs on
+    // a second stream, forked from and joined back into the context stream
+    // by events (created on first use): the strict UTF-8 check beside the
+    // tokenizer, and in document batches the small units' waves beside the
+    // tiles and the sort path beside the chunk matcher.  Off (everything on
+    // one stream) when fork_ok is false.
+    bool fork_ok = true;
+    hipStream_t aux = nullptr;
+    hipEvent_t fork_ev[2] = {}, join_ev[2] = {};
     PinnedBuf h_part;
     // Newline scan of device text: segment counts, bases and status; the
     // newline offsets and gathered line starts of the device chunker.

@@ -108,6 +108,7 @@ struct LaunchState {
     uint32_t n_records;     // JSON Lines records (cjx_records.hip)
     uint32_t rec_overflow;  // more records than the caller's table holds
     uint32_t chunk_ticket;  // dynamic matching-chunk ticket (k_match_local)
+    uint32_t n_slow;        // units that are not well nested (listed in slow_units)
 };
 
 // One JSON Lines record (cjx_records.hip); same layout as cjx_record.
@@ -134,6 +135,7 @@ struct LaunchBufs {
     uint32_t *small_units; // units of at most kWaveBytes bytes (host-built)
     uint32_t *chunk_unit;  // unit of every matching chunk (device-built)
     uint32_t *big_units;   // units with more than 64 matching chunks
+    uint32_t *slow_units;  // units found not well nested, in no order (the sort path's list)
     uint32_t *unit_part;   // per 256 units: 4 block sums of the unit-base scan
     UnitOut *uout;
     LaunchState *ls;
@@ -194,6 +196,13 @@ hipError_t launch_utf8_strict(hipStream_t s, const LaunchBufs &b, const uint8_t 
 hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool concat);
 hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, uint32_t chunk_bound,
                         uint32_t group_bound, int32_t *pair_row);
+// launch_match in two parts: the chunk walks, the small units' wave matcher
+// and the chunk scans (which decide which units are well nested), then the
+// chunk matcher and the cross-chunk pairs.
+hipError_t launch_match_front(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
+                              uint32_t chunk_bound, int32_t *pair_row);
+hipError_t launch_match_local(hipStream_t s, const LaunchBufs &b, uint32_t chunk_bound,
+                              int32_t *pair_row);
 // Packs the units' row regions densely: structural row i of unit u goes to
 // dst_struct[row_base_u + i], its pair cell to dst_pair[row_base_u + i]; with
 // host_layout, dst_struct = dst + 1 and dst_pair = dst + N + 2 of one 2(N+2)
@@ -210,8 +219,9 @@ hipError_t launch_assemble_docs(hipStream_t s, const LaunchBufs &b, uint32_t n_u
 // pinned host memory); n1 = 0 for one range.
 hipError_t launch_copy(hipStream_t s, const void *src0, void *dst0, uint64_t n0, const void *src1,
                        void *dst1, uint64_t n1);
-// Sort-path phase 3 for every unit that is not well nested, one block per
-// unit (launches with several units).
+// Sort-path phase 3 for every unit that is not well nested (listed in
+// slow_units by the matching kernels), one block per unit at a time
+// (launches with several units).
 hipError_t launch_sort_units(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool depth_u16,
                              int32_t *pair_row);
 // Per-record table of JSON Lines units: write = false counts the records of

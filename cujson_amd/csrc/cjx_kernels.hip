@@ -570,7 +570,7 @@ __global__ __launch_bounds__(kP3Threads) void k_sort_unit(
     const uint8_t *__restrict__ oc_char, const uint32_t *__restrict__ oc_idx,
     uint16_t *__restrict__ keys, uint32_t *__restrict__ sorted, uint32_t *__restrict__ tmp,
     uint32_t *__restrict__ spos, UnitOut *__restrict__ uo, int32_t *__restrict__ pair_row,
-    const LaunchState *__restrict__ ls) {
+    const LaunchState *__restrict__ ls, const uint32_t *__restrict__ slow_units) {
     constexpr int kW = kP3Threads / 64;
     __shared__ uint32_t h[2][256];
     __shared__ uint32_t run[256];
@@ -579,8 +579,12 @@ __global__ __launch_bounds__(kP3Threads) void k_sort_unit(
     __shared__ int32_t row[kPairChunk];
     __shared__ uint32_t s_m0, s_m1;
     if (ls->overflow) return;
-    const uint32_t u = blockIdx.x;
-    if (uo[u].fast) return;
+    // The units the matching kernels found not well nested, one per block
+    // at a time (the others need no sort and cost no block).
+    const uint32_t n_slow = ls->n_slow;
+    for (uint32_t q = blockIdx.x; q < n_slow; q += gridDim.x) {
+    __syncthreads();  // the previous unit's LDS is no longer read
+    const uint32_t u = slow_units[q];
     const uint32_t tid = threadIdx.x, w = tid >> 6;
     const uint64_t lt = lanemask_lt();
     const uint32_t kmask = U16 ? 0xFFFFu : 0xFFu;
@@ -706,17 +710,23 @@ __global__ __launch_bounds__(kP3Threads) void k_sort_unit(
     }
     err = block_reduce_sum(err, wsum);
     if (tid == 0 && err) atomicOr(&uo[u].struct_err, 1u);
+    }
 }
 
 hipError_t launch_sort_units(hipStream_t s, const LaunchBufs &b, uint32_t n_units, bool depth_u16,
                              int32_t *pair_row) {
     if (n_units == 0) return hipSuccess;
+    // The list's length is known on the device only: a grid that fills the
+    // chip walks it.
+    const uint32_t g = n_units < 2048u ? n_units : 2048u;
     if (depth_u16)
-        hipLaunchKernelGGL(k_sort_unit<true>, dim3(n_units), dim3(kP3Threads), 0, s, b.oc_char,
-                           b.oc_idx, b.keys, b.sorted, b.tmp, b.spos, b.uout, pair_row, b.ls);
+        hipLaunchKernelGGL(k_sort_unit<true>, dim3(g), dim3(kP3Threads), 0, s, b.oc_char,
+                           b.oc_idx, b.keys, b.sorted, b.tmp, b.spos, b.uout, pair_row, b.ls,
+                           b.slow_units);
     else
-        hipLaunchKernelGGL(k_sort_unit<false>, dim3(n_units), dim3(kP3Threads), 0, s, b.oc_char,
-                           b.oc_idx, b.keys, b.sorted, b.tmp, b.spos, b.uout, pair_row, b.ls);
+        hipLaunchKernelGGL(k_sort_unit<false>, dim3(g), dim3(kP3Threads), 0, s, b.oc_char,
+                           b.oc_idx, b.keys, b.sorted, b.tmp, b.spos, b.uout, pair_row, b.ls,
+                           b.slow_units);
     return hipGetLastError();
 }
 

@@ -41,6 +41,11 @@ __device__ __forceinline__ DepthAgg agg_compose(const DepthAgg &a, const DepthAg
     return r;
 }
 
+// Lists a unit that is not well nested for the sort path (k_sort_unit).
+__device__ __forceinline__ void list_slow(LaunchState *ls, uint32_t *slow_units, uint32_t u) {
+    slow_units[atomicAdd(&ls->n_slow, 1u)] = u;
+}
+
 __device__ __forceinline__ int32_t delta_of(uint8_t c) {
     // '[' 0x5B, '{' 0x7B open; ']' 0x5D, '}' 0x7D close (bit 1 tells them apart).
     return (c & 2u) ? 1 : -1;
@@ -143,7 +148,7 @@ __device__ __forceinline__ void chunk_scan_blocks(
     uint32_t bid, uint32_t nblk, UnitOut *__restrict__ uo, ChunkStat *__restrict__ ck, int32_t *__restrict__ gmin,
     int32_t *__restrict__ gmin1, int32_t *__restrict__ gmin2, DepthAgg *__restrict__ gagg,
     int32_t *__restrict__ gd0, const uint32_t *__restrict__ big_units,
-    const LaunchState *__restrict__ ls) {
+    LaunchState *__restrict__ ls, uint32_t *__restrict__ slow_units) {
     constexpr int kW = kScanThreads / 64;
     __shared__ DepthAgg part[kW];
     if (ls->overflow) return;
@@ -203,7 +208,9 @@ __device__ __forceinline__ void chunk_scan_blocks(
     }
     if (tid == kScanThreads - 1) {
         const DepthAgg t = agg_compose(pre, incl);
-        uo[u].fast = (t.d == 0 && t.mn >= 0 && t.mx <= kMatchMaxDepth) ? 1u : 0u;
+        const bool fast = t.d == 0 && t.mn >= 0 && t.mx <= kMatchMaxDepth;
+        uo[u].fast = fast ? 1u : 0u;
+        if (!fast) list_slow(ls, slow_units, u);
     }
     __syncthreads();
     // 4096-chunk minima.
@@ -249,7 +256,8 @@ __device__ __forceinline__ void chunk_scan_small_block(uint32_t bid, UnitOut *__
                                                           int32_t *__restrict__ gmin,
                                                           int32_t *__restrict__ gmin1,
                                                           int32_t *__restrict__ gmin2,
-                                                          const LaunchState *__restrict__ ls) {
+                                                          LaunchState *__restrict__ ls,
+                                                          uint32_t *__restrict__ slow_units) {
     if (ls->overflow) return;
     const uint32_t lane = lane_id();
     const uint32_t u = bid * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -274,7 +282,9 @@ __device__ __forceinline__ void chunk_scan_small_block(uint32_t bid, UnitOut *__
     if (lane == 0) {
         gmin1[uo[u].grp0] = m;
         gmin2[uo[u].sgrp0] = m;
-        uo[u].fast = (t.d == 0 && t.mn >= 0 && t.mx <= kMatchMaxDepth) ? 1u : 0u;
+        const bool fast = t.d == 0 && t.mn >= 0 && t.mx <= kMatchMaxDepth;
+        uo[u].fast = fast ? 1u : 0u;
+        if (!fast) list_slow(ls, slow_units, u);
     }
 }
 
@@ -561,7 +571,8 @@ __device__ __forceinline__ void match_small_block(uint32_t bid, const uint8_t *_
                                                      const uint32_t *__restrict__ oc_idx,
                                                      UnitOut *__restrict__ uo, uint32_t n_units,
                                                      int32_t *__restrict__ pair_row,
-                                                     const LaunchState *__restrict__ ls) {
+                                                     LaunchState *__restrict__ ls,
+                                                     uint32_t *__restrict__ slow_units) {
     __shared__ uint32_t s_nrow[4][kMatchKeys];  // per key: row of the next bracket after the group
     __shared__ uint8_t s_nch[4][kMatchKeys];    // and its character
     if (ls->overflow) return;
@@ -589,7 +600,10 @@ __device__ __forceinline__ void match_small_block(uint32_t bid, const uint8_t *_
         bad = bad || (has && (incl < 0 || incl > kMatchMaxDepth));
         key[g] = (uint32_t)(d > 0 ? incl - 1 : incl) & 0xFFu;  // exact depth, in [0, 255]
     }
-    if (__ballot(bad) || carry != 0) return;  // not well nested
+    if (__ballot(bad) || carry != 0) {  // not well nested
+        if (lane == 0) list_slow(ls, slow_units, u);
+        return;
+    }
     for (uint32_t k = lane; k < (uint32_t)kMatchKeys; k += 64) s_nrow[w][k] = 0;
     int32_t pval[kWaveGroups];
     bool err = false;
@@ -770,38 +784,53 @@ __global__ __launch_bounds__(64) void k_match_cross(
 __global__ __launch_bounds__(kMatchThreads) void k_match_front(
     const uint8_t *__restrict__ oc_char, const uint32_t *__restrict__ oc_idx,
     UnitOut *__restrict__ uo, uint32_t n_units, const uint32_t *__restrict__ chunk_unit,
-    ChunkStat *__restrict__ ck, int32_t *__restrict__ pair_row, const LaunchState *__restrict__ ls,
-    uint32_t g_stats) {
+    ChunkStat *__restrict__ ck, int32_t *__restrict__ pair_row, LaunchState *__restrict__ ls,
+    uint32_t g_stats, uint32_t *__restrict__ slow_units) {
     if (blockIdx.x < g_stats)
         chunk_stats_blocks(blockIdx.x, g_stats, oc_char, uo, chunk_unit, ck, ls);
     else
-        match_small_block(blockIdx.x - g_stats, oc_char, oc_idx, uo, n_units, pair_row, ls);
+        match_small_block(blockIdx.x - g_stats, oc_char, oc_idx, uo, n_units, pair_row, ls,
+                          slow_units);
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_chunk_scans(
     UnitOut *__restrict__ uo, uint32_t n_units, ChunkStat *__restrict__ ck,
     int32_t *__restrict__ gmin, int32_t *__restrict__ gmin1, int32_t *__restrict__ gmin2,
     DepthAgg *__restrict__ gagg, int32_t *__restrict__ gd0, const uint32_t *__restrict__ big_units,
-    const LaunchState *__restrict__ ls, uint32_t g_big) {
+    LaunchState *__restrict__ ls, uint32_t g_big, uint32_t *__restrict__ slow_units) {
     if (blockIdx.x < g_big)
-        chunk_scan_blocks(blockIdx.x, g_big, uo, ck, gmin, gmin1, gmin2, gagg, gd0, big_units, ls);
+        chunk_scan_blocks(blockIdx.x, g_big, uo, ck, gmin, gmin1, gmin2, gagg, gd0, big_units, ls,
+                          slow_units);
     else
-        chunk_scan_small_block(blockIdx.x - g_big, uo, n_units, ck, gmin, gmin1, gmin2, ls);
+        chunk_scan_small_block(blockIdx.x - g_big, uo, n_units, ck, gmin, gmin1, gmin2, ls,
+                               slow_units);
 }
 
-hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, uint32_t chunk_bound,
-                        uint32_t group_bound, int32_t *pair_row) {
-    (void)group_bound;
-    // The chunk count is only known on the device: grid-stride over a grid
-    // that fills the chip (256 CUs) without launching one block per bound.
-    const uint32_t g = chunk_bound < 2048u ? (chunk_bound ? chunk_bound : 1u) : 2048u;
-    const uint32_t gc = chunk_bound < 8192u ? (chunk_bound ? chunk_bound : 1u) : 8192u;
+namespace {
+// The chunk count is only known on the device: grid-stride over a grid
+// that fills the chip (256 CUs) without launching one block per bound.
+uint32_t chunk_grid(uint32_t chunk_bound, uint32_t cap) {
+    return chunk_bound < cap ? (chunk_bound ? chunk_bound : 1u) : cap;
+}
+}  // namespace
+
+hipError_t launch_match_front(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
+                              uint32_t chunk_bound, int32_t *pair_row) {
+    const uint32_t g = chunk_grid(chunk_bound, 2048u);
     const uint32_t gb = n_units < 256u ? n_units : 256u;
     hipLaunchKernelGGL(k_match_front, dim3(g + (n_units + 3) / 4), dim3(kMatchThreads), 0, s,
-                       b.oc_char, b.oc_idx, b.uout, n_units, b.chunk_unit, b.ck, pair_row, b.ls, g);
+                       b.oc_char, b.oc_idx, b.uout, n_units, b.chunk_unit, b.ck, pair_row, b.ls, g,
+                       b.slow_units);
     hipLaunchKernelGGL(k_chunk_scans, dim3(gb + (n_units + kScanThreads / 64 - 1) / (kScanThreads / 64)),
                        dim3(kScanThreads), 0, s, b.uout, n_units, b.ck, b.gmin, b.gmin1, b.gmin2,
-                       b.gagg, b.gd0, b.big_units, b.ls, gb);
+                       b.gagg, b.gd0, b.big_units, b.ls, gb, b.slow_units);
+    return hipGetLastError();
+}
+
+hipError_t launch_match_local(hipStream_t s, const LaunchBufs &b, uint32_t chunk_bound,
+                              int32_t *pair_row) {
+    const uint32_t g = chunk_grid(chunk_bound, 2048u);
+    const uint32_t gc = chunk_grid(chunk_bound, 8192u);
     // uo twice: the kernel's fields as read-only (scalar loads) and the
     // error flag it sets; it writes no field that it reads.
     hipLaunchKernelGGL(k_match_local, dim3(g), dim3(kMatchThreads), 0, s, b.oc_char, b.oc_idx,
@@ -811,6 +840,13 @@ hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, ui
                        b.chunk_unit, b.ck, b.gmin, b.gmin1, b.gmin2, b.lows, b.highs, pair_row,
                        b.ls);
     return hipGetLastError();
+}
+
+hipError_t launch_match(hipStream_t s, const LaunchBufs &b, uint32_t n_units, uint32_t chunk_bound,
+                        uint32_t group_bound, int32_t *pair_row) {
+    (void)group_bound;
+    const hipError_t e = launch_match_front(s, b, n_units, chunk_bound, pair_row);
+    return e != hipSuccess ? e : launch_match_local(s, b, chunk_bound, pair_row);
 }
 
 }  // namespace cjx

@@ -37,7 +37,7 @@ constexpr uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
 struct NaLayout {
     uint64_t units, tile_unit, uout, ls, tile_st, oc_char, oc_idx, keys, sorted, tmp, spos,
         block_delta, hist, ck, gmin, gmin1, gmin2, gagg, gd0, lows, highs, chunk_unit,
-        big_units, unit_part, end;
+        big_units, slow_units, unit_part, end;
 };
 
 constexpr NaLayout make_layout() {
@@ -71,6 +71,7 @@ constexpr NaLayout make_layout() {
     L.highs = take(kNaChunks * kMatchKeys * 4 + 16);
     L.chunk_unit = take(kNaChunks * 4 + 16);
     L.big_units = take(4 + 16);
+    L.slow_units = take(4 + 16);  // one unit
     L.unit_part = take(2 * 32);
     L.end = o;
     return L;
@@ -182,6 +183,7 @@ int cjx_parse_device_noalloc(const uint8_t *d_input, size_t n, unsigned flags,
     b.small_units = b.tile_unit + ntiles;
     b.chunk_unit = reinterpret_cast<uint32_t *>(base + kNa.chunk_unit);
     b.big_units = reinterpret_cast<uint32_t *>(base + kNa.big_units);
+    b.slow_units = reinterpret_cast<uint32_t *>(base + kNa.slow_units);
     b.unit_part = reinterpret_cast<uint32_t *>(base + kNa.unit_part);
     b.uout = reinterpret_cast<UnitOut *>(base + kNa.uout);
     b.ls = reinterpret_cast<LaunchState *>(base + kNa.ls);

@@ -448,8 +448,10 @@ hipError_t wait_event(Slot &s) {
 // CU of the device: no CU is left out, and the runtime gives such a stream a
 // hardware queue of its own, so the slot never waits behind another stream's
 // copies on a shared queue.  Where that cannot be had, the slot keeps the
-// context's ordinary stream.
+// context's ordinary stream.  A slot keeps all its work on that one stream
+// (no second stream beside it), so it holds exactly one queue.
 int own_queue_stream(int device, cjx_context *ctx) {
+    ctx->fork_ok = false;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess || prop.multiProcessorCount <= 0)
         return CJX_OK;
