@@ -218,11 +218,21 @@ int upload_units(cjx_context *ctx, const std::vector<UnitDesc> &units, uint32_t 
     uint8_t *h = static_cast<uint8_t *>(ctx->up[slot].p);
     std::memcpy(h, units.data(), nb);
     uint32_t *tu = reinterpret_cast<uint32_t *>(h + nb);
-    uint32_t *su = tu + ntiles;  // then the small units, in unit order
+    // then the small units: those of at most one tokenizer round first (the
+    // one-round kernel takes them), each part in unit order
+    uint32_t *su = tu + ntiles;
+    uint32_t n_tiny = 0;
     for (size_t u = 0; u < units.size(); u++) {
         for (uint32_t t = 0; t < units[u].ntiles; t++) tu[units[u].tile0 + t] = (uint32_t)u;
-        if (units[u].ntiles == 0) *su++ = (uint32_t)u;
+        if (units[u].ntiles == 0 && units[u].n <= (uint64_t)kRoundBytes) {
+            *su++ = (uint32_t)u;
+            n_tiny++;
+        }
     }
+    for (size_t u = 0; u < units.size(); u++)
+        if (units[u].ntiles == 0 && units[u].n > (uint64_t)kRoundBytes) *su++ = (uint32_t)u;
+    (void)n_small;
+    ctx->n_tiny = n_tiny;
     if (sb) std::memcpy(h + nb + tb, unit_src, sb);
     if (hipMemcpyAsync(ctx->units.p, h, nb + tb + sb, hipMemcpyHostToDevice, ctx->stream) !=
         hipSuccess)
@@ -362,12 +372,12 @@ int enqueue_parse(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, unsign
         return CJX_ERR_DEVICE;
     if (fork && two) {
         if (launch_tokenize(ctx->aux, b, d_in, 0, sh.n_small, jsonl, !strict, d_struct,
-                            ctx->tok_shape) != hipSuccess ||
+                            ctx->tok_shape, ctx->n_tiny) != hipSuccess ||
             launch_tokenize(s, b, d_in, sh.ntiles, 0, jsonl, !strict, d_struct,
                             ctx->tok_shape) != hipSuccess)
             return CJX_ERR_DEVICE;
     } else if (launch_tokenize(s, b, d_in, sh.ntiles, sh.n_small, jsonl, !strict, d_struct,
-                               ctx->tok_shape) != hipSuccess) {
+                               ctx->tok_shape, ctx->n_tiny) != hipSuccess) {
         return CJX_ERR_DEVICE;
     }
     if (fork && !join_aux(ctx, 0)) return CJX_ERR_DEVICE;
@@ -497,6 +507,7 @@ int enqueue_maybe_graph(cjx_context *ctx, const uint8_t *d_in, const Shape &sh, 
     key.one_unit_dev = ctx->one_unit_dev;
     if (ctx->one_unit_dev) key.one_unit = ctx->one_unit;
     key.tok_shape = ctx->tok_shape;
+    key.n_tiny = ctx->n_tiny;
     if (!ctx->graph || std::memcmp(&key, &ctx->graph_key, sizeof(key)) != 0) {
         if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
         ctx->graph = nullptr;
@@ -1397,6 +1408,7 @@ int cjx_parse_device(cjx_context *ctx, const uint8_t *d_input, size_t n, unsigne
     ctx->unit_src = nullptr;
     ctx->one_unit = units[0];
     ctx->one_unit_dev = true;
+    ctx->n_tiny = (n_small && n <= (uint64_t)kRoundBytes) ? 1u : 0u;
     ctx->n_units = 1;
     (void)hipEventRecord(ctx->ev0, ctx->stream);
     rc = enqueue_maybe_graph(ctx, d_input, sh, flags & ~CJX_FLAG_JSONL,

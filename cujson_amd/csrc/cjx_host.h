@@ -78,6 +78,7 @@ struct cjx_context {
     uint64_t timing_count = 0;
     bool timing = false;
     cjx::TokShape tok_shape;  // tokenizer launch shape (cjx_context_set_tokenizer_shape)
+    uint32_t n_tiny = 0;      // small units of at most one round, first in the small list
     DevBuf in, structural, pair;
     DevBuf span;  // host units copied as one span of their source, before placement
     DevBuf units, uout, ls, tile_st, layout, chunk_unit, big_units, slow_units, unit_part,
@@ -151,6 +152,7 @@ struct cjx_context {
         bool one_unit_dev;
         cjx::UnitDesc one_unit;
         cjx::TokShape tok_shape;
+        uint32_t n_tiny;
     } graph_key{};
 };
 

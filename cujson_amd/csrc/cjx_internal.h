@@ -181,10 +181,12 @@ inline bool tok_shape_valid(uint32_t tile_waves, uint32_t small_waves) {
     return t && sm;
 }
 // Units of at most kWaveBytes bytes are "small": no tiles, one wave each
-// (listed in small_units); the others are cut into kTileBytes tiles.
+// (listed in small_units, the first n_tiny of them of at most kRoundBytes
+// bytes); the others are cut into kTileBytes tiles.
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
                            uint32_t ntiles, uint32_t n_small, bool jsonl, bool weak_utf8,
-                           uint32_t *out_struct, TokShape shape = TokShape{});
+                           uint32_t *out_struct, TokShape shape = TokShape{},
+                           uint32_t n_tiny = 0);
 // `count` more big-tile workgroups (tickets continue from the launches before
 // on the same stream), no small units: the tokenizer of a launch split into
 // parts that each wait for their input bytes.

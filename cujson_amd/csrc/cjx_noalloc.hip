@@ -212,7 +212,9 @@ int cjx_parse_device_noalloc(const uint8_t *d_input, size_t n, unsigned flags,
     if (hipGetLastError() != hipSuccess) return CJX_ERR_DEVICE;
     if (strict && launch_utf8_strict(s, b, d_input, ntiles, n_small) != hipSuccess)
         return CJX_ERR_DEVICE;
-    if (launch_tokenize(s, b, d_input, ntiles, n_small, false, !strict, d_st) != hipSuccess ||
+    const uint32_t n_tiny = (n_small && n <= (uint64_t)kRoundBytes) ? 1u : 0u;
+    if (launch_tokenize(s, b, d_input, ntiles, n_small, false, !strict, d_st, TokShape{},
+                        n_tiny) != hipSuccess ||
         launch_unit_bases(s, b, 1, false) != hipSuccess ||
         launch_match(s, b, 1, chunks, groups, d_pair) != hipSuccess ||
         launch_phase3(s, b, 0, phase3_blocks(rows), pair_blocks(rows), u16, d_pair) != hipSuccess)

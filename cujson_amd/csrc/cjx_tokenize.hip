@@ -440,22 +440,23 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
 // one wave per unit, which starts in state (0, 0) and owns the unit's whole
 // output region, so there is no look-back and no workgroup barrier.
 // A tile is WAVES waves of ROUNDS rounds (WAVES x ROUNDS x 2 KiB = 64 KiB);
-// a small-unit workgroup is WAVES waves of kTokRounds rounds (one wave covers
-// the largest small unit).  WPE is the register budget in waves per SIMD:
+// a small-unit workgroup is WAVES waves of SROUNDS rounds, one unit per wave:
+// kTokRounds rounds cover the largest small unit, one round the units of at
+// most 2 KiB (fewer registers, so more of those waves are resident).  WPE is the register budget in waves per SIMD:
 // 4 x 8 at 4 waves per SIMD uses 103 VGPRs without spills; with the interior
 // and edge rounds as separate instantiations, the 5-wave budget (96 VGPRs)
 // spills and measured 4 % slower.
-template <bool JSONL, bool SMALL, int WAVES, int WPE>
+template <bool JSONL, bool SMALL, int WAVES, int WPE, int SROUNDS = kTokRounds>
 __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 * WAVES) void k_tokenize(
     const uint8_t *__restrict__ in, const UnitDesc *__restrict__ units,
     const uint32_t *__restrict__ tile_unit, const uint32_t *__restrict__ small_units,
     uint32_t n_small, unsigned long long *__restrict__ tile_st, LaunchState *__restrict__ ls,
     UnitOut *__restrict__ uo, uint32_t *__restrict__ out_struct, uint8_t *__restrict__ oc_char,
     uint32_t *__restrict__ oc_idx, uint8_t *__restrict__ row_nl, uint32_t weak_utf8) {
-    constexpr int ROUNDS = SMALL ? kTokRounds : kTileBytes / (WAVES * kRoundBytes);
+    constexpr int ROUNDS = SMALL ? SROUNDS : kTileBytes / (WAVES * kRoundBytes);
     constexpr uint64_t WAVE_BYTES = (uint64_t)ROUNDS * kRoundBytes;
     static_assert(SMALL || WAVES * ROUNDS * kRoundBytes == kTileBytes, "a tile is 64 KiB");
-    static_assert(!SMALL || WAVE_BYTES == kWaveBytes, "one wave covers a small unit");
+    static_assert(!SMALL || WAVE_BYTES <= kWaveBytes, "one wave covers a small unit");
     __shared__ uint32_t s_tile, s_unit;
     __shared__ uint32_t s_map[WAVES];
     __shared__ uint32_t s_cnt0[WAVES], s_cnt_all[WAVES];
@@ -736,12 +737,12 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
 
 namespace {
 
-template <bool JSONL, bool SMALL, int WAVES, int WPE>
-void launch_variant(hipStream_t s, uint32_t grid, const LaunchBufs &b, const uint8_t *d_in,
-                    uint32_t n_small, uint32_t *out_struct, uint32_t weak) {
-    hipLaunchKernelGGL((k_tokenize<JSONL, SMALL, WAVES, WPE>), dim3(grid), dim3(64 * WAVES), 0, s,
-                       d_in, b.units, b.tile_unit, b.small_units, n_small, b.tile_st, b.ls, b.uout,
-                       out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
+template <bool JSONL, bool SMALL, int WAVES, int WPE, int SROUNDS = kTokRounds>
+void launch_variant(hipStream_t s, uint32_t grid, const LaunchBufs &b, const uint32_t *small_units,
+                    const uint8_t *d_in, uint32_t n_small, uint32_t *out_struct, uint32_t weak) {
+    hipLaunchKernelGGL((k_tokenize<JSONL, SMALL, WAVES, WPE, SROUNDS>), dim3(grid),
+                       dim3(64 * WAVES), 0, s, d_in, b.units, b.tile_unit, small_units, n_small,
+                       b.tile_st, b.ls, b.uout, out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
 }
 
 // Register budgets: 4 waves per SIMD for 4, 8 and 16 waves of 16, 8 and
@@ -751,38 +752,67 @@ template <bool JSONL>
 void launch_tiles(hipStream_t s, uint32_t grid, uint32_t tile_waves, const LaunchBufs &b,
                   const uint8_t *d_in, uint32_t *out_struct, uint32_t weak) {
     switch (tile_waves) {
-    case 2: launch_variant<JSONL, false, 2, 3>(s, grid, b, d_in, 0u, out_struct, weak); break;
-    case 8: launch_variant<JSONL, false, 8, 4>(s, grid, b, d_in, 0u, out_struct, weak); break;
-    case 16: launch_variant<JSONL, false, 16, 4>(s, grid, b, d_in, 0u, out_struct, weak); break;
-    default: launch_variant<JSONL, false, 4, 4>(s, grid, b, d_in, 0u, out_struct, weak); break;
+    case 2: launch_variant<JSONL, false, 2, 3>(s, grid, b, nullptr, d_in, 0u, out_struct, weak); break;
+    case 8: launch_variant<JSONL, false, 8, 4>(s, grid, b, nullptr, d_in, 0u, out_struct, weak); break;
+    case 16: launch_variant<JSONL, false, 16, 4>(s, grid, b, nullptr, d_in, 0u, out_struct, weak); break;
+    default: launch_variant<JSONL, false, 4, 4>(s, grid, b, nullptr, d_in, 0u, out_struct, weak); break;
     }
 }
 
-template <bool JSONL>
+template <bool JSONL, int SROUNDS>
 void launch_small(hipStream_t s, uint32_t n_small, uint32_t small_waves, const LaunchBufs &b,
-                  const uint8_t *d_in, uint32_t *out_struct, uint32_t weak) {
+                  const uint32_t *list, const uint8_t *d_in, uint32_t *out_struct, uint32_t weak) {
+    // one round needs 47 VGPRs: a budget of 8 waves per SIMD
+    constexpr int WPE = SROUNDS == 1 ? 8 : 4;
     const uint32_t sw = small_waves;
     const uint32_t grid = (n_small + sw - 1) / sw;
     switch (sw) {
-    case 1: launch_variant<JSONL, true, 1, 4>(s, grid, b, d_in, n_small, out_struct, weak); break;
-    case 2: launch_variant<JSONL, true, 2, 4>(s, grid, b, d_in, n_small, out_struct, weak); break;
-    case 8: launch_variant<JSONL, true, 8, 4>(s, grid, b, d_in, n_small, out_struct, weak); break;
-    default: launch_variant<JSONL, true, 4, 4>(s, grid, b, d_in, n_small, out_struct, weak); break;
+    case 1:
+        launch_variant<JSONL, true, 1, WPE, SROUNDS>(s, grid, b, list, d_in, n_small, out_struct,
+                                                weak);
+        break;
+    case 2:
+        launch_variant<JSONL, true, 2, WPE, SROUNDS>(s, grid, b, list, d_in, n_small, out_struct,
+                                                weak);
+        break;
+    case 8:
+        launch_variant<JSONL, true, 8, WPE, SROUNDS>(s, grid, b, list, d_in, n_small, out_struct,
+                                                weak);
+        break;
+    default:
+        launch_variant<JSONL, true, 4, WPE, SROUNDS>(s, grid, b, list, d_in, n_small, out_struct,
+                                                weak);
+        break;
     }
+}
+
+// The small units: the first n_tiny of the list (units of at most one round)
+// with the one-round kernel, the rest with the full small-unit kernel.
+template <bool JSONL>
+void launch_smalls(hipStream_t s, uint32_t n_small, uint32_t n_tiny, uint32_t small_waves,
+                   const LaunchBufs &b, const uint8_t *d_in, uint32_t *out_struct, uint32_t weak) {
+    if (n_tiny > n_small) n_tiny = n_small;
+    if (n_tiny)
+        launch_small<JSONL, 1>(s, n_tiny, small_waves, b, b.small_units, d_in, out_struct, weak);
+    if (n_small > n_tiny)
+        launch_small<JSONL, kTokRounds>(s, n_small - n_tiny, small_waves, b, b.small_units + n_tiny,
+                                        d_in, out_struct, weak);
 }
 
 }  // namespace
 
 hipError_t launch_tokenize(hipStream_t s, const LaunchBufs &b, const uint8_t *d_in,
                            uint32_t ntiles, uint32_t n_small, bool jsonl, bool weak_utf8,
-                           uint32_t *out_struct, TokShape shape) {
+                           uint32_t *out_struct, TokShape shape, uint32_t n_tiny) {
     const uint32_t weak = weak_utf8 ? 1u : 0u;
     if (jsonl) {
         if (ntiles) launch_tiles<true>(s, ntiles, shape.tile_waves, b, d_in, out_struct, weak);
-        if (n_small) launch_small<true>(s, n_small, shape.small_waves, b, d_in, out_struct, weak);
+        if (n_small)
+            launch_smalls<true>(s, n_small, n_tiny, shape.small_waves, b, d_in, out_struct, weak);
     } else {
         if (ntiles) launch_tiles<false>(s, ntiles, shape.tile_waves, b, d_in, out_struct, weak);
-        if (n_small) launch_small<false>(s, n_small, shape.small_waves, b, d_in, out_struct, weak);
+        if (n_small)
+            launch_smalls<false>(s, n_small, n_tiny, shape.small_waves, b, d_in, out_struct, weak);
     }
     return hipGetLastError();
 }

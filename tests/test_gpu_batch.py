@@ -77,6 +77,28 @@ def test_batch_of_tiny_and_mixed_documents(ctx):
         check_batch(ctx, data, np.array(offs), sizes, flags, f"tiny flags={flags}")
 
 
+def test_batch_round_and_wave_edge_documents(ctx):
+    """Documents at the one-round (2 KiB) and one-wave (16 KiB) limits of
+    the small-unit kernels and just past them (a tile), strings and
+    backslash runs across those edges, in one batch, every flag set."""
+    base = bytes(gen.config1()[:40000])
+    docs = []
+    for n in (1, 2, 31, 32, 33, 2046, 2047, 2048, 2049, 2050, 4096, 16383, 16384, 16385, 20000):
+        docs.append(base[:n])
+        for k in (1, 2, 3):
+            pre = b'["' + b"x" * max(0, n - 6 - k)
+            docs.append(pre + b"\\" * k + b'"]",1]')
+    offs, pos, parts = [], 0, []
+    for d in docs:
+        offs.append(pos)
+        parts.append(d)
+        pos += len(d)
+    data = np.frombuffer(b"".join(parts), np.uint8)
+    sizes = np.array([len(d) for d in docs])
+    for flags in (0, oracle.FLAG_DEPTH_U16, oracle.FLAG_UTF8_STRICT):
+        check_batch(ctx, data, np.array(offs), sizes, flags, f"edges flags={flags}")
+
+
 def test_batch_rejects_empty_document(ctx):
     data = np.frombuffer(b"[1][2]", np.uint8)
     r = cj.parse_batch(data, [0, 3, 3], [3, 0, 3], 0, ctx)
