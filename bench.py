@@ -201,6 +201,73 @@ def many_docs_bench(cj, gen, n_docs: int, reps: int, plain_compare: bool = True)
     }
 
 
+def many_docs_large_bench(cj, gen, n_docs: int, copies: int, launches: int, reps: int = 3):
+    """A multi-GB document batch: the n_docs-document batch laid out `copies`
+    times over in one device buffer (the same documents repeated; below 2^31
+    bytes, the launch limit), parsed by `launches` batch launches back to
+    back per step, so launches x copies x n_docs documents per step.
+    Device time per step: the launches' sequences summed (context timing)."""
+    from cujson_amd import autotune
+    from cujson_amd.hip import DeviceBuffer
+    data1, offs1, sizes1, _ = gen.many_docs(n_docs=n_docs)
+    stride = -(-(int(offs1[-1]) + int(sizes1[-1])) // 16) * 16
+    data = np.concatenate([np.tile(data1[:stride], copies), np.zeros(64, np.uint8)])
+    offs = np.concatenate([offs1 + k * stride for k in range(copies)])
+    sizes = np.tile(sizes1, copies)
+    del data1
+    per_launch = int(sizes.sum())
+    assert per_launch < (1 << 31)
+    ctx = cj.Context(0)
+    d_in = DeviceBuffer(data.size)
+    d_in.upload(data)
+    del data
+    cap = per_launch + 64
+    d_st, d_pr = DeviceBuffer(cap * 4), DeviceBuffer(cap * 4)
+
+    def launch(flags):
+        rc = ctx.parse_batch_device(d_in.ptr, offs, sizes, d_st.ptr, d_pr.ptr, cap, flags)
+        if rc:
+            raise RuntimeError(f"parse_batch_device: {cj.strerror(rc)}")
+
+    launch(cj.CJX_FLAG_GRAPH)
+    rc, docs, N, M, over = ctx.batch_wait()
+    if rc or over:
+        raise RuntimeError(f"large batch failed rc={rc} overflow={over}")
+    shape, shape_table = autotune.tune_tokenizer_shape(
+        ctx, lambda: launch(cj.CJX_FLAG_GRAPH), signature=f"batch:{len(sizes)}:{per_launch}",
+        tiles=True, small=True, reps=3, use_cache=False)
+    res = {}
+    for name, flags in (("graph", cj.CJX_FLAG_GRAPH), ("launched", 0)):
+        launch(flags)
+        ctx.batch_wait()
+        ctx.set_timing(True)
+        for _ in range(reps * launches):
+            launch(flags)
+        t = np.array([x[2] for x in ctx.timings(reps * launches)]).reshape(reps, launches)
+        ctx.set_timing(False)
+        ctx.batch_wait()
+        res[name] = float(np.median(t.sum(axis=1)))
+    ctx.close()
+    form = "graph" if res["graph"] <= res["launched"] else "launched"
+    ms = res[form]
+    n_total = launches * len(sizes)
+    b_total = launches * per_launch
+    return {
+        "workload": f"{n_total} standard-JSON documents, {b_total / 1e9:.2f} GB per step: "
+                    f"{launches} batch launches of the {n_docs}-document batch (Pareto(1.1) "
+                    f"sizes from 200 B clipped to 16 MB, 5% nested 65-512 deep, 10% with one "
+                    f"fault) laid out {copies} times over ({per_launch / 1e9:.2f} GB, below "
+                    f"2^31 bytes); the same documents repeated; device-resident",
+        "docs": n_total, "bytes": b_total, "structurals": int(N) * launches,
+        "tokenizer_shape": {"tile_waves": shape[0], "small_waves": shape[1],
+                            "autotune_ms": shape_table},
+        "batched_ms": round(ms, 3), "form": form,
+        "graph_ms": round(res["graph"], 3), "launched_ms": round(res["launched"], 3),
+        "batched_gbs": round(b_total / (ms * 1e-3) / 1e9, 2),
+        "docs_per_s": round(n_total / (ms * 1e-3), 0),
+    }
+
+
 def lines_bench(cj, gen, n_records: int, reps: int, plain_compare: bool = True):
     """8M JSON Lines records (log-normal sizes, median 400 B, sigma 1.2,
     clipped to 64 KiB, depth 1-6), device-resident, in launches of at most
@@ -624,6 +691,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 50,000-document batched-vs-plain measurement")
     ap.add_argument("--docs", type=int, default=50_000)
+    ap.add_argument("--docs-copies", type=int, default=20,
+                    help="copies of the document batch per launch in the multi-GB batch leg "
+                         "(two launches; 0: skip)")
     ap.add_argument("--no-lines", action="store_true",
                     help="skip the JSON Lines batched-vs-plain measurement")
     ap.add_argument("--records", type=int, default=8_000_000)
@@ -832,6 +902,10 @@ def main():
         log(f"batched vs plain on {args.docs} documents")
         out["batched"] = many_docs_bench(cj, gen, args.docs, reps=5,
                                          plain_compare=not args.no_plain_compare)
+        if args.docs_copies > 1:
+            log(f"batched: 2 launches of {args.docs} documents x {args.docs_copies}")
+            out["batched_large"] = many_docs_large_bench(cj, gen, args.docs, args.docs_copies,
+                                                         launches=2)
     if not args.no_lines:
         log(f"JSON Lines batched vs plain on {args.records} records")
         out["lines"] = lines_bench(cj, gen, args.records, reps=3,
