@@ -89,38 +89,59 @@ __device__ __forceinline__ DepthAgg wave_scan_agg(DepthAgg a) {
 
 }  // namespace
 
-// K1: depth walk of every chunk.
+// Lane t's 16 brackets [16t, 16t + 16) of a chunk whose brackets start at
+// oc_char[first] (cnt of them): one 16-byte load where the chunk's bytes are
+// 16-byte aligned and the lane's 16 are all inside it, byte loads otherwise
+// (zero past the end).
+__device__ __forceinline__ uint4 chunk_lane_bytes(const uint8_t *__restrict__ oc_char,
+                                                  uint32_t first, uint32_t cnt, uint32_t t) {
+    const uint32_t e0 = 16u * t;
+    if ((first & 15u) == 0 && e0 + 16u <= cnt)
+        return *reinterpret_cast<const uint4 *>(oc_char + first + e0);
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (uint32_t k = 0; k < 16; k++)
+        if (e0 + k < cnt) w[k >> 2] |= (uint32_t)oc_char[first + e0 + k] << (8 * (k & 3));
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// K1: depth walk of every chunk.  Thread t walks brackets [16t, 16t + 16)
+// of a chunk straight from registers; the next chunk's bytes are loaded
+// before this one's walk is reduced.
 __device__ __forceinline__ void chunk_stats_blocks(uint32_t bid, uint32_t nblk,
                                                    const uint8_t *__restrict__ oc_char,
-                                                               const UnitOut *__restrict__ uo,
-                                                               const uint32_t *__restrict__ chunk_unit,
-                                                               ChunkStat *__restrict__ ck,
-                                                               const LaunchState *__restrict__ ls) {
-    constexpr int kPer = kMatchChunk / kMatchThreads;
+                                                   const UnitOut *__restrict__ uo,
+                                                   const uint32_t *__restrict__ chunk_unit,
+                                                   ChunkStat *__restrict__ ck,
+                                                   const LaunchState *__restrict__ ls) {
+    static_assert(kMatchChunk == 16 * kMatchThreads, "16 brackets per thread");
     __shared__ DepthAgg part[kMatchThreads / 64];
-    __shared__ __align__(16) uint8_t s_ch[kMatchChunk];
     if (ls->overflow) return;
     const uint32_t n_chunks = ls->n_chunks;
-    for (uint32_t c = bid; c < n_chunks; c += nblk) {
+    const uint32_t t = threadIdx.x;
+    auto chunk_range = [&](uint32_t c, uint32_t &first, uint32_t &cnt) {
         const uint32_t u = chunk_unit[c];
-        const uint32_t first = uo[u].m_base + (c - uo[u].chunk0) * kMatchChunk;
-        const uint32_t cnt =
-            uo[u].m_end > first ? min((uint32_t)kMatchChunk, uo[u].m_end - first) : 0u;
-        uint8_t vc[kPer];
-#pragma unroll
-        for (int k = 0; k < kPer; k++) {
-            const uint32_t e = threadIdx.x + k * kMatchThreads;
-            vc[k] = e < cnt ? oc_char[first + e] : 0;
+        first = uo[u].m_base + (c - uo[u].chunk0) * kMatchChunk;
+        cnt = uo[u].m_end > first ? min((uint32_t)kMatchChunk, uo[u].m_end - first) : 0u;
+    };
+    uint32_t c = bid;
+    if (c >= n_chunks) return;
+    uint32_t first, cnt;
+    chunk_range(c, first, cnt);
+    uint4 cur = chunk_lane_bytes(oc_char, first, cnt, t);
+    for (;;) {
+        const uint32_t cn = c + nblk;
+        uint32_t first_n = 0, cnt_n = 0;
+        uint4 nxt = make_uint4(0u, 0u, 0u, 0u);
+        if (cn < n_chunks) {
+            chunk_range(cn, first_n, cnt_n);
+            nxt = chunk_lane_bytes(oc_char, first_n, cnt_n, t);
         }
-#pragma unroll
-        for (int k = 0; k < kPer; k++) s_ch[threadIdx.x + k * kMatchThreads] = vc[k];
-        __syncthreads();
         DepthAgg a{0, 0, 0};
-        const uint32_t e0 = threadIdx.x * kPer;
-        const uint4 cw = *reinterpret_cast<const uint4 *>(&s_ch[e0]);
-        const uint32_t cv[4] = {cw.x, cw.y, cw.z, cw.w};
+        const uint32_t cv[4] = {cur.x, cur.y, cur.z, cur.w};
+        const uint32_t e0 = 16u * t;
 #pragma unroll
-        for (int k = 0; k < kPer; k++) {
+        for (uint32_t k = 0; k < 16; k++) {
             if (e0 + k < cnt) {
                 a.d += delta_of((uint8_t)(cv[k >> 2] >> (8 * (k & 3))));
                 a.mn = min(a.mn, a.d);
@@ -128,14 +149,18 @@ __device__ __forceinline__ void chunk_stats_blocks(uint32_t bid, uint32_t nblk,
             }
         }
         a = wave_reduce_agg(a);
-        if (lane_id() == 0) part[threadIdx.x >> 6] = a;
+        if (lane_id() == 0) part[t >> 6] = a;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            DepthAgg t = part[0];
-            for (int k = 1; k < kMatchThreads / 64; k++) t = agg_compose(t, part[k]);
-            ck[c] = ChunkStat{t.d, t.mn, t.mx, 0};
+        if (t == 0) {
+            DepthAgg r = part[0];
+            for (int k = 1; k < kMatchThreads / 64; k++) r = agg_compose(r, part[k]);
+            ck[c] = ChunkStat{r.d, r.mn, r.mx, 0};
         }
         __syncthreads();
+        if (cn >= n_chunks) break;
+        c = cn;
+        cur = nxt;
+        cnt = cnt_n;
     }
 }
 
@@ -144,6 +169,7 @@ __device__ __forceinline__ void chunk_stats_blocks(uint32_t bid, uint32_t nblk,
 // groups giving their start depths and minima, the 4096-chunk minima and
 // whether the unit is well nested, (c) each chunk's start depth and minimum.
 constexpr int kScanThreads = 1024;
+constexpr int kScanBatch = 2;  // 64-chunk groups per wave step of the scan passes
 __device__ __forceinline__ void chunk_scan_blocks(
     uint32_t bid, uint32_t nblk, UnitOut *__restrict__ uo, ChunkStat *__restrict__ ck, int32_t *__restrict__ gmin,
     int32_t *__restrict__ gmin1, int32_t *__restrict__ gmin2, DepthAgg *__restrict__ gagg,
@@ -158,18 +184,24 @@ __device__ __forceinline__ void chunk_scan_blocks(
     const uint32_t c0 = uo[u].chunk0, nch = uo[u].nchunks;
     const uint32_t g0 = uo[u].grp0, h0 = uo[u].sgrp0;
     const uint32_t G = (nch + 63) / 64, H = (nch + 4095) / 4096;
-    // (a) group walks, two groups per wave step.
-    for (uint32_t j = w; j < G; j += 2 * kW) {
-        const uint32_t j2 = j + kW;
-        const uint32_t x1 = 64 * j + lane, x2 = 64 * j2 + lane;
-        ChunkStat s1{0, 0, 0, 0}, s2{0, 0, 0, 0};
-        if (x1 < nch) s1 = ck[c0 + x1];
-        if (j2 < G && x2 < nch) s2 = ck[c0 + x2];
-        const DepthAgg r1 = wave_reduce_agg(DepthAgg{s1.d, s1.mn, s1.mx});
-        const DepthAgg r2 = wave_reduce_agg(DepthAgg{s2.d, s2.mn, s2.mx});
-        if (lane == 0) {
-            gagg[g0 + j] = r1;
-            if (j2 < G) gagg[g0 + j2] = r2;
+    // (a) group walks: kScanBatch groups per wave step, their loads issued
+    // together.
+    for (uint32_t j0 = w; j0 < G; j0 += kScanBatch * kW) {
+        DepthAgg v[kScanBatch];
+#pragma unroll
+        for (int i = 0; i < kScanBatch; i++) {
+            const uint32_t j = j0 + i * kW, x = 64 * j + lane;
+            v[i] = DepthAgg{0, 0, 0};
+            if (j < G && x < nch) {
+                const ChunkStat st = ck[c0 + x];
+                v[i] = DepthAgg{st.d, st.mn, st.mx};
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kScanBatch; i++) {
+            const uint32_t j = j0 + i * kW;
+            const DepthAgg r = wave_reduce_agg(v[i]);
+            if (lane == 0 && j < G) gagg[g0 + j] = r;
         }
     }
     __syncthreads();
@@ -219,29 +251,31 @@ __device__ __forceinline__ void chunk_scan_blocks(
         for (int off = 32; off >= 1; off >>= 1) m = min(m, __shfl_xor(m, off, 64));
         if (lane == 0) gmin2[h0 + h] = m;
     }
-    // (c) chunk start depths and minima, two groups per wave step.
-    for (uint32_t j = w; j < G; j += 2 * kW) {
-        const uint32_t j2 = j + kW;
-        const uint32_t x1 = 64 * j + lane, x2 = 64 * j2 + lane;
-        ChunkStat s1{0, 0, 0, 0}, s2{0, 0, 0, 0};
-        int32_t b1 = 0, b2 = 0;
-        b1 = gd0[g0 + j];
-        if (x1 < nch) s1 = ck[c0 + x1];
-        if (j2 < G) {
-            b2 = gd0[g0 + j2];
-            if (x2 < nch) s2 = ck[c0 + x2];
+    // (c) chunk start depths and minima, kScanBatch groups per wave step.
+    for (uint32_t j0 = w; j0 < G; j0 += kScanBatch * kW) {
+        int32_t bd[kScanBatch], sd[kScanBatch], smn[kScanBatch];
+#pragma unroll
+        for (int i = 0; i < kScanBatch; i++) {
+            const uint32_t j = j0 + i * kW, x = 64 * j + lane;
+            bd[i] = sd[i] = smn[i] = 0;
+            if (j < G) {
+                bd[i] = gd0[g0 + j];
+                if (x < nch) {
+                    const ChunkStat st = ck[c0 + x];
+                    sd[i] = st.d;
+                    smn[i] = st.mn;
+                }
+            }
         }
-        const int32_t i1 = (int32_t)wave_incl_scan((uint32_t)s1.d);
-        const int32_t i2 = (int32_t)wave_incl_scan((uint32_t)s2.d);
-        if (x1 < nch) {
-            const int32_t d0 = b1 + i1 - s1.d;
-            ck[c0 + x1].d0 = d0;
-            gmin[c0 + x1] = d0 + s1.mn;
-        }
-        if (j2 < G && x2 < nch) {
-            const int32_t d0 = b2 + i2 - s2.d;
-            ck[c0 + x2].d0 = d0;
-            gmin[c0 + x2] = d0 + s2.mn;
+#pragma unroll
+        for (int i = 0; i < kScanBatch; i++) {
+            const uint32_t j = j0 + i * kW, x = 64 * j + lane;
+            const int32_t in = (int32_t)wave_incl_scan((uint32_t)sd[i]);
+            if (j < G && x < nch) {
+                const int32_t d0 = bd[i] + in - sd[i];
+                ck[c0 + x].d0 = d0;
+                gmin[c0 + x] = d0 + smn[i];
+            }
         }
     }
     }
