@@ -711,6 +711,15 @@ def main():
     ap.add_argument("--serve-sustainable-rate", type=float, default=16000.0,
                     help="second serving run at a rate the host link carries (0: skip)")
     ap.add_argument("--serve-sustainable-seconds", type=float, default=20.0)
+    ap.add_argument("--no-strict", action="store_true",
+                    help="skip the strict UTF-8 timing (counter runs of the default mode alone)")
+    ap.add_argument("--tokenizer-shape", default="",
+                    help="fixed tokenizer shape TILExSMALL (e.g. 4x4) instead of autotuning it")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run only the document-batch legs and print their results "
+                         "(kernel profiles of the batch alone)")
+    ap.add_argument("--only-lines", action="store_true",
+                    help="run only the JSON Lines leg and print its result")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only copy this many bytes device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -728,6 +737,20 @@ def main():
     import cujson_amd as cj
     from cujson_amd import autotune, gen
     from cujson_amd.hip import DeviceBuffer, Event, synchronize
+
+    if args.only_lines:
+        print(json.dumps({"lines": lines_bench(cj, gen, args.records, reps=max(args.steps, 1),
+                                               plain_compare=not args.no_plain_compare)}),
+              flush=True)
+        return
+    if args.only_batched:
+        res = {"batched": many_docs_bench(cj, gen, args.docs, reps=max(args.steps, 1),
+                                          plain_compare=not args.no_plain_compare)}
+        if args.docs_copies > 1:
+            res["batched_large"] = many_docs_large_bench(cj, gen, args.docs, args.docs_copies,
+                                                         launches=2)
+        print(json.dumps(res), flush=True)
+        return
 
     t0 = time.time()
     doc = gen.config2_parallel(total_bytes=args.bytes)
@@ -754,8 +777,13 @@ def main():
     log(f"N={N} M={M} rho={N / n:.4f}")
     # tokenizer launch shape for this workload (untimed): every tile shape
     # timed on the parse itself, the fastest left set on the context
-    shape, shape_table = autotune.tune_tokenizer_shape(ctx, step, signature=f"plain:{n}",
-                                                       reps=5, use_cache=False)
+    if args.tokenizer_shape:
+        shape = tuple(int(x) for x in args.tokenizer_shape.split("x"))
+        ctx.set_tokenizer_shape(*shape)
+        shape_table = {}
+    else:
+        shape, shape_table = autotune.tune_tokenizer_shape(ctx, step, signature=f"plain:{n}",
+                                                           reps=5, use_cache=False)
     log(f"tokenizer shape {shape} from {shape_table}")
     for _ in range(args.warmup):
         step()
@@ -818,22 +846,24 @@ def main():
         rc = ctx.parse_device(d_in.ptr, n, d_st.ptr, d_pr.ptr, cap, cj.CJX_FLAG_UTF8_STRICT)
         if rc != 0:
             raise RuntimeError(f"parse_device (strict): {cj.strerror(rc)}")
-    strict_step()
-    rc, N4, M4, over = ctx.wait()
-    assert rc == 0 and N4 == N and M4 == M and not over
-    ctx.set_timing(True)
-    s_steps = max(3, min(args.steps, 10))
-    sevs = [Event() for _ in range(2)]
-    synchronize()
-    sevs[0].record(stream)
-    for _ in range(s_steps):
+    strict_ms = strict_front_ms = float("nan")
+    if not args.no_strict:
         strict_step()
-    sevs[1].record(stream)
-    synchronize()
-    s_tim = ctx.timings(s_steps)
-    ctx.set_timing(False)
-    strict_ms = sevs[0].elapsed_ms(sevs[1]) / s_steps
-    strict_front_ms = float(np.mean([t[0] for t in s_tim]))  # strict check + tokenizer
+        rc, N4, M4, over = ctx.wait()
+        assert rc == 0 and N4 == N and M4 == M and not over
+        ctx.set_timing(True)
+        s_steps = max(3, min(args.steps, 10))
+        sevs = [Event() for _ in range(2)]
+        synchronize()
+        sevs[0].record(stream)
+        for _ in range(s_steps):
+            strict_step()
+        sevs[1].record(stream)
+        synchronize()
+        s_tim = ctx.timings(s_steps)
+        ctx.set_timing(False)
+        strict_ms = sevs[0].elapsed_ms(sevs[1]) / s_steps
+        strict_front_ms = float(np.mean([t[0] for t in s_tim]))  # strict check + tokenizer
 
     tok_bytes = n + 4 * N + 5 * M          # k_tokenize: read input, write rows it owns
     path_bytes = n + 8 * (N + 2)            # the whole path's algorithmic bytes
@@ -884,8 +914,8 @@ def main():
             "wall_ms_per_step": round((wall1 - wall0) * 1e3 / args.steps, 4),
             "launched_ms_per_step": round(total_ms / args.steps, 4),
             "graph_ms_per_step": round(g_ms_per_step, 4),
-            "strict_utf8_ms_per_step": round(strict_ms, 4),
-            "strict_utf8_check_and_tokenize_ms": round(strict_front_ms, 4),
+            "strict_utf8_ms_per_step": _num(strict_ms, 4),
+            "strict_utf8_check_and_tokenize_ms": _num(strict_front_ms, 4),
             "strict_utf8_form": "CJX_FLAG_UTF8_STRICT (RFC 3629) instead of the reference's "
                                 "per-word rule, launched; the second figure is the strict "
                                 "check plus the tokenizer",
