@@ -112,38 +112,106 @@ __device__ __forceinline__ void load_lane(const uint8_t *base, uint64_t o, uint6
     }
 }
 
+// Bits 0..31 of the 64-bit value hi:lo shifted right by s: (hi << (32 - s)) |
+// (lo >> s), one v_alignbit.  With s = 32 - k it shifts hi left by k and
+// brings in the top k bits of lo (the bytes before this lane's 32).
+__host__ __device__ __forceinline__ uint32_t alignbit(uint32_t hi, uint32_t lo, uint32_t s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_alignbit(hi, lo, s);
+#else
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> s);
+#endif
+}
+
+// Character classes of a lane's 32 bytes from its eight bit planes (R[k] bit
+// i = bit k of byte i), as products of plane bits (each a few 3-input bit
+// operations for all 32 bytes at once):
+//   '"' 0x22 = 0010 0010   ':' 0x3A = 0011 1010   ',' 0x2C = 0010 1100
+//   '\\' 0x5C = 0101 1100  '\n' 0x0A = 0000 1010
+//   '[' 0x5B, ']' 0x5D, '{' 0x7B, '}' 0x7D = 01c1 1abd with abd = 011 or 101:
+//   byte bit 1 tells an opener, byte bit 5 a curly bracket.
+// op: the structural characters (':' ',' and brackets; '\n' too for JSON
+// Lines); b1 / b5: bit 1 / bit 5 of the bracket bytes;
+// for JSON Lines b5 also holds the '\n' bytes (never brackets).
+struct LaneClasses {
+    uint32_t bs, q, br, op, b1, b5;
+};
+
+template <bool JSONL>
+__host__ __device__ __forceinline__ LaneClasses classes_from_planes(const uint32_t (&R)[8]) {
+    const uint32_t lo010 = ~R[2] & R[1] & ~R[0];  // low bits 010: '"' ':' '\n'
+    const uint32_t lo100 = R[2] & ~R[1] & ~R[0];  // low bits 100: ',' '\\'
+    const uint32_t h001 = ~R[7] & ~R[6] & R[5];   // high bits 001: '"' ':' ','
+    const uint32_t h01x11 = ~R[7] & R[6] & R[4] & R[3];  // brackets and '\\'
+    LaneClasses c;
+    c.q = h001 & lo010 & ~R[4] & ~R[3];
+    const uint32_t colon = h001 & lo010 & R[4] & R[3];
+    const uint32_t comma = h001 & lo100 & ~R[4] & R[3];
+    c.bs = h01x11 & lo100 & ~R[5];
+    c.br = h01x11 & R[0] & (R[1] ^ R[2]);
+    c.op = colon | comma | c.br;
+    c.b1 = R[1] & c.br;
+    c.b5 = R[5] & c.br;
+    if (JSONL) {
+        const uint32_t nl = ~R[7] & ~R[6] & ~R[5] & ~R[4] & R[3] & lo010;
+        c.op |= nl;
+        c.b5 |= nl;
+    }
+    return c;
+}
+
+// UTF-8 byte classes as masks: continuation (80-BF), lead of two or more
+// (>= C0), of three or more (>= E0), of four or more (>= F0).
+struct Utf8Masks {
+    uint32_t c, l, e, f;
+};
+__host__ __device__ __forceinline__ Utf8Masks utf8_masks(const uint32_t (&R)[8]) {
+    Utf8Masks u;
+    u.c = R[7] & ~R[6];
+    u.l = R[7] & R[6];
+    u.e = u.l & R[5];
+    u.f = u.e & R[4];
+    return u;
+}
+
+// The masks of the four bytes before a wave (a little-endian word, its last
+// byte at bit 31), as the previous lane's masks would hold them.
+__host__ __device__ __forceinline__ Utf8Masks utf8_masks_before(uint32_t prev_dw) {
+    Utf8Masks u{0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t b = (prev_dw >> (8 * j)) & 0xFFu, bit = 1u << (28 + j);
+        if ((b & 0xC0u) == 0x80u) u.c |= bit;
+        if (b >= 0xC0u) u.l |= bit;
+        if (b >= 0xE0u) u.e |= bit;
+        if (b >= 0xF0u) u.f |= bit;
+    }
+    return u;
+}
+
 // Reference UTF-8 rule (checkUTF8, parse_standard_json.cu:210-262, reduced to
-// what survives :122-123) over the lane's eight words, on transposed data:
-// word 2s is slot s of transposed words 0-3, word 2s+1 slot s of words 4-7.
-//  * a word with a high byte: byte p errs iff
-//      [p-1 and p are continuations] != [p-2 >= E0 or p-3 >= F0]
+// what survives :122-123) on the masks of a lane's 32 bytes; p holds the
+// masks of the 32 bytes before them (only their top three bits are read).
+// The reference checks 4-byte words (bytes 4w..4w+3 of the lane here):
+//  * a word with a high byte: byte i errs iff
+//      [i-1 and i are continuations] != [i-2 >= E0 or i-3 >= F0]
 //  * a word without one errs iff the previous word ends in a byte >= C0,
-//    or its third byte is >= E0, or its second >= F0 (:219-223, :239-242)
-// The class bits are CONT = bit 1, E = bit 5, F = bit 6; the per-byte error
-// terms are formed in bit 1 and the ASCII-word terms in bit 7.
-// pk5/pk6/pk7/pr7 carry the previous lane's transposed words 5, 6, 7 (classes)
-// and 7 (raw).  vA/vB: 0xFF in slot s if word 2s / 2s+1 lies inside the unit.
-__device__ __forceinline__ uint32_t utf8_lane_error(const uint32_t (&r)[8], const uint32_t (&k)[8],
-                                                    uint32_t pk5, uint32_t pk6, uint32_t pk7,
-                                                    uint32_t pr7, uint32_t vA, uint32_t vB) {
-    const uint32_t ks5 = alignbyte3(k[5], pk5), ks6 = alignbyte3(k[6], pk6);
-    const uint32_t ks7 = alignbyte3(k[7], pk7), rs7 = alignbyte3(r[7], pr7);
-    const uint32_t e0 = (ks7 & k[0]) ^ ((ks6 >> 4) | (ks5 >> 5));
-    const uint32_t e1 = (k[0] & k[1]) ^ ((ks7 >> 4) | (ks6 >> 5));
-    const uint32_t e2 = (k[1] & k[2]) ^ ((k[0] >> 4) | (ks7 >> 5));
-    const uint32_t e3 = (k[2] & k[3]) ^ ((k[1] >> 4) | (k[0] >> 5));
-    const uint32_t e4 = (k[3] & k[4]) ^ ((k[2] >> 4) | (k[1] >> 5));
-    const uint32_t e5 = (k[4] & k[5]) ^ ((k[3] >> 4) | (k[2] >> 5));
-    const uint32_t e6 = (k[5] & k[6]) ^ ((k[4] >> 4) | (k[3] >> 5));
-    const uint32_t e7 = (k[6] & k[7]) ^ ((k[5] >> 4) | (k[4] >> 5));
-    const uint32_t errA = (e0 | e1 | e2 | e3) & 0x02020202u;
-    const uint32_t errB = (e4 | e5 | e6 | e7) & 0x02020202u;
-    const uint32_t gA = (r[0] | r[1] | r[2] | r[3]) & 0x80808080u;
-    const uint32_t gB = (r[4] | r[5] | r[6] | r[7]) & 0x80808080u;
-    const uint32_t gen = ((errA & (gA >> 6)) & vA) | ((errB & (gB >> 6)) & vB);
-    const uint32_t ascA = ((rs7 & ~(ks7 << 6)) | (ks6 << 2) | (ks5 << 1)) & ~gA & vA;
-    const uint32_t ascB = ((r[3] & ~(k[3] << 6)) | (k[2] << 2) | (k[1] << 1)) & ~gB & vB;
-    return gen | ((ascA | ascB) & 0x80808080u);
+//    or its third byte is >= E0, or its second >= F0 (:219-223, :239-242),
+//    i.e. the same three tests at the word's first byte.
+// Results are gathered at bit 4w+3 of each word; `valid` selects the words
+// that lie in the unit (bit 4w+3 set iff byte 4w is inside it).
+__host__ __device__ __forceinline__ uint32_t utf8_plane_error(uint32_t high, const Utf8Masks &u,
+                                                              const Utf8Masks &p, uint32_t valid) {
+    const uint32_t c1 = alignbit(u.c, p.c, 31), l1 = alignbit(u.l, p.l, 31);
+    const uint32_t e2 = alignbit(u.e, p.e, 30), f3 = alignbit(u.f, p.f, 29);
+    const uint32_t x = e2 | f3;
+    const uint32_t eb = (u.c & c1) ^ x;  // per byte, for words with a high byte
+    const uint32_t ea = (l1 | x) << 3;   // at a word's first byte, moved to its bit 3
+    uint32_t wh = high | (high << 1);
+    wh |= wh << 2;  // bit 4w+3: word w has a high byte
+    uint32_t we = eb | (eb << 1);
+    we |= we << 2;  // bit 4w+3: some byte of word w errs
+    return ((wh & we) | (~wh & ea)) & valid;
 }
 
 // ---------------------------------------------------------------------------
@@ -176,85 +244,72 @@ __device__ __forceinline__ void publish_inc(unsigned long long *st, uint32_t til
                   (2ull << 62) | ((unsigned long long)state << 60) | n_incl);
 }
 
-// One wave walks back from `tile` over a window of 64 x kLookPer tiles: lane
-// i reads tiles tile - 1 - (kLookPer * i + k), k = 0..kLookPer-1 (position
-// q = kLookPer * i + k, 0 = the newest).  When an inclusive tile lies in the
-// window and every newer tile is an aggregate, the maps of those tiles are
-// composed from the oldest to give each one's entry state, hence its e bit
-// and its counts; their sum plus the inclusive tile's counts is this tile's
-// exclusive prefix.  Tiles before the unit read as inclusive with state 0 and
-// counts 0.  The walk waits while no inclusive tile is in the window (the
-// oldest running tile always finds one, so waiting cannot deadlock).  The
-// inclusive frontier advances up to a window per memory round trip, which is
-// what bounds a launch of many tiles.
-#ifndef CJX_LOOK_PER
-#define CJX_LOOK_PER 1
-#endif
-// Sleep between polls (s_sleep units of 64 cycles).  Pollers read the status
-// lines the frontier tiles are publishing into; fewer polls let the
-// inclusive frontier advance faster: 2 -> 48 measured 0.950 -> 0.922 ms per
-// 2 GB (16: 0.933, 32: 0.925, 64: 0.928, 127: 0.936; an exponential backoff
-// and a first probe of the A words only were slower).
+// Sleep between polls of a window with a tile that has published nothing
+// yet (s_sleep units of 64 cycles).
 #ifndef CJX_LB_SLEEP
-#define CJX_LB_SLEEP 48
+#define CJX_LB_SLEEP 8
 #endif
-constexpr int kLookPer = CJX_LOOK_PER;
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+    return wave_read(wave_incl_scan(x), 63);
+}
+
+// Counts of a span of tiles as a function of the state (c, s) it is entered
+// in.  Entered at (c, s), tile j of the span is entered at an affine image of
+// (c, s), and its e bit is s ^ (a linear term in c) ^ (a constant): s flips
+// every tile's e, which turns each tile's n into opc - n.  So the span's count
+// entered at (c, s) is t[c] when s = 0 and sum - t[c] when s = 1.
+struct SpanCounts {
+    uint32_t n0, n1, nsum, m0, m1, msum;
+};
+__device__ __forceinline__ uint32_t span_n(const SpanCounts &f, uint32_t st) {
+    const uint32_t t = (st & 1u) ? f.n1 : f.n0;
+    return (st & 2u) ? f.nsum - t : t;
+}
+__device__ __forceinline__ uint32_t span_m(const SpanCounts &f, uint32_t st) {
+    const uint32_t t = (st & 1u) ? f.m1 : f.m0;
+    return (st & 2u) ? f.msum - t : t;
+}
+
+// Decoupled look-back over windows of 64 tiles (lane i reads tile base - i,
+// lane 0 the newest).  A window whose tiles have all published an aggregate
+// is folded into the span seen so far (its map, and its counts as a function
+// of its entry state) and the walk moves on to the 64 older tiles; it stops
+// at the newest inclusive tile, whose state and counts start the chain.  It
+// waits only for a tile newer than that one that has published nothing yet;
+// tiles are taken in launch order and every running tile publishes its
+// aggregate before it looks back, so the wait always ends.  Tiles before the
+// unit read as inclusive with state 0 and counts 0.
 __device__ __forceinline__ void lookback_tile(const unsigned long long *st, uint32_t tile,
                                               uint32_t unit_tile0, uint32_t &entry,
                                               uint32_t &n_excl, uint32_t &m_excl) {
     const uint32_t lane = lane_id();
+    // The span between the window being read and this tile: its map (the
+    // older part first) and its counts.
+    uint32_t fmap = kMapIdentity;
+    SpanCounts f{0u, 0u, 0u, 0u, 0u, 0u};
+    int64_t base = (int64_t)tile - 1;
     for (;;) {
-        unsigned long long a[kLookPer], b[kLookPer];
-#pragma unroll
-        for (int k = 0; k < kLookPer; k++) {
-            const int64_t j = (int64_t)tile - 1 - (int64_t)(kLookPer * lane + k);
-            a[k] = b[k] = 2ull << 62;
-            if (j >= (int64_t)unit_tile0) {
-                a[k] = agent_load64(&st[2 * j]);
-                b[k] = agent_load64(&st[2 * j + 1]);
-            }
+        const int64_t j = base - (int64_t)lane;
+        unsigned long long a = 2ull << 62, b = 2ull << 62;
+        if (j >= (int64_t)unit_tile0) {
+            a = agent_load64(&st[2 * j]);
+            b = agent_load64(&st[2 * j + 1]);
         }
-        // Newest inclusive entry of the lane, and whether every newer entry
-        // of the lane is an aggregate.
-        uint32_t k_inc = kLookPer, inc_state = 0;
-        bool ok = true;
-#pragma unroll
-        for (int k = kLookPer - 1; k >= 0; k--) {
-            const uint32_t fa = (uint32_t)(a[k] >> 62), fb = (uint32_t)(b[k] >> 62);
-            const uint32_t kind = (fa == fb) ? fa : kStInvalid;
-            if (kind == kStInc) {
-                k_inc = (uint32_t)k;
-                inc_state = (uint32_t)(a[k] >> 60) & 3u;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kLookPer; k++) {
-            const uint32_t fa = (uint32_t)(a[k] >> 62), fb = (uint32_t)(b[k] >> 62);
-            const uint32_t kind = (fa == fb) ? fa : kStInvalid;
-            if ((uint32_t)k < k_inc && kind != kStAgg) ok = false;
-        }
-        const uint64_t inc_lanes = __ballot(k_inc < (uint32_t)kLookPer);
-        if (!inc_lanes) {
+        const uint32_t fa = (uint32_t)(a >> 62), fb = (uint32_t)(b >> 62);
+        const uint32_t kind = (fa == fb) ? fa : kStInvalid;
+        const uint64_t inc_lanes = __ballot(kind == kStInc);
+        const uint32_t li = inc_lanes ? (uint32_t)__ffsll((long long)inc_lanes) - 1 : 64u;
+        const uint64_t newer = (li >= 64) ? ~0ull : ((1ull << li) - 1ull);
+        if (__ballot(kind != kStAgg) & newer) {  // a newer tile has published nothing yet
             __builtin_amdgcn_s_sleep(CJX_LB_SLEEP);
             continue;
         }
-        const uint32_t li = (uint32_t)__ffsll((long long)inc_lanes) - 1;
-        const uint64_t upto = (li == 63) ? ~0ull : ((2ull << li) - 1ull);
-        if (__ballot(!ok) & upto) {  // a tile newer than the inclusive one is not ready
-            __builtin_amdgcn_s_sleep(CJX_LB_SLEEP);
-            continue;
-        }
-        const uint32_t qi = kLookPer * li + wave_read(k_inc, (int)li);
-        const uint32_t sk = wave_read(inc_state, (int)li);
-        // Lane map: its newer-than-inclusive entries, oldest first.
-        uint32_t L = kMapIdentity;
-#pragma unroll
-        for (int k = kLookPer - 1; k >= 0; k--) {
-            const uint32_t q = kLookPer * lane + (uint32_t)k;
-            if (q < qi) L = map_compose(L, (uint32_t)((a[k] >> 58) & 0xFu));
-        }
-        // P = L(lane 63) then ... then L(lane): inclusive scan from the oldest.
-        uint32_t P = L;
+        // The aggregates newer than the inclusive tile (or the whole window):
+        // maps composed from the oldest, P = L(63) then ... then L(lane).
+        const bool part = lane < li;
+        const uint32_t map = (uint32_t)((a >> 58) & 0xFu);
+        uint32_t P = part ? map : kMapIdentity;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t other = (uint32_t)__shfl_down((int)P, off, 64);
@@ -262,39 +317,40 @@ __device__ __forceinline__ void lookback_tile(const unsigned long long *st, uint
         }
         uint32_t E = (uint32_t)__shfl_down((int)P, 1, 64);
         if (lane == 63) E = kMapIdentity;
-        uint32_t stj = map_apply(E, sk);
-        uint32_t cn = 0, cm = 0;
-#pragma unroll
-        for (int k = kLookPer - 1; k >= 0; k--) {
-            const uint32_t q = kLookPer * lane + (uint32_t)k;
-            if (q < qi) {
-                const uint32_t map = (uint32_t)((a[k] >> 58) & 0xFu);
-                const uint32_t e = ((stj >> 1) ^ (stj & (map >> 2))) & 1u;
-                const uint32_t n0 = (uint32_t)((a[k] >> 29) & kMask29), m0 = (uint32_t)(a[k] & kMask29);
-                const uint32_t opc = (uint32_t)((b[k] >> 29) & kMask29);
-                const uint32_t brc = (uint32_t)(b[k] & kMask29);
-                cn += e ? opc - n0 : n0;
-                cm += e ? brc - m0 : m0;
-                stj = map_apply(map, stj);
-            } else if (q == qi) {
-                cn += (uint32_t)(a[k] & kMask31);
-                cm += (uint32_t)(b[k] & kMask31);
-            }
+        // The tile's counts with the part entered at (0, 0) and at (1, 0).
+        uint32_t cn0 = 0, cn1 = 0, cm0 = 0, cm1 = 0, opc = 0, brc = 0;
+        if (part) {
+            const uint32_t n0 = (uint32_t)((a >> 29) & kMask29), m0 = (uint32_t)(a & kMask29);
+            opc = (uint32_t)((b >> 29) & kMask29);
+            brc = (uint32_t)(b & kMask29);
+            const uint32_t s0 = map_apply(E, 0u), s1 = map_apply(E, 1u);
+            const uint32_t e0 = ((s0 >> 1) ^ (s0 & (map >> 2))) & 1u;
+            const uint32_t e1 = ((s1 >> 1) ^ (s1 & (map >> 2))) & 1u;
+            cn0 = e0 ? opc - n0 : n0;
+            cn1 = e1 ? opc - n0 : n0;
+            cm0 = e0 ? brc - m0 : m0;
+            cm1 = e1 ? brc - m0 : m0;
         }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            cn += (uint32_t)__shfl_xor((int)cn, off, 64);
-            cm += (uint32_t)__shfl_xor((int)cm, off, 64);
+        const uint32_t gmap = wave_read(P, 0);
+        SpanCounts g{wave_sum(cn0), wave_sum(cn1), wave_sum(opc),
+                     wave_sum(cm0), wave_sum(cm1), wave_sum(brc)};
+        // The part (older) then the span seen so far.
+        const uint32_t x0 = map_apply(gmap, 0u), x1 = map_apply(gmap, 1u);
+        SpanCounts h{g.n0 + span_n(f, x0), g.n1 + span_n(f, x1), g.nsum + f.nsum,
+                     g.m0 + span_m(f, x0), g.m1 + span_m(f, x1), g.msum + f.msum};
+        f = h;
+        fmap = map_compose(gmap, fmap);
+        if (li < 64) {
+            const uint32_t sk = (uint32_t)(wave_read((uint32_t)(a >> 32), (int)li) >> 28) & 3u;
+            const uint32_t ni = wave_read((uint32_t)(a & kMask31), (int)li);
+            const uint32_t mi = wave_read((uint32_t)(b & kMask31), (int)li);
+            entry = map_apply(fmap, sk);
+            n_excl = ni + span_n(f, sk);
+            m_excl = mi + span_m(f, sk);
+            return;
         }
-        n_excl = cn;
-        m_excl = cm;
-        entry = map_apply(wave_read(P, 0), sk);
-        return;
+        base -= 64;
     }
-}
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
-    return wave_read(wave_incl_scan(x), 63);
 }
 
 // State of a wave after phases 1 and 2 (everything before the look-back).
@@ -324,15 +380,11 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
     if (wave_off > 0 && wave_off < n)
         prev_dw = *reinterpret_cast<const uint32_t *>(ubase + wave_off - 4);
 
-    // Carries entering the wave: the four bytes before it (UTF-8 context).
-    uint32_t c_k5, c_k6, c_k7, c_r7;
-    {
-        const uint32_t cp = classify<JSONL>(prev_dw);
-        c_k7 = cp;
-        c_k6 = cp << 8;
-        c_k5 = cp << 16;
-        c_r7 = prev_dw;
-    }
+    // UTF-8 masks of the bytes before the lane: the previous lane's, for lane
+    // 0 the previous round's lane 63; entering the wave, the four bytes
+    // before it at bits 28-31.
+    Utf8Masks cu = utf8_masks_before(prev_dw);
+    bool hi_carry = ((cu.c | cu.l) >> 29) != 0u;
 
     uint32_t(&instr_r)[ROUNDS] = ws.instr_r;
     uint32_t(&op_r)[ROUNDS] = ws.op_r;
@@ -353,51 +405,49 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
             continue;
         }
         const uint64_t lane_off = wave_off + (uint64_t)i * kRoundBytes + (uint64_t)lane * kLaneBytes;
-        uint32_t r[8], k[8];
-        transpose4(x[i & 1][0], x[i & 1][2], x[i & 1][4], x[i & 1][6], r[0], r[1], r[2], r[3]);
-        transpose4(x[i & 1][1], x[i & 1][3], x[i & 1][5], x[i & 1][7], r[4], r[5], r[6], r[7]);
+        // Byte transposes (word d, byte b <- lane byte 8b + d), then one 8x8
+        // bit transpose per byte slot: R[k] is the plane of byte bit k.
+        uint32_t R[8];
+        transpose4(x[i & 1][0], x[i & 1][2], x[i & 1][4], x[i & 1][6], R[0], R[1], R[2], R[3]);
+        transpose4(x[i & 1][1], x[i & 1][3], x[i & 1][5], x[i & 1][7], R[4], R[5], R[6], R[7]);
         if (i + 2 < ROUNDS)
             load_lane(ubase, lane_off + 2 * kRoundBytes, n, INSIDE, x[i & 1]);
-#pragma unroll
-        for (int d = 0; d < 8; d++) k[d] = classify<JSONL>(r[d]);
+        transpose8x8(R);
 
-        // ---- phase 1: UTF-8 ---------------------------------------------------
-        {
-            const uint32_t pk5 = wave_shr1(k[5], c_k5), pk6 = wave_shr1(k[6], c_k6);
-            const uint32_t pk7 = wave_shr1(k[7], c_k7), pr7 = wave_shr1(r[7], c_r7);
-            const uint32_t high = ((r[0] | r[1] | r[2] | r[3] | r[4] | r[5] | r[6] | r[7]) &
-                                   0x80808080u) |
-                                  ((pk5 | pk6 | pk7) & 0x62000000u) | (pr7 & 0x80000000u);
-            if (weak_utf8 && __ballot(high != 0)) {
-                uint32_t vA = ~0u, vB = ~0u;
+        // ---- phase 1: UTF-8 (reference rule) --------------------------------
+        // Skipped for a round without high bytes whose carry has none either.
+        if (weak_utf8) {
+            const Utf8Masks u = utf8_masks(R);
+            if (__ballot(R[7] != 0u) || hi_carry) {
+                const Utf8Masks pm{wave_shr1(u.c, cu.c), wave_shr1(u.l, cu.l), wave_shr1(u.e, cu.e),
+                                   wave_shr1(u.f, cu.f)};
+                uint32_t valid = 0x88888888u;
                 if (!INSIDE && lane_off + 32 > n) {
-                    vA = vB = 0;
+                    valid = 0;
 #pragma unroll
-                    for (int s = 0; s < 4; s++) {
-                        if (lane_off + 8 * s < n) vA |= 0xFFu << (8 * s);
-                        if (lane_off + 8 * s + 4 < n) vB |= 0xFFu << (8 * s);
-                    }
+                    for (int w4 = 0; w4 < 8; w4++)
+                        if (lane_off + 4 * w4 < n) valid |= 8u << (4 * w4);
                 }
-                if (high && utf8_lane_error(r, k, pk5, pk6, pk7, pr7, vA, vB)) utf8_bad = 1;
+                if (utf8_plane_error(R[7], u, pm, valid)) utf8_bad = 1;
+                cu.c = wave_read(u.c, 63);
+                cu.l = wave_read(u.l, 63);
+                cu.e = wave_read(u.e, 63);
+                cu.f = wave_read(u.f, 63);
+                hi_carry = ((cu.c | cu.l) >> 29) != 0u;
+            } else {
+                cu = Utf8Masks{0u, 0u, 0u, 0u};
+                hi_carry = false;
             }
-            c_k5 = wave_read(k[5], 63);
-            c_k6 = wave_read(k[6], 63);
-            c_k7 = wave_read(k[7], 63);
-            c_r7 = wave_read(r[7], 63);
         }
 
         // ---- phase 2: masks, escapes, strings ------------------------------
-        // Class bits 1 and 5 (CONT, E) give way to raw bits 1 and 5, then one
-        // transpose yields every mask.
-#pragma unroll
-        for (int d = 0; d < 8; d++) k[d] = (k[d] & ~0x22222222u) | (r[d] & 0x22222222u);
-        transpose8x8(k);
-        const uint32_t bs = k[4];
-        const uint32_t q = k[7];
-        br_r[i] = k[2];
-        op_r[i] = k[0] | k[2] | k[3];
-        b1[i][lane] = k[1] & k[2];
-        b5[i][lane] = (k[5] & k[2]) | (JSONL ? (k[0] & ~k[5]) : 0u);
+        const LaneClasses cls = classes_from_planes<JSONL>(R);
+        const uint32_t bs = cls.bs;
+        const uint32_t q = cls.q;
+        br_r[i] = cls.br;
+        op_r[i] = cls.op;
+        b1[i][lane] = cls.b1;
+        b5[i][lane] = cls.b5;
 
         const uint32_t all_bs = (bs == 0xFFFFFFFFu);
         const uint32_t run = (uint32_t)__clz(~bs);  // trailing backslash run of the lane
@@ -434,6 +484,9 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
         }
         if (NB) c_run = (uint32_t)((TP >> (63 - __clzll(NB))) & 1ull);
         s_run ^= (uint32_t)(__popcll(QP) & 1);
+        // The round's masks are formed here, not sunk past the loop: that
+        // keeps each round's intermediates from living across later rounds.
+        __asm__ volatile("" : "+v"(instr_r[i]), "+v"(op_r[i]), "+v"(br_r[i]));
     }
     lane_cnt0 = cnt_op0 | (cnt_br0 << 16);
     lane_all = cnt_op | (cnt_br << 16);
@@ -753,14 +806,26 @@ void launch_variant(hipStream_t s, uint32_t grid, const LaunchBufs &b, const uin
 // Register budgets: 4 waves per SIMD for 4, 8 and 16 waves of 16, 8 and
 // 4 KiB; the 32 KiB waves of the 2-wave tile carry twice the per-round
 // counts and take a 3-wave budget.
+#ifndef CJX_WPE_T2
+#define CJX_WPE_T2 3
+#endif
+#ifndef CJX_WPE_T4
+#define CJX_WPE_T4 4
+#endif
+#ifndef CJX_WPE_T8
+#define CJX_WPE_T8 4
+#endif
+#ifndef CJX_WPE_T16
+#define CJX_WPE_T16 4
+#endif
 template <bool JSONL>
 void launch_tiles(hipStream_t s, uint32_t grid, uint32_t tile_waves, const LaunchBufs &b,
                   const uint8_t *d_in, uint32_t *out_struct, uint32_t weak) {
     switch (tile_waves) {
-    case 2: launch_variant<JSONL, false, 2, 3>(s, grid, b, nullptr, d_in, 0u, out_struct, weak); break;
-    case 8: launch_variant<JSONL, false, 8, 4>(s, grid, b, nullptr, d_in, 0u, out_struct, weak); break;
-    case 16: launch_variant<JSONL, false, 16, 4>(s, grid, b, nullptr, d_in, 0u, out_struct, weak); break;
-    default: launch_variant<JSONL, false, 4, 4>(s, grid, b, nullptr, d_in, 0u, out_struct, weak); break;
+    case 2: launch_variant<JSONL, false, 2, CJX_WPE_T2>(s, grid, b, nullptr, d_in, 0u, out_struct, weak); break;
+    case 8: launch_variant<JSONL, false, 8, CJX_WPE_T8>(s, grid, b, nullptr, d_in, 0u, out_struct, weak); break;
+    case 16: launch_variant<JSONL, false, 16, CJX_WPE_T16>(s, grid, b, nullptr, d_in, 0u, out_struct, weak); break;
+    default: launch_variant<JSONL, false, 4, CJX_WPE_T4>(s, grid, b, nullptr, d_in, 0u, out_struct, weak); break;
     }
 }
 
@@ -867,4 +932,56 @@ extern "C" uint32_t cjx_debug_byte_class(uint32_t byte, int jsonl) {
     const uint32_t b = host_perm((uint32_t)(t35 >> 32), (uint32_t)t35, (r >> 3) & 7u);
     const uint32_t c = host_perm(0u, (uint32_t)kLutBits67, (r >> 6) & 3u);
     return (a & b & c) & 0xFFu;
+}
+
+// Host run of the tokenizer's lane step: the byte transposes, the bit planes
+// and the character classes of 32 bytes (out: bs, q, br, op, b1, b5).
+extern "C" void cjx_debug_lane_classes(const uint8_t *b32, int jsonl, uint32_t *out) {
+    using namespace cjx;
+    uint32_t x[8];
+    for (int k = 0; k < 8; k++)
+        x[k] = (uint32_t)b32[4 * k] | ((uint32_t)b32[4 * k + 1] << 8) |
+               ((uint32_t)b32[4 * k + 2] << 16) | ((uint32_t)b32[4 * k + 3] << 24);
+    uint32_t R[8];
+    transpose4(x[0], x[2], x[4], x[6], R[0], R[1], R[2], R[3]);
+    transpose4(x[1], x[3], x[5], x[7], R[4], R[5], R[6], R[7]);
+    transpose8x8(R);
+    const LaneClasses c = jsonl ? classes_from_planes<true>(R) : classes_from_planes<false>(R);
+    out[0] = c.bs; out[1] = c.q; out[2] = c.br; out[3] = c.op; out[4] = c.b1; out[5] = c.b5;
+}
+
+// Host run of the tokenizer's reference-rule UTF-8 check over a unit: lanes
+// of 32 bytes in order, each carrying its masks to the next, and every
+// `wave_bytes` bytes a new wave whose carry comes from the four bytes before
+// it, as on the device.  Returns 1 if the unit fails the check.
+extern "C" int cjx_debug_weak_utf8(const uint8_t *d, size_t n, size_t wave_bytes) {
+    using namespace cjx;
+    int bad = 0;
+    Utf8Masks cu{0u, 0u, 0u, 0u};
+    for (size_t o = 0; o < n; o += 32) {
+        if (wave_bytes && o % wave_bytes == 0) {
+            uint32_t prev_dw = 0;
+            if (o > 0)
+                prev_dw = (uint32_t)d[o - 4] | ((uint32_t)d[o - 3] << 8) |
+                          ((uint32_t)d[o - 2] << 16) | ((uint32_t)d[o - 1] << 24);
+            cu = utf8_masks_before(prev_dw);
+        }
+        uint8_t b[32] = {0};
+        for (size_t k = 0; k < 32 && o + k < n; k++) b[k] = d[o + k];
+        uint32_t x[8];
+        for (int k = 0; k < 8; k++)
+            x[k] = (uint32_t)b[4 * k] | ((uint32_t)b[4 * k + 1] << 8) |
+                   ((uint32_t)b[4 * k + 2] << 16) | ((uint32_t)b[4 * k + 3] << 24);
+        uint32_t R[8];
+        transpose4(x[0], x[2], x[4], x[6], R[0], R[1], R[2], R[3]);
+        transpose4(x[1], x[3], x[5], x[7], R[4], R[5], R[6], R[7]);
+        transpose8x8(R);
+        const Utf8Masks u = utf8_masks(R);
+        uint32_t valid = 0;
+        for (int w4 = 0; w4 < 8; w4++)
+            if (o + 4 * (size_t)w4 < n) valid |= 8u << (4 * w4);
+        if (utf8_plane_error(R[7], u, cu, valid)) bad = 1;
+        cu = u;
+    }
+    return bad;
 }

@@ -291,3 +291,75 @@ def test_strict_utf8_word_steps_against_oracle():
             d[int(rng.integers(0, len(d)))] = int(rng.choice(pool))
         d = bytes(d)
         assert f(d, len(d)) == oracle.utf8_check(d, oracle.FLAG_UTF8_STRICT), d
+
+
+def _lane_class_want(b, jsonl):
+    """Class bits of one byte: bs, q, br, op, b1, b5 (as the tokenizer's masks)."""
+    br = b in (0x5B, 0x5D, 0x7B, 0x7D)
+    nl = jsonl and b == 0x0A
+    return (b == 0x5C, b == 0x22, br, br or b in (0x3A, 0x2C) or nl,
+            br and b in (0x5B, 0x7B), (br and b in (0x7B, 0x7D)) or nl)
+
+
+def test_lane_classes_from_bit_planes():
+    """The tokenizer's character classes from bit planes (host run of the
+    same byte transposes, bit transpose and plane products): every byte value
+    at every one of a lane's 32 positions, both modes."""
+    f = cj.lib().cjx_debug_lane_classes
+    f.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
+    f.restype = None
+    rng = np.random.default_rng(21)
+    lanes = [bytes((v + 37 * i) & 0xFF for i in range(32)) for v in range(256)]
+    lanes += [rng.integers(0, 256, 32, dtype=np.uint8).tobytes() for _ in range(300)]
+    lanes += [b'{"a":[1,2],"b\\\\":"x"}\n[]{}::,,\\"\n\n\x0a\x22\x5c'[:32].ljust(32, b" ")]
+    out = (ctypes.c_uint32 * 6)()
+    for jsonl in (0, 1):
+        for lane in lanes:
+            assert len(lane) == 32
+            f(lane, jsonl, out)
+            for i, b in enumerate(lane):
+                want = _lane_class_want(b, jsonl)
+                got = tuple(bool((out[k] >> i) & 1) for k in range(6))
+                assert got == want, (jsonl, hex(b), i, got, want)
+
+
+def test_weak_utf8_bit_planes_against_oracle():
+    """Host run of the tokenizer's reference-rule UTF-8 check on bit planes
+    (lane chain, wave starts carried from the four bytes before them, word
+    validity at the end) against the oracle's per-word restatement of the
+    reference's checkUTF8: every short sequence at every offset of a 96-byte
+    window for three wave sizes, and random byte strings."""
+    f = cj.lib().cjx_debug_weak_utf8
+    f.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_size_t]
+    f.restype = ctypes.c_int
+    seqs = [b"\xc3\xa9", b"\xe2\x82\xac", b"\xf0\x9f\x98\x80", b"\xc3", b"\xe2\x82",
+            b"\xf0\x9f\x98", b"\xc0\xaf", b"\xe0\x80\xaf", b"\xed\xa0\x80", b"\xf5\x80", b"\xff",
+            b"\x80", b"\xbf\xbf", b"\xc3\xa9\xa9", b"\xe2\x82\xac\x80", b"\xf8\x88\x80\x80",
+            b"\xe2", b"\xf0", b"\xf0\x9f"]
+    n_err = 0
+    for sq in seqs:
+        for off in range(0, 96 - len(sq)):
+            for tail in (0, 1, 3, 6):
+                d = b"a" * off + sq + b"b" * tail
+                want = oracle.utf8_check(d, 0)
+                n_err += want
+                for wb in (32, 64, 2048):
+                    assert f(d, len(d), wb) == want, (sq, off, tail, wb)
+    assert n_err > 0
+    rng = np.random.default_rng(23)
+    pool = np.array([0x41, 0x22, 0x80, 0x8F, 0xA0, 0xBF, 0xC0, 0xC2, 0xDF, 0xE0, 0xE1, 0xED,
+                     0xEF, 0xF0, 0xF4, 0xF5, 0xFF], np.uint8)
+    for _ in range(6000):
+        d = rng.choice(pool, rng.integers(1, 200)).tobytes()
+        want = oracle.utf8_check(d, 0)
+        for wb in (32, 128):
+            assert f(d, len(d), wb) == want, (d, wb)
+    for _ in range(200):  # valid text with an occasional stray byte
+        s = "".join(chr(int(c)) for c in rng.choice([0x41, 0xE9, 0x20AC, 0x1F600, 0x7F], 400))
+        d = bytearray(s.encode())
+        if rng.random() < 0.5:
+            d[int(rng.integers(0, len(d)))] = int(rng.choice(pool))
+        d = bytes(d)
+        want = oracle.utf8_check(d, 0)
+        for wb in (32, 256, 2048):
+            assert f(d, len(d), wb) == want, (d, wb)
