@@ -10,7 +10,8 @@ __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << __lane_id())
 
 // DPP control codes (GFX9 encoding).
 constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
-constexpr int kDppWaveShr1 = 0x138, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+constexpr int kDppWaveShr1 = 0x138, kDppWaveRor1 = 0x13C, kDppRowBcast15 = 0x142,
+              kDppRowBcast31 = 0x143;
 
 template <int CTRL, int ROW_MASK = 0xF>
 __device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t v) {
@@ -38,6 +39,13 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
 // Value of lane l-1 in lane l; lane 0 receives `carry`.
 __device__ __forceinline__ uint32_t wave_shr1(uint32_t v, uint32_t carry) {
     return dpp<kDppWaveShr1>(carry, v);
+}
+
+// Value of lane l-1 in lane l; lane 0 receives lane 63 of `prev` (the same
+// register one step earlier): a rotate of prev, then a shift of v over it.
+__device__ __forceinline__ uint32_t wave_shr1_from(uint32_t v, uint32_t prev) {
+    const uint32_t rot = (uint32_t)__builtin_amdgcn_mov_dpp((int)prev, kDppWaveRor1, 0xF, 0xF, false);
+    return dpp<kDppWaveShr1>(rot, v);
 }
 
 // Set bits of a wave-wide mask in the lanes below this one (v_mbcnt).

@@ -244,11 +244,25 @@ __device__ __forceinline__ void publish_inc(unsigned long long *st, uint32_t til
                   (2ull << 62) | ((unsigned long long)state << 60) | n_incl);
 }
 
-// Sleep between polls of a window with a tile that has published nothing
-// yet (s_sleep units of 64 cycles).
-#ifndef CJX_LB_SLEEP
-#define CJX_LB_SLEEP 8
+// Waiting in the look-back (s_sleep units of 64 cycles).  Polls are
+// agent-scope loads that miss the XCD's L2 and compete with the streaming
+// traffic, so the walk first sleeps once (by then the tiles ahead have mostly
+// published), then polls a window of kLookLanes tiles, and sleeps between
+// polls of a window that is not ready.  Measured per 2 GB, with 5 waves per
+// SIMD: windows of 8 / 16 / 32 / 48 / 64 tiles 0.900 / 0.822 / 0.791 / 0.791
+// / 0.811 ms (sleep 64); sleep 8 / 16 / 32 / 64 / 127 between polls (64-tile
+// windows) 0.883 (4 waves per SIMD) / 0.844 / 0.827 / 0.811 / 0.798 ms; a
+// first sleep of 127 before the first poll 0.794 -> 0.779 ms.
+#ifndef CJX_LB_PRESLEEP
+#define CJX_LB_PRESLEEP 127
 #endif
+#ifndef CJX_LB_SLEEP
+#define CJX_LB_SLEEP 64
+#endif
+#ifndef CJX_LB_LANES
+#define CJX_LB_LANES 32
+#endif
+constexpr int kLookLanes = CJX_LB_LANES;
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     return wave_read(wave_incl_scan(x), 63);
@@ -271,8 +285,8 @@ __device__ __forceinline__ uint32_t span_m(const SpanCounts &f, uint32_t st) {
     return (st & 2u) ? f.msum - t : t;
 }
 
-// Decoupled look-back over windows of 64 tiles (lane i reads tile base - i,
-// lane 0 the newest).  A window whose tiles have all published an aggregate
+// Decoupled look-back over windows of kLookLanes tiles (lane i reads tile
+// base - i, lane 0 the newest).  A window whose tiles have all published an aggregate
 // is folded into the span seen so far (its map, and its counts as a function
 // of its entry state) and the walk moves on to the 64 older tiles; it stops
 // at the newest inclusive tile, whose state and counts start the chain.  It
@@ -289,17 +303,19 @@ __device__ __forceinline__ void lookback_tile(const unsigned long long *st, uint
     uint32_t fmap = kMapIdentity;
     SpanCounts f{0u, 0u, 0u, 0u, 0u, 0u};
     int64_t base = (int64_t)tile - 1;
+    __builtin_amdgcn_s_sleep(CJX_LB_PRESLEEP);
     for (;;) {
         const int64_t j = base - (int64_t)lane;
         unsigned long long a = 2ull << 62, b = 2ull << 62;
-        if (j >= (int64_t)unit_tile0) {
+        const bool in_win = lane < (uint32_t)kLookLanes;
+        if (in_win && j >= (int64_t)unit_tile0) {
             a = agent_load64(&st[2 * j]);
             b = agent_load64(&st[2 * j + 1]);
         }
         const uint32_t fa = (uint32_t)(a >> 62), fb = (uint32_t)(b >> 62);
         const uint32_t kind = (fa == fb) ? fa : kStInvalid;
-        const uint64_t inc_lanes = __ballot(kind == kStInc);
-        const uint32_t li = inc_lanes ? (uint32_t)__ffsll((long long)inc_lanes) - 1 : 64u;
+        const uint64_t inc_lanes = __ballot(kind == kStInc && in_win);
+        const uint32_t li = inc_lanes ? (uint32_t)__ffsll((long long)inc_lanes) - 1 : (uint32_t)kLookLanes;
         const uint64_t newer = (li >= 64) ? ~0ull : ((1ull << li) - 1ull);
         if (__ballot(kind != kStAgg) & newer) {  // a newer tile has published nothing yet
             __builtin_amdgcn_s_sleep(CJX_LB_SLEEP);
@@ -340,7 +356,7 @@ __device__ __forceinline__ void lookback_tile(const unsigned long long *st, uint
                      g.m0 + span_m(f, x0), g.m1 + span_m(f, x1), g.msum + f.msum};
         f = h;
         fmap = map_compose(gmap, fmap);
-        if (li < 64) {
+        if (li < (uint32_t)kLookLanes) {
             const uint32_t sk = (uint32_t)(wave_read((uint32_t)(a >> 32), (int)li) >> 28) & 3u;
             const uint32_t ni = wave_read((uint32_t)(a & kMask31), (int)li);
             const uint32_t mi = wave_read((uint32_t)(b & kMask31), (int)li);
@@ -349,7 +365,7 @@ __device__ __forceinline__ void lookback_tile(const unsigned long long *st, uint
             m_excl = mi + span_m(f, sk);
             return;
         }
-        base -= 64;
+        base -= kLookLanes;
     }
 }
 
@@ -360,6 +376,9 @@ struct WaveScan {
     uint32_t c_run, s_run, seen, dflag, utf8_bad, lane_cnt0, lane_all;
 };
 
+#ifndef CJX_PREFETCH
+#define CJX_PREFETCH 2
+#endif
 // Phases 1 and 2 over the rounds of one wave.  INSIDE: the wave's whole
 // range lies inside the unit (a uniform condition, true for all but the last
 // waves of a unit), so the rounds need no end checks and the loads no bounds
@@ -370,12 +389,15 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
                                             uint32_t lane, uint64_t lt, uint32_t weak_utf8,
                                             uint32_t (&b1)[ROUNDS][64],
                                             uint32_t (&b5)[ROUNDS][64], WaveScan<ROUNDS> &ws) {
-    // Rounds stream through a double buffer: round i+2 is loaded as soon as
+    // Rounds stream through PF buffers: round i+PF is loaded as soon as
     // round i has been transposed.
-    uint32_t x[2][8];
-    load_lane(ubase, wave_off + (uint64_t)lane * kLaneBytes, n, INSIDE, x[0]);
-    if (ROUNDS > 1)
-        load_lane(ubase, wave_off + kRoundBytes + (uint64_t)lane * kLaneBytes, n, INSIDE, x[1]);
+    constexpr int PF = CJX_PREFETCH;  // rounds loaded ahead
+    uint32_t x[PF][8];
+#pragma unroll
+    for (int r = 0; r < PF; r++)
+        if (r < ROUNDS)
+            load_lane(ubase, wave_off + (uint64_t)r * kRoundBytes + (uint64_t)lane * kLaneBytes, n,
+                      INSIDE, x[r]);
     uint32_t prev_dw = 0;
     if (wave_off > 0 && wave_off < n)
         prev_dw = *reinterpret_cast<const uint32_t *>(ubase + wave_off - 4);
@@ -384,7 +406,7 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
     // 0 the previous round's lane 63; entering the wave, the four bytes
     // before it at bits 28-31.
     Utf8Masks cu = utf8_masks_before(prev_dw);
-    bool hi_carry = ((cu.c | cu.l) >> 29) != 0u;
+    bool hi_prev = ((cu.c | cu.l) >> 29) != 0u;
 
     uint32_t(&instr_r)[ROUNDS] = ws.instr_r;
     uint32_t(&op_r)[ROUNDS] = ws.op_r;
@@ -408,19 +430,22 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
         // Byte transposes (word d, byte b <- lane byte 8b + d), then one 8x8
         // bit transpose per byte slot: R[k] is the plane of byte bit k.
         uint32_t R[8];
-        transpose4(x[i & 1][0], x[i & 1][2], x[i & 1][4], x[i & 1][6], R[0], R[1], R[2], R[3]);
-        transpose4(x[i & 1][1], x[i & 1][3], x[i & 1][5], x[i & 1][7], R[4], R[5], R[6], R[7]);
-        if (i + 2 < ROUNDS)
-            load_lane(ubase, lane_off + 2 * kRoundBytes, n, INSIDE, x[i & 1]);
+        uint32_t(&xi)[8] = x[i % PF];
+        transpose4(xi[0], xi[2], xi[4], xi[6], R[0], R[1], R[2], R[3]);
+        transpose4(xi[1], xi[3], xi[5], xi[7], R[4], R[5], R[6], R[7]);
+        if (i + PF < ROUNDS)
+            load_lane(ubase, lane_off + PF * kRoundBytes, n, INSIDE, xi);
         transpose8x8(R);
 
         // ---- phase 1: UTF-8 (reference rule) --------------------------------
         // Skipped for a round without high bytes whose carry has none either.
+        // cu: the previous round's masks (lane 63 of them reaches lane 0).
         if (weak_utf8) {
             const Utf8Masks u = utf8_masks(R);
-            if (__ballot(R[7] != 0u) || hi_carry) {
-                const Utf8Masks pm{wave_shr1(u.c, cu.c), wave_shr1(u.l, cu.l), wave_shr1(u.e, cu.e),
-                                   wave_shr1(u.f, cu.f)};
+            const bool hi = __ballot(R[7] != 0u) != 0ull;
+            if (hi || hi_prev) {
+                const Utf8Masks pm{wave_shr1_from(u.c, cu.c), wave_shr1_from(u.l, cu.l),
+                                   wave_shr1_from(u.e, cu.e), wave_shr1_from(u.f, cu.f)};
                 uint32_t valid = 0x88888888u;
                 if (!INSIDE && lane_off + 32 > n) {
                     valid = 0;
@@ -429,15 +454,9 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
                         if (lane_off + 4 * w4 < n) valid |= 8u << (4 * w4);
                 }
                 if (utf8_plane_error(R[7], u, pm, valid)) utf8_bad = 1;
-                cu.c = wave_read(u.c, 63);
-                cu.l = wave_read(u.l, 63);
-                cu.e = wave_read(u.e, 63);
-                cu.f = wave_read(u.f, 63);
-                hi_carry = ((cu.c | cu.l) >> 29) != 0u;
-            } else {
-                cu = Utf8Masks{0u, 0u, 0u, 0u};
-                hi_carry = false;
             }
+            cu = u;
+            hi_prev = hi;
         }
 
         // ---- phase 2: masks, escapes, strings ------------------------------
@@ -500,10 +519,8 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
 // A tile is WAVES waves of ROUNDS rounds (WAVES x ROUNDS x 2 KiB = 64 KiB);
 // a small-unit workgroup is WAVES waves of SROUNDS rounds, one unit per wave:
 // kTokRounds rounds cover the largest small unit, one round the units of at
-// most 2 KiB (fewer registers, so more of those waves are resident).  WPE is the register budget in waves per SIMD:
-// 4 x 8 at 4 waves per SIMD uses 103 VGPRs without spills; with the interior
-// and edge rounds as separate instantiations, the 5-wave budget (96 VGPRs)
-// spills and measured 4 % slower.
+// most 2 KiB (fewer registers, so more of those waves are resident).  WPE is
+// the register budget in waves per SIMD (see launch_tiles).
 template <bool JSONL, bool SMALL, int WAVES, int WPE, int SROUNDS = kTokRounds>
 __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 * WAVES) void k_tokenize(
     const uint8_t *__restrict__ in, const UnitDesc *__restrict__ units,
@@ -803,17 +820,19 @@ void launch_variant(hipStream_t s, uint32_t grid, const LaunchBufs &b, const uin
                        b.tile_st, b.ls, b.uout, out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
 }
 
-// Register budgets: 4 waves per SIMD for 4, 8 and 16 waves of 16, 8 and
-// 4 KiB; the 32 KiB waves of the 2-wave tile carry twice the per-round
-// counts and take a 3-wave budget.
+// Register budgets: 5 waves per SIMD for 4 and 8 waves of 16 and 8 KiB (88
+// VGPRs, JSON Lines 96, no spills; measured 0.852 -> 0.811 ms per 2 GB against
+// 4 waves; 6 waves spill even with one round loaded ahead, 0.810 ms), 4 for
+// 16 waves of 4 KiB; the 32 KiB waves of the 2-wave tile carry twice the
+// per-round counts and take a 3-wave budget.
 #ifndef CJX_WPE_T2
 #define CJX_WPE_T2 3
 #endif
 #ifndef CJX_WPE_T4
-#define CJX_WPE_T4 4
+#define CJX_WPE_T4 5
 #endif
 #ifndef CJX_WPE_T8
-#define CJX_WPE_T8 4
+#define CJX_WPE_T8 5
 #endif
 #ifndef CJX_WPE_T16
 #define CJX_WPE_T16 4
