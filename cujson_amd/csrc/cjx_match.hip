@@ -336,7 +336,12 @@ constexpr int kLaneBrackets = kMatchChunk / kMatchThreads;  // 16
 constexpr int kRowTile = 4096;
 constexpr uint32_t kWaveElems = kMatchChunk / (kMatchThreads / 64);  // a wave's brackets
 constexpr uint32_t kWaveTile = kRowTile / (kMatchThreads / 64);      // a wave's tile rows
-static_assert(kWaveElems == 64 * kLaneBrackets, "lane l of wave w: elements 1024 w + l + 64 k");
+static_assert(kWaveElems == 64 * kLaneBrackets, "a wave's elements: 64 lanes x 16");
+// Element of register k of lane l in wave w: 1024 w + 256 (k / 4) + 4 l + k % 4
+// (a contiguous range per wave, four consecutive elements per 16-byte access).
+__device__ __forceinline__ uint32_t elem_of(uint32_t w, uint32_t l, int k) {
+    return w * kWaveElems + 256u * (uint32_t)(k >> 2) + 4u * l + (uint32_t)(k & 3);
+}
 constexpr uint16_t kNoPart = 0xFFFF;
 
 // LDS writes of the wave are visible to all its lanes after this.
@@ -361,7 +366,7 @@ __launch_bounds__(kMatchThreads) void k_match_local(
     // rows they are copied into the matching state's LDS once matching is
     // done, so no LDS of their own limits the blocks per CU.
     __shared__ __align__(16) uint8_t s_ch[kMatchChunk];
-    __shared__ uint16_t s_part[kMatchChunk];
+    __shared__ __align__(16) uint16_t s_part[kMatchChunk];
     // Stack columns and low closers during matching; the row tile afterwards
     // (16-byte aligned: the tile is filled and read as int4).
     __shared__ __align__(16) union {
@@ -404,24 +409,37 @@ __launch_bounds__(kMatchThreads) void k_match_local(
             const bool last_chunk = xc + 1 == U.nchunks;
             const uint32_t r1 = last_chunk ? U.n_end : oc_idx[first + kMatchChunk];
             const int32_t d0 = ck[c].d0;
-            // All loads in flight before the first LDS store.
-            uint8_t vc[kLaneBrackets];
+            // All loads in flight before the first LDS store: the thread's 16
+            // characters [16 tid, 16 tid + 16) in one load, and the rows of
+            // the wave's elements, wave w holding [1024 w, 1024 w + 1024) for
+            // the pair-row tiles at the end (register k of lane l: element
+            // elem_of(w, l, k), four consecutive rows per 16-byte load).
+            const uint4 cw = chunk_lane_bytes(oc_char, first, cnt, tid);
             uint32_t vi[kLaneBrackets];
-            // Wave w holds elements [1024 w, 1024 w + 1024), lane l elements
-            // 1024 w + l + 64 k, for the pair-row tiles at the end.
+            if ((first & 3u) == 0 && (w + 1) * kWaveElems <= cnt) {
 #pragma unroll
-            for (int k = 0; k < kLaneBrackets; k++) {
-                const uint32_t e = w * kWaveElems + lane + 64u * k;
-                vc[k] = e < cnt ? oc_char[first + e] : 0;
-                vi[k] = e < cnt ? oc_idx[first + e] : 0;
+                for (int q = 0; q < kLaneBrackets / 4; q++) {
+                    const uint4 r = *reinterpret_cast<const uint4 *>(oc_idx + first + elem_of(w, lane, 4 * q));
+                    vi[4 * q] = r.x;
+                    vi[4 * q + 1] = r.y;
+                    vi[4 * q + 2] = r.z;
+                    vi[4 * q + 3] = r.w;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < kLaneBrackets; k++) {
+                    const uint32_t e = elem_of(w, lane, k);
+                    vi[k] = e < cnt ? oc_idx[first + e] : 0;
+                }
             }
             __syncthreads();  // LDS of the previous chunk is no longer read
             if (tid == 0) s_next = next;
-#pragma unroll
-            for (int k = 0; k < kLaneBrackets; k++) {
-                const uint32_t e = w * kWaveElems + lane + 64u * k;
-                s_ch[e] = vc[k];
-                s_part[e] = kNoPart;
+            *reinterpret_cast<uint4 *>(&s_ch[16u * tid]) = cw;
+            {
+                uint4 *sp4 = reinterpret_cast<uint4 *>(&s_part[16u * tid]);
+                const uint32_t none = 0xFFFFFFFFu;  // two kNoPart per word
+                sp4[0] = make_uint4(none, none, none, none);
+                sp4[1] = make_uint4(none, none, none, none);
             }
             if (lane == 0) s_wrow[w] = vi[0];  // the wave's first bracket row
             __syncthreads();
@@ -429,7 +447,6 @@ __launch_bounds__(kMatchThreads) void k_match_local(
 
             const uint32_t e0 = tid * kLaneBrackets;
             const uint32_t ne = cnt > e0 ? min((uint32_t)kLaneBrackets, cnt - e0) : 0u;
-            const uint4 cw = *reinterpret_cast<const uint4 *>(&s_ch[e0]);
             const uint32_t cv[4] = {cw.x, cw.y, cw.z, cw.w};
             int32_t mine = 0;
 #pragma unroll
@@ -531,13 +548,15 @@ __launch_bounds__(kMatchThreads) void k_match_local(
             // The chunk's bracket rows go to LDS over the matching state, for
             // the partners' rows.
 #pragma unroll
-            for (int k = 0; k < kLaneBrackets; k++)
-                s_u.row[w * kWaveElems + lane + 64u * k] = (int32_t)vi[k];
+            for (int q = 0; q < kLaneBrackets / 4; q++)
+                *reinterpret_cast<int4 *>(&s_u.row[elem_of(w, lane, 4 * q)]) =
+                    make_int4((int32_t)vi[4 * q], (int32_t)vi[4 * q + 1], (int32_t)vi[4 * q + 2],
+                              (int32_t)vi[4 * q + 3]);
             __syncthreads();
             int32_t pv[kLaneBrackets];
 #pragma unroll
             for (int k = 0; k < kLaneBrackets; k++) {
-                const uint32_t e = w * kWaveElems + lane + 64u * k;
+                const uint32_t e = elem_of(w, lane, k);
                 const uint32_t p = s_part[e];
                 pv[k] = p != kNoPart ? (int32_t)((uint32_t)s_u.row[p] - nb + pbase) : -1;
             }
