@@ -26,6 +26,9 @@ __device__ __forceinline__ uint32_t dpp_zero(uint32_t v) {
 
 // Inclusive prefix sum over the 64 lanes of a wave (row shifts, then row
 // broadcasts 15 and 31).
+// The result is opaque to the optimizer: otherwise a caller's `incl - x`
+// becomes the sum of the six shifted partials, which keeps them alive and
+// stops each shift from folding into its add.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
     x += dpp_zero<kDppRowShr1>(x);
     x += dpp_zero<kDppRowShr2>(x);
@@ -33,6 +36,7 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
     x += dpp_zero<kDppRowShr8>(x);
     x += dpp<kDppRowBcast15, 0xA>(0u, x);
     x += dpp<kDppRowBcast31, 0xC>(0u, x);
+    __asm__("" : "+v"(x));
     return x;
 }
 

@@ -387,8 +387,7 @@ struct WaveScan {
 template <bool JSONL, bool INSIDE, int ROUNDS>
 __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_off, uint64_t n,
                                             uint32_t lane, uint64_t lt, uint32_t weak_utf8,
-                                            uint32_t (&b1)[ROUNDS][64],
-                                            uint32_t (&b5)[ROUNDS][64], WaveScan<ROUNDS> &ws) {
+                                            uint2 (&b15)[ROUNDS][64], WaveScan<ROUNDS> &ws) {
     // Rounds stream through PF buffers: round i+PF is loaded as soon as
     // round i has been transposed.
     constexpr int PF = CJX_PREFETCH;  // rounds loaded ahead
@@ -465,8 +464,7 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
         const uint32_t q = cls.q;
         br_r[i] = cls.br;
         op_r[i] = cls.op;
-        b1[i][lane] = cls.b1;
-        b5[i][lane] = cls.b5;
+        b15[i][lane] = make_uint2(cls.b1, cls.b5);
 
         const uint32_t all_bs = (bs == 0xFFFFFFFFu);
         const uint32_t run = (uint32_t)__clz(~bs);  // trailing backslash run of the lane
@@ -541,14 +539,15 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
     // Bracket characters as two bit masks per lane-round, kept in LDS until
     // compaction: byte bit 1 ('[' / '{' have it, ']' / '}' do not) and byte
     // bit 5 (curly), both restricted to bracket bytes.
-    // For JSON Lines s_b5 also holds the '\n' bytes, which are never brackets:
-    // class 0 without byte bit 5 (':' has it, '\n' not); they give the
-    // per-row newline flags of record tables (row_nl).
-    __shared__ uint32_t s_b1[WAVES][ROUNDS][64], s_b5[WAVES][ROUNDS][64];
+    // For JSON Lines the bit-5 mask also holds the '\n' bytes, which are never
+    // brackets; they give the per-row newline flags of record tables (row_nl).
+    __shared__ uint2 s_b15[WAVES][ROUNDS][64];  // (bit-1 mask, bit-5 mask)
     const bool want_nl = JSONL && row_nl != nullptr;
 
     const uint32_t tid = threadIdx.x;
-    const uint32_t w = tid >> 6, lane = tid & 63;
+    // Wave-uniform values live in scalar registers (readfirstlane), so the
+    // addresses and bounds derived from them cost no vector instructions.
+    const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const uint64_t lt = lanemask_lt();
 
     uint32_t tile = 0, u;
@@ -563,8 +562,8 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
             s_unit = tile_unit[t];
         }
         __syncthreads();
-        tile = s_tile;
-        u = s_unit;
+        tile = __builtin_amdgcn_readfirstlane(s_tile);
+        u = __builtin_amdgcn_readfirstlane(s_unit);
     }
     const UnitDesc U = units[u];
     const uint32_t tile_in_unit = SMALL ? 0u : tile - U.tile0;
@@ -575,9 +574,9 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
     const bool inside = wave_off + (uint64_t)WAVE_BYTES <= U.n;
     WaveScan<ROUNDS> ws;
     if (inside)
-        scan_rounds<JSONL, true, ROUNDS>(ubase, wave_off, U.n, lane, lt, weak_utf8, s_b1[w], s_b5[w], ws);
+        scan_rounds<JSONL, true, ROUNDS>(ubase, wave_off, U.n, lane, lt, weak_utf8, s_b15[w], ws);
     else
-        scan_rounds<JSONL, false, ROUNDS>(ubase, wave_off, U.n, lane, lt, weak_utf8, s_b1[w], s_b5[w], ws);
+        scan_rounds<JSONL, false, ROUNDS>(ubase, wave_off, U.n, lane, lt, weak_utf8, s_b15[w], ws);
     const uint32_t(&instr_r)[ROUNDS] = ws.instr_r;
     const uint32_t(&op_r)[ROUNDS] = ws.op_r;
     const uint32_t(&br_r)[ROUNDS] = ws.br_r;
@@ -586,7 +585,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
 
     const uint64_t bad_lanes = __ballot(utf8_bad);
     const uint32_t wcnt0 = wave_sum(lane_cnt0), wall = wave_sum(lane_all);
-    uint32_t fix, nb, mb;
+    uint32_t fix, nb, mb, wave_rows;
     if constexpr (SMALL) {
         // The unit is this wave: entered in state (0, 0), so no fix.
         const uint32_t nu = wcnt0 & 0xFFFFu, mu = wcnt0 >> 16;
@@ -600,6 +599,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
         }
         fix = 0;
         nb = mb = U.out_off;
+        wave_rows = nu;
         (void)wall;
         (void)dflag;
         (void)seen;
@@ -662,13 +662,23 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
             }
         }
         __syncthreads();
-        fix = s_fix[w] ? ~0u : 0u;
+        const uint32_t fw = s_fix[w], c0 = s_cnt0[w] & 0xFFFFu, ca = s_cnt_all[w] & 0xFFFFu;
+        fix = fw ? ~0u : 0u;
         nb = s_nbase[w];
         mb = s_mbase[w];
+        wave_rows = fw ? ca - c0 : c0;
     }
+    // Uniform values in scalar registers.
+    nb = __builtin_amdgcn_readfirstlane(nb);
+    mb = __builtin_amdgcn_readfirstlane(mb);
+    fix = __builtin_amdgcn_readfirstlane(fix);
+    wave_rows = __builtin_amdgcn_readfirstlane(wave_rows);
 
     // ---- select structurals and compact them ---------------------------------
     const uint32_t cap_end = U.out_off + U.out_cap;
+    // Every row of the wave lies in the unit's region (bracket rows too: they
+    // are fewer and start no later), so its rounds need no capacity checks.
+    const bool fits = (uint64_t)nb + wave_rows <= cap_end;
     uint32_t overflow = 0;
 #pragma unroll
     for (int i = 0; i < ROUNDS; i++) {
@@ -679,7 +689,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
         const uint32_t cnt = (uint32_t)__popc(sti) | ((uint32_t)__popc(oci) << 16);
         const uint32_t incl = wave_incl_scan(cnt);
         const uint32_t excl = incl - cnt;
-        const uint32_t rtot = wave_read(incl, 63);
+        const uint32_t rtot = __builtin_amdgcn_readfirstlane(wave_read(incl, 63));
         const uint32_t idx0 = nb + (excl & 0xFFFFu);
         uint32_t midx = mb + (excl >> 16);
         const uint32_t pos0 = (uint32_t)(U.pos_base + lane_off + 1);
@@ -689,8 +699,8 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
         // than kStageRows of them, or one that reaches the end of the region,
         // is stored lane by lane.
         const uint32_t rn = rtot & 0xFFFFu;
-        const uint32_t nlm = want_nl ? (s_b5[w][i][lane] & ~br_r[i]) : 0u;
-        if (rn <= (uint32_t)kStageRows && (uint64_t)nb + rn <= cap_end) {
+        const uint32_t nlm = want_nl ? (s_b15[w][i][lane].y & ~br_r[i]) : 0u;
+        if (fits && rn <= (uint32_t)kStageRows) {
             // The lane's structurals go to slots [k, kh] of the stage, taken
             // from both ends of its mask at once (lowest bit to slot k, highest
             // to slot kh), so a wave runs half as many steps as its densest
@@ -754,9 +764,10 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
         // LDS and stored as contiguous runs when the round's rows fit.
         // Characters: (open, curly) -> ']' 0x5D, '[' 0x5B, '}' 0x7D, '{' 0x7B.
         m = oci;
-        const uint32_t b1 = s_b1[w][i][lane], b5 = s_b5[w][i][lane];
+        const uint2 b15 = s_b15[w][i][lane];
+        const uint32_t b1 = b15.x, b5 = b15.y;
         const uint32_t rm = rtot >> 16;
-        if (rn <= (uint32_t)kStageRows && (uint64_t)mb + rm <= cap_end) {
+        if (fits && rn <= (uint32_t)kStageRows) {
             // Both ends of the bracket mask at once, as for the structurals.
             // A stage entry holds the bracket's structural row within the
             // round (below 2^11) and its (open, curly) code in bits 16-17; the
@@ -820,19 +831,19 @@ void launch_variant(hipStream_t s, uint32_t grid, const LaunchBufs &b, const uin
                        b.tile_st, b.ls, b.uout, out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
 }
 
-// Register budgets: 5 waves per SIMD for 4 and 8 waves of 16 and 8 KiB (88
-// VGPRs, JSON Lines 96, no spills; measured 0.852 -> 0.811 ms per 2 GB against
-// 4 waves; 6 waves spill even with one round loaded ahead, 0.810 ms), 4 for
+// Register budgets: 6 waves per SIMD for 4 and 8 waves of 16 and 8 KiB (79
+// VGPRs, JSON Lines 80, no spills; per 2 GB 0.852 ms at 4 waves, 0.811 at 5,
+// 0.764 at 6 once the wave-uniform values moved to scalar registers), 4 for
 // 16 waves of 4 KiB; the 32 KiB waves of the 2-wave tile carry twice the
 // per-round counts and take a 3-wave budget.
 #ifndef CJX_WPE_T2
 #define CJX_WPE_T2 3
 #endif
 #ifndef CJX_WPE_T4
-#define CJX_WPE_T4 5
+#define CJX_WPE_T4 6
 #endif
 #ifndef CJX_WPE_T8
-#define CJX_WPE_T8 5
+#define CJX_WPE_T8 6
 #endif
 #ifndef CJX_WPE_T16
 #define CJX_WPE_T16 4
