@@ -123,6 +123,9 @@ __host__ __device__ __forceinline__ uint32_t alignbit(uint32_t hi, uint32_t lo, 
 #endif
 }
 
+// Four dwords at a dword-aligned address (one 16-byte global access).
+typedef uint32_t U32x4A4 __attribute__((ext_vector_type(4), aligned(4)));
+
 // Character classes of a lane's 32 bytes from its eight bit planes (R[k] bit
 // i = bit k of byte i), as products of plane bits (each a few 3-input bit
 // operations for all 32 bytes at once):
@@ -535,7 +538,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
     __shared__ uint32_t s_cnt0[WAVES], s_cnt_all[WAVES];
     __shared__ uint32_t s_fix[WAVES], s_nbase[WAVES], s_mbase[WAVES];
     __shared__ uint32_t s_stage[WAVES][kStageRows];  // one round's positions per wave
-    __shared__ uint8_t s_bchar[WAVES][kStageRows];   // and its bracket characters
+    __shared__ uint8_t s_bchar[WAVES][kStageRows];   // and its newline flags (JSON Lines)
     // Bracket characters as two bit masks per lane-round, kept in LDS until
     // compaction: byte bit 1 ('[' / '{' have it, ']' / '}' do not) and byte
     // bit 5 (curly), both restricted to bracket bytes.
@@ -732,7 +735,24 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            for (uint32_t t = lane; t < rn; t += 64) out_struct[nb + t] = s_stage[w][t];
+            if (rn >= 4u) {
+                // Four rows per lane and one 16-byte store each (global
+                // accesses of a dword or more need only dword alignment).  The
+                // last lane's group is moved back to end at row rn; the rows it
+                // shares with the lane before are written twice, with the same
+                // values.
+                for (uint32_t t = 4u * lane; t < rn; t += 256u) {
+                    const uint32_t s = min(t, rn - 4u);
+                    U32x4A4 v;
+                    v.x = s_stage[w][s];
+                    v.y = s_stage[w][s + 1u];
+                    v.z = s_stage[w][s + 2u];
+                    v.w = s_stage[w][s + 3u];
+                    *reinterpret_cast<U32x4A4 *>(out_struct + nb + s) = v;
+                }
+            } else {
+                for (uint32_t t = lane; t < rn; t += 64) out_struct[nb + t] = s_stage[w][t];
+            }
             if (want_nl)
                 for (uint32_t t = lane; t < rn; t += 64) row_nl[nb + t] = s_bchar[w][t];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -770,19 +790,17 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
         if (fits && rn <= (uint32_t)kStageRows) {
             // Both ends of the bracket mask at once, as for the structurals.
             // A stage entry holds the bracket's structural row within the
-            // round (below 2^11) and its (open, curly) code in bits 16-17; the
-            // row index and the character are formed when the run is stored.
+            // round (below 2^11) and, in bits 16-26, its byte offset within
+            // the round (lane * 32 + bit); the character is formed when the
+            // run is stored, from the (open, curly) masks of the lane that
+            // holds the byte.
             uint32_t k = excl >> 16, kh = k + (cnt >> 16) - 1u;
-            const uint32_t r0 = excl & 0xFFFFu;  // the lane's first row in the round
+            const uint32_t r0 = (excl & 0xFFFFu) + (lane << 21);
             while (m) {
                 const uint32_t jl = (uint32_t)__builtin_ctz(m), jz = (uint32_t)__builtin_clz(m);
                 const uint32_t jh = 31u - jz;
-                const uint32_t el = ((uint32_t)__popc(sti & ((1u << jl) - 1u)) + r0) +
-                                    (__builtin_amdgcn_ubfe(b1, jl, 1u) << 16) +
-                                    (__builtin_amdgcn_ubfe(b5, jl, 1u) << 17);
-                const uint32_t eh = ((uint32_t)__popc(sti & ((1u << jh) - 1u)) + r0) +
-                                    (__builtin_amdgcn_ubfe(b1, jh, 1u) << 16) +
-                                    (__builtin_amdgcn_ubfe(b5, jh, 1u) << 17);
+                const uint32_t el = ((uint32_t)__popc(sti & ((1u << jl) - 1u)) + r0) + (jl << 16);
+                const uint32_t eh = ((uint32_t)__popc(sti & ((1u << jh) - 1u)) + r0) + (jh << 16);
                 s_stage[w][k] = el;
                 s_stage[w][kh] = eh;
                 m = (m & (m - 1u)) & ~(0x80000000u >> jz);
@@ -794,8 +812,11 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             for (uint32_t t = lane; t < rm; t += 64) {
                 const uint32_t e = s_stage[w][t];
+                const uint2 bm = s_b15[w][i][e >> 21];
+                const uint32_t j = (e >> 16) & 31u;
+                const uint32_t sel = __builtin_amdgcn_ubfe(bm.x, j, 1u) | (__builtin_amdgcn_ubfe(bm.y, j, 1u) << 1);
                 oc_idx[mb + t] = nb + (e & 0xFFFFu);
-                oc_char[mb + t] = (uint8_t)perm(0u, 0x7B7D5B5Du, e >> 16);
+                oc_char[mb + t] = (uint8_t)perm(0u, 0x7B7D5B5Du, sel);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
