@@ -181,7 +181,7 @@ int ensure_work(cjx_context *ctx, const Shape &sh) {
               ctx->tile_st.ensure((size_t)sh.ntiles * 16) && ctx->oc_char.ensure(sh.rows + 16) &&
               ctx->oc_idx.ensure(sh.rows * 4 + 16) && ctx->keys.ensure(sh.rows * 2 + 16) &&
               ctx->sorted.ensure(sh.rows * 4 + 16) && ctx->spos.ensure(sh.rows * 4 + 16) &&
-              ctx->block_delta.ensure(512 * 4) && ctx->hist.ensure(512 * 256 * 4) &&
+              ctx->hist.ensure(512 * 256 * 4) &&
               ctx->ck.ensure(sh.chunks * sizeof(ChunkStat) + 16) &&
               ctx->gmin.ensure(sh.chunks * 4 + 16) && ctx->gmin1.ensure(sh.groups * 4 + 16) &&
               ctx->gmin2.ensure(sh.groups * 4 + 16) &&
@@ -262,7 +262,6 @@ LaunchBufs bufs_of(cjx_context *ctx) {
     b.sorted = ctx->sorted.as<uint32_t>();
     b.tmp = ctx->tmp.p ? ctx->tmp.as<uint32_t>() : nullptr;
     b.spos = ctx->spos.as<uint32_t>();
-    b.block_delta = ctx->block_delta.as<int32_t>();
     b.hist = ctx->hist.as<uint32_t>();
     b.ck = ctx->ck.as<ChunkStat>();
     b.gmin = ctx->gmin.as<int32_t>();
@@ -984,7 +983,7 @@ void cjx_context_destroy(cjx_context *ctx) {
     DevBuf *bufs[] = {&ctx->in,      &ctx->structural, &ctx->pair,   &ctx->units,
                       &ctx->uout,    &ctx->ls,         &ctx->tile_st, &ctx->layout,
                       &ctx->oc_char, &ctx->oc_idx,     &ctx->keys,   &ctx->sorted,
-                      &ctx->tmp,     &ctx->spos,       &ctx->block_delta, &ctx->hist,
+                      &ctx->tmp,     &ctx->spos,       &ctx->hist,
                       &ctx->ck,      &ctx->gmin,       &ctx->gmin1,  &ctx->gmin2,
                       &ctx->gagg,    &ctx->gd0,        &ctx->lows,   &ctx->highs,
                       &ctx->chunk_unit, &ctx->big_units, &ctx->slow_units, &ctx->unit_part,

@@ -93,7 +93,7 @@ struct cjx_context {
     PinnedBuf up[kUploadSlots];
     hipEvent_t up_ev[kUploadSlots] = {};
     uint64_t up_count = 0;
-    DevBuf oc_char, oc_idx, keys, sorted, tmp, spos, block_delta, hist;
+    DevBuf oc_char, oc_idx, keys, sorted, tmp, spos, hist;
     DevBuf ck, gmin, gmin1, gmin2, gagg, gd0, lows, highs;
     DevBuf row_nl;         // newline flag per structural row (record tables)
     bool want_nl = false;  // the launch being enqueued writes row_nl

@@ -109,6 +109,7 @@ struct LaunchState {
     uint32_t rec_overflow;  // more records than the caller's table holds
     uint32_t chunk_ticket;  // dynamic matching-chunk ticket (k_match_local)
     uint32_t n_slow;        // units that are not well nested (listed in slow_units)
+    uint32_t hist_ticket;   // blocks done with their histogram (sort path, one unit)
 };
 
 // One JSON Lines record (cjx_records.hip); same layout as cjx_record.
@@ -146,7 +147,6 @@ struct LaunchBufs {
     uint32_t *sorted;
     uint32_t *tmp;
     uint32_t *spos;
-    int32_t *block_delta;
     uint32_t *hist;
     ChunkStat *ck;    // per matching chunk
     int32_t *gmin;    // per chunk: lowest depth reached (d0 + mn)

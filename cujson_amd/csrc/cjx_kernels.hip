@@ -2,8 +2,9 @@
 //
 // Phases 1+2 (the fused tokenizer) live in cjx_tokenize.hip.
 // Phase 3 (parse_standard_json.cu:1345-1582) runs over the compact bracket
-// array: a segmented +1/-1 depth scan (k_depth_totals, k_depth_keys), a
-// stable LDS counting sort by depth key (k_scan_hist, k_scatter), and a
+// array: a segmented +1/-1 depth scan (k_depth_keys, from the chunk start
+// depths of the matching scan), a stable LDS counting sort by depth key
+// (k_depth_keys' last block scans the histograms, then k_scatter), and a
 // dense pair-row writer that applies the reference's group-of-four pairing
 // rule (k_pairs).
 #include "cjx_device.h"
@@ -308,12 +309,15 @@ __global__ __launch_bounds__(256) void k_assemble_docs(const UnitOut *__restrict
 
 // ---------------------------------------------------------------------------
 // Phase 3 helpers.  All phase-3 kernels split the unit's bracket range
-// [m_base, m_end) into gridDim.x contiguous block ranges the same way.
+// [m_base, m_end) into gridDim.x contiguous block ranges the same way: whole
+// matching chunks (kMatchChunk brackets) per block, so a block's start depth
+// is the start depth of its first chunk (ChunkStat::d0).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void block_range(uint32_t base, uint32_t end, uint32_t b, uint32_t G,
                                             uint32_t &b0, uint32_t &b1) {
     const uint32_t M = end - base;
-    const uint32_t chunk = (M + G - 1) / G;
+    const uint64_t per = ((uint64_t)M + G - 1) / G;
+    const uint64_t chunk = (per + kMatchChunk - 1) / kMatchChunk * kMatchChunk;
     const uint64_t lo = (uint64_t)base + (uint64_t)b * chunk;
     const uint64_t hi = lo + chunk;
     b0 = (uint32_t)(lo < end ? lo : end);
@@ -335,41 +339,65 @@ __device__ __forceinline__ int bracket_delta(uint8_t c) {
     return (c == 0x5B || c == 0x7B) ? 1 : ((c == 0x5D || c == 0x7D) ? -1 : 0);
 }
 
-// Sum of +1/-1 per block (depth_init_MathAPI, parse_standard_json.cu:1345-1358).
-__global__ __launch_bounds__(kP3Threads) void k_depth_totals(const uint8_t *__restrict__ oc_char,
-                                                             const UnitOut *__restrict__ uo,
-                                                             uint32_t u,
-                                                             int32_t *__restrict__ block_delta,
-                                                             const LaunchState *__restrict__ ls) {
-    __shared__ int32_t red[kP3Threads / 64];
-    // Capacity exceeded (ranges are not backed by buffers), or a well-nested
-    // unit that the matching kernels pair.
-    if (ls->overflow || uo[u].fast) return;
-    uint32_t b0, b1;
-    block_range(uo[u].m_base, uo[u].m_end, blockIdx.x, gridDim.x, b0, b1);
-    int32_t s = 0;
-    for (uint32_t i = b0 + threadIdx.x; i < b1; i += kP3Threads) s += bracket_delta(oc_char[i]);
-    s = block_reduce_sum(s, red);
-    if (threadIdx.x == 0) block_delta[blockIdx.x] = s;
+// Exclusive offsets per (block, digit) of the G blocks' histograms:
+// digit-major, block-minor, so that a stable scatter reproduces
+// thrust::stable_sort_by_key (parse_standard_json.cu:1546).  Run by the last
+// block of the kernel that wrote the histograms (all kP3Threads threads,
+// thread k for digit k); `wsum` is kP3Threads / 64 words of LDS.
+__device__ __forceinline__ void scan_hist_block(uint32_t *__restrict__ hist, uint32_t G,
+                                                uint32_t *wsum) {
+    static_assert(kP3Threads == 256, "one thread per digit");
+    const uint32_t k = threadIdx.x, w = k >> 6;
+    uint32_t total = 0;
+    for (uint32_t b = 0; b < G; b++) total += hist[b * 256 + k];
+    const uint32_t incl = wave_incl_scan(total);
+    if (lane_id() == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t run = incl - total;
+    for (uint32_t j = 0; j < w; j++) run += wsum[j];
+    for (uint32_t b = 0; b < G; b++) {
+        const uint32_t hv = hist[b * 256 + k];
+        hist[b * 256 + k] = run;
+        run += hv;
+    }
+}
+
+// Whether this block is the last of the grid to finish writing its
+// histogram (its writes and everyone's are then visible to it).  The last
+// block resets the ticket for the next histogram kernel of the launch.
+__device__ __forceinline__ bool last_hist_block(LaunchState *ls, uint32_t *flag) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) *flag = atomicAdd(&ls->hist_ticket, 1u) == gridDim.x - 1 ? 1u : 0u;
+    __syncthreads();
+    const bool last = *flag != 0;
+    if (last) {
+        __threadfence();
+        if (threadIdx.x == 0) ls->hist_ticket = 0;
+    }
+    return last;
 }
 
 // Depth keys (inclusive scan modulo 2^8 or 2^16, parse_standard_json.cu:1537;
 // openers take depth-1, :1540 / utils.cu:104-125) and the histogram of the
-// low key byte per block.
+// low key byte per block; the last block turns the histograms into scatter
+// offsets.  A block starts at the start depth of its first chunk, from the
+// matching scan (which runs for every unit, well nested or not).
 __global__ __launch_bounds__(kP3Threads) void k_depth_keys(
     const uint8_t *__restrict__ oc_char, const UnitOut *__restrict__ uo, uint32_t u,
-    uint32_t kmask, const int32_t *__restrict__ block_delta, uint16_t *__restrict__ keys,
-    uint32_t *__restrict__ hist, const LaunchState *__restrict__ ls) {
-    __shared__ int32_t red[kP3Threads / 64];
+    uint32_t kmask, const ChunkStat *__restrict__ ck, uint16_t *__restrict__ keys,
+    uint32_t *__restrict__ hist, LaunchState *__restrict__ ls) {
+    __shared__ uint32_t s_last;
     if (ls->overflow || uo[u].fast) return;
     __shared__ int32_t wsum[kP3Threads / 64];
     __shared__ uint32_t h[256];
     const uint32_t tid = threadIdx.x, w = tid >> 6;
+    const uint32_t mb = uo[u].m_base;
     uint32_t b0, b1;
-    block_range(uo[u].m_base, uo[u].m_end, blockIdx.x, gridDim.x, b0, b1);
-    int32_t pre = 0;
-    for (uint32_t k = tid; k < blockIdx.x; k += kP3Threads) pre += block_delta[k];
-    pre = block_reduce_sum(pre, red);
+    block_range(mb, uo[u].m_end, blockIdx.x, gridDim.x, b0, b1);
+    // A block past the first starts a chunk of a unit of more than
+    // kMatchChunk brackets, so the unit has chunks and their start depths.
+    int32_t pre = b0 > mb && b0 < b1 ? ck[uo[u].chunk0 + (b0 - mb) / kMatchChunk].d0 : 0;
     h[tid] = 0;
     __syncthreads();
     for (uint32_t base = b0; base < b1; base += kP3Threads) {
@@ -395,16 +423,19 @@ __global__ __launch_bounds__(kP3Threads) void k_depth_keys(
         __syncthreads();
     }
     hist[blockIdx.x * 256 + tid] = h[tid];
+    if (last_hist_block(ls, &s_last)) scan_hist_block(hist, gridDim.x, (uint32_t *)wsum);
 }
 
 // Histogram of key byte `shift` over elements taken in the order perm_in
-// (second pass of the 16-bit sort).
+// (second pass of the 16-bit sort); the last block turns the histograms into
+// scatter offsets.
 __global__ __launch_bounds__(kP3Threads) void k_hist(const uint16_t *__restrict__ keys,
                                                      const UnitOut *__restrict__ uo, uint32_t u,
                                                      const uint32_t *__restrict__ perm_in,
                                                      uint32_t shift, uint32_t *__restrict__ hist,
-                                                     const LaunchState *__restrict__ ls) {
+                                                     LaunchState *__restrict__ ls) {
     __shared__ uint32_t h[256];
+    __shared__ uint32_t s_last, wsum[kP3Threads / 64];
     if (ls->overflow || uo[u].fast) return;
     const uint32_t tid = threadIdx.x;
     uint32_t b0, b1;
@@ -415,28 +446,7 @@ __global__ __launch_bounds__(kP3Threads) void k_hist(const uint16_t *__restrict_
         atomicAdd(&h[(keys[perm_in[e]] >> shift) & 0xFFu], 1u);
     __syncthreads();
     hist[blockIdx.x * 256 + tid] = h[tid];
-}
-
-// Exclusive offsets per (block, digit): digit-major, block-minor, so that a
-// stable scatter reproduces thrust::stable_sort_by_key (parse_standard_json.cu:1546).
-__global__ __launch_bounds__(256) void k_scan_hist(uint32_t *__restrict__ hist, uint32_t G,
-                                                  const UnitOut *__restrict__ uo, uint32_t u,
-                                                  const LaunchState *__restrict__ ls) {
-    __shared__ uint32_t wsum[4];
-    if (ls->overflow || uo[u].fast) return;
-    const uint32_t k = threadIdx.x, w = k >> 6;
-    uint32_t total = 0;
-    for (uint32_t b = 0; b < G; b++) total += hist[b * 256 + k];
-    const uint32_t incl = wave_incl_scan(total);
-    if (lane_id() == 63) wsum[w] = incl;
-    __syncthreads();
-    uint32_t run = incl - total;
-    for (uint32_t j = 0; j < w; j++) run += wsum[j];
-    for (uint32_t b = 0; b < G; b++) {
-        const uint32_t hv = hist[b * 256 + k];
-        hist[b * 256 + k] = run;
-        run += hv;
-    }
+    if (last_hist_block(ls, &s_last)) scan_hist_block(hist, gridDim.x, wsum);
 }
 
 // Stable scatter by one key byte.  Ranks among equal digits inside a 256-wide
@@ -561,8 +571,8 @@ __global__ __launch_bounds__(kP3Threads) void k_pairs(
 namespace cjx {
 
 // Sort path of one unit in one block, for launches with several units (a
-// batch of documents, JSON Lines chunks): the steps of k_depth_totals /
-// k_depth_keys / k_scan_hist / k_scatter / k_pairs with the block-level
+// batch of documents, JSON Lines chunks): the steps of k_depth_keys /
+// scan_hist_block / k_scatter / k_pairs with the block-level
 // ranges replaced by one sequential pass per step.  Units that are well
 // nested (paired by cjx_match.hip) are skipped.
 template <bool U16>
@@ -872,11 +882,8 @@ hipError_t launch_unit_bases(hipStream_t s, const LaunchBufs &b, uint32_t n_unit
 hipError_t launch_phase3(hipStream_t s, const LaunchBufs &b, uint32_t u, uint32_t G,
                          uint32_t pair_grid, bool depth_u16, int32_t *pair_row) {
     const uint32_t kmask = depth_u16 ? 0xFFFFu : 0xFFu;
-    hipLaunchKernelGGL(k_depth_totals, dim3(G), dim3(kP3Threads), 0, s, b.oc_char, b.uout, u,
-                       b.block_delta, b.ls);
     hipLaunchKernelGGL(k_depth_keys, dim3(G), dim3(kP3Threads), 0, s, b.oc_char, b.uout, u, kmask,
-                       b.block_delta, b.keys, b.hist, b.ls);
-    hipLaunchKernelGGL(k_scan_hist, dim3(1), dim3(256), 0, s, b.hist, G, b.uout, u, b.ls);
+                       b.ck, b.keys, b.hist, b.ls);
     if (!depth_u16) {
         hipLaunchKernelGGL((k_scatter<true, true>), dim3(G), dim3(kP3Threads), 0, s, b.keys,
                            b.uout, u, 0u, b.hist, (const uint32_t *)nullptr, b.sorted, b.spos,
@@ -887,7 +894,6 @@ hipError_t launch_phase3(hipStream_t s, const LaunchBufs &b, uint32_t u, uint32_
                            b.ls);
         hipLaunchKernelGGL(k_hist, dim3(G), dim3(kP3Threads), 0, s, b.keys, b.uout, u, b.tmp, 8u,
                            b.hist, b.ls);
-        hipLaunchKernelGGL(k_scan_hist, dim3(1), dim3(256), 0, s, b.hist, G, b.uout, u, b.ls);
         hipLaunchKernelGGL((k_scatter<false, true>), dim3(G), dim3(kP3Threads), 0, s, b.keys,
                            b.uout, u, 8u, b.hist, b.tmp, b.sorted, b.spos, b.ls);
     }

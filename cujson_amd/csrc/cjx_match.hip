@@ -323,8 +323,8 @@ __device__ __forceinline__ void chunk_scan_small_block(uint32_t bid, UnitOut *__
 }
 
 // K3: matching inside one chunk.  Thread t owns the 16 brackets
-// [16t, 16t + 16) of the chunk and matches them with a stack kept in LDS
-// (column t, so the threads' pushes and pops do not conflict).  What is left
+// [16t, 16t + 16) of the chunk and matches them with a stack of 4-bit lane
+// indices packed in a 64-bit register (no LDS round trips).  What is left
 // is, per lane, closers that reach the lane's successive new lows (s_low) and
 // openers left open at the lane minimum, minimum + 1, ... (the stack).  Each
 // left-open opener is closed by the first later lane whose minimum reaches
@@ -367,11 +367,10 @@ __launch_bounds__(kMatchThreads) void k_match_local(
     // done, so no LDS of their own limits the blocks per CU.
     __shared__ __align__(16) uint8_t s_ch[kMatchChunk];
     __shared__ __align__(16) uint16_t s_part[kMatchChunk];
-    // Stack columns and low closers during matching; the row tile afterwards
-    // (16-byte aligned: the tile is filled and read as int4).
+    // Low closers during matching; the row tile afterwards (16-byte
+    // aligned: the tile is filled and read as int4).
     __shared__ __align__(16) union {
         struct {
-            uint16_t stk[kLaneBrackets][kMatchThreads];
             uint16_t low[kLaneBrackets][kMatchThreads];
         } m;
         int32_t row[kRowTile];
@@ -459,22 +458,29 @@ __launch_bounds__(kMatchThreads) void k_match_local(
             int32_t ds = d0 + incl - mine;
             for (uint32_t k = 0; k < w; k++) ds += s_wsum[k];
 
-            // Stack walk of the lane.
-            uint32_t sp = 0, nlow = 0;
+            // Stack walk of the lane, in registers: the open brackets' lane
+            // indices, 4 bits each, packed into a 64-bit stack (top in the low
+            // bits), and a mask of the curly ones; partners and low closers go
+            // to LDS, which the walk never reads back.  An opener and a closer
+            // pair up exactly when both are curly or both square (byte bit 5).
+            uint64_t stk = 0;
+            uint32_t sp = 0, nlow = 0, curly = 0;
             int32_t err = 0;
 #pragma unroll
             for (int k = 0; k < kLaneBrackets; k++) {
                 if ((uint32_t)k >= ne) break;
-                const uint8_t ch = (uint8_t)(cv[k >> 2] >> (8 * (k & 3)));
+                const uint32_t ch = cv[k >> 2] >> (8 * (k & 3));
                 const uint32_t e = e0 + k;
+                curly |= ((ch >> 5) & 1u) << k;
                 if (ch & 2u) {
-                    s_u.m.stk[sp][tid] = (uint16_t)e;
+                    stk = (stk << 4) | (uint64_t)k;
                     sp++;
                 } else if (sp) {
                     sp--;
-                    const uint32_t i = s_u.m.stk[sp][tid];
-                    if ((s_ch[i] ^ ch) != 6) err = 1;
-                    s_part[i] = (uint16_t)e;
+                    const uint32_t i = (uint32_t)stk & 15u;
+                    stk >>= 4;
+                    if ((curly >> i ^ curly >> k) & 1u) err = 1;
+                    s_part[e0 + i] = (uint16_t)e;
                 } else {
                     s_u.m.low[nlow][tid] = (uint16_t)e;
                     nlow++;
@@ -512,7 +518,8 @@ __launch_bounds__(kMatchThreads) void k_match_local(
             uint32_t L = tid + 1;
             for (int32_t j = (int32_t)sp - 1; j >= 0; j--) {
                 const int32_t d = dm + j;
-                const uint32_t i = s_u.m.stk[j][tid];
+                const uint32_t i = e0 + ((uint32_t)stk & 15u);
+                stk >>= 4;
                 while (L < kMatchThreads) {
                     if ((L & 15u) == 0 && s_gmin[L >> 4] > d) {
                         L += 16;

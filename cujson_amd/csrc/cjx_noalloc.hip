@@ -36,7 +36,7 @@ constexpr uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
 
 struct NaLayout {
     uint64_t units, tile_unit, uout, ls, tile_st, oc_char, oc_idx, keys, sorted, tmp, spos,
-        block_delta, hist, ck, gmin, gmin1, gmin2, gagg, gd0, lows, highs, chunk_unit,
+        hist, ck, gmin, gmin1, gmin2, gagg, gd0, lows, highs, chunk_unit,
         big_units, slow_units, unit_part, end;
 };
 
@@ -59,7 +59,6 @@ constexpr NaLayout make_layout() {
     L.sorted = take(kNaRows * 4 + 16);
     L.tmp = take(kNaRows * 4 + 16);
     L.spos = take(kNaRows * 4 + 16);
-    L.block_delta = take(512 * 4);
     L.hist = take(512 * 256 * 4);
     L.ck = take(kNaChunks * sizeof(ChunkStat) + 16);
     L.gmin = take(kNaChunks * 4 + 16);
@@ -194,7 +193,6 @@ int cjx_parse_device_noalloc(const uint8_t *d_input, size_t n, unsigned flags,
     b.sorted = reinterpret_cast<uint32_t *>(base + kNa.sorted);
     b.tmp = reinterpret_cast<uint32_t *>(base + kNa.tmp);
     b.spos = reinterpret_cast<uint32_t *>(base + kNa.spos);
-    b.block_delta = reinterpret_cast<int32_t *>(base + kNa.block_delta);
     b.hist = reinterpret_cast<uint32_t *>(base + kNa.hist);
     b.ck = reinterpret_cast<ChunkStat *>(base + kNa.ck);
     b.gmin = reinterpret_cast<int32_t *>(base + kNa.gmin);

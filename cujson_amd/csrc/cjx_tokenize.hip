@@ -487,8 +487,11 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
         const uint32_t rq = unescaped32(bs, q, ov);
         const uint32_t px = prefix_xor32(rq);
         const uint64_t QP = __ballot(__popc(rq) & 1u);
-        const uint32_t s_lane = s_run ^ (mbcnt64(QP) & 1u);
-        instr_r[i] = px ^ (s_lane ? ~0u : 0u);
+        // String state entering the lane: s_run plus the odd-quote lanes
+        // below it (the count's bit 0, sign-extended to a mask).
+        const uint32_t s_cnt = __builtin_amdgcn_mbcnt_hi((uint32_t)(QP >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)QP, s_run));
+        instr_r[i] = px ^ (uint32_t)__builtin_amdgcn_sbfe((int32_t)s_cnt, 0u, 1u);
         cnt_op0 += (uint32_t)__popc(op_r[i] & ~instr_r[i]);
         cnt_br0 += (uint32_t)__popc(br_r[i] & ~instr_r[i]);
         cnt_op += (uint32_t)__popc(op_r[i]);
