@@ -170,6 +170,99 @@ __device__ __forceinline__ void chunk_stats_blocks(uint32_t bid, uint32_t nblk,
 // whether the unit is well nested, (c) each chunk's start depth and minimum.
 constexpr int kScanThreads = 1024;
 constexpr int kScanBatch = 2;  // 64-chunk groups per wave step of the scan passes
+constexpr int kScanSlots = 12;  // groups per wave of the resident pass (units of <= 12288 chunks)
+
+// Running minimum / maximum over the lanes up to this one (DPP row shifts
+// and broadcasts; lanes without a source keep their own value).
+template <bool MAX>
+__device__ __forceinline__ int32_t wave_incl_minmax(int32_t x) {
+    auto step = [&](int32_t o) { x = MAX ? max(x, o) : min(x, o); };
+    step((int32_t)dpp<kDppRowShr1>((uint32_t)x, (uint32_t)x));
+    step((int32_t)dpp<kDppRowShr2>((uint32_t)x, (uint32_t)x));
+    step((int32_t)dpp<kDppRowShr4>((uint32_t)x, (uint32_t)x));
+    step((int32_t)dpp<kDppRowShr8>((uint32_t)x, (uint32_t)x));
+    step((int32_t)dpp<kDppRowBcast15, 0xA>((uint32_t)x, (uint32_t)x));
+    step((int32_t)dpp<kDppRowBcast31, 0xC>((uint32_t)x, (uint32_t)x));
+    return x;
+}
+
+// Resident pass of K2 for a unit of at most kScanThreads / 64 * kScanSlots
+// groups.  Wave w holds groups w, w + kW, ... (lane l: chunk 64 j + l), all
+// loaded at once.  Within a group the chunks' start depths are a DPP prefix
+// sum, and the group's walk is (sum of d, minimum and maximum over chunks of
+// start + mn / mx); the group scan is one pass over threads (thread j: group
+// j); chunk start depths and minima then come from the registers.  One
+// round trip to memory in all, and no LDS shuffles.
+__device__ __forceinline__ void chunk_scan_resident(uint32_t u, uint32_t c0, uint32_t nch, uint32_t g0,
+                                                    uint32_t h0, UnitOut *__restrict__ uo,
+                                                    ChunkStat *__restrict__ ck,
+                                                    int32_t *__restrict__ gmin,
+                                                    int32_t *__restrict__ gmin1,
+                                                    int32_t *__restrict__ gmin2,
+                                                    LaunchState *__restrict__ ls,
+                                                    uint32_t *__restrict__ slow_units) {
+    constexpr int kW = kScanThreads / 64;
+    __shared__ DepthAgg s_g[kW * kScanSlots];
+    __shared__ DepthAgg s_part[kW];
+    __shared__ int32_t s_gd0[kW * kScanSlots];
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = lane_id();
+    const uint32_t G = (nch + 63) / 64;
+    ChunkStat st[kScanSlots];
+#pragma unroll
+    for (int i = 0; i < kScanSlots; i++) {
+        const uint32_t j = w + kW * i, x = 64 * j + lane;
+        st[i] = (j < G && x < nch) ? ck[c0 + x] : ChunkStat{0, 0, 0, 0};
+    }
+    int32_t ex[kScanSlots], lo[kScanSlots];  // start depth in the group; start + mn
+#pragma unroll
+    for (int i = 0; i < kScanSlots; i++) {
+        const uint32_t j = w + kW * i;
+        if (j < G) {
+            const int32_t incl = (int32_t)wave_incl_scan((uint32_t)st[i].d);
+            ex[i] = incl - st[i].d;
+            lo[i] = ex[i] + st[i].mn;
+            const int32_t mn = wave_incl_minmax<false>(lo[i]);
+            const int32_t mx = wave_incl_minmax<true>(ex[i] + st[i].mx);
+            if (lane == 63) s_g[j] = DepthAgg{incl, mn, mx};
+        }
+    }
+    __syncthreads();
+    // Ordered scan of the groups, thread j holding group j.
+    const DepthAgg a = tid < G ? s_g[tid] : DepthAgg{0, 0, 0};
+    const DepthAgg incl = wave_scan_agg(a);
+    if (lane == 63) s_part[w] = incl;
+    __syncthreads();
+    DepthAgg pre{0, 0, 0};
+    for (uint32_t k = 0; k < w; k++) pre = agg_compose(pre, s_part[k]);
+    const DepthAgg up = shfl_agg_up(incl, 1);
+    const DepthAgg excl = lane ? agg_compose(pre, up) : pre;
+    int32_t m1 = INT32_MAX;
+    if (tid < G) {
+        s_gd0[tid] = excl.d;
+        m1 = excl.d + a.mn;
+        gmin1[g0 + tid] = m1;
+    }
+    if (tid == kScanThreads - 1) {
+        const DepthAgg t = agg_compose(pre, incl);
+        const bool fast = t.d == 0 && t.mn >= 0 && t.mx <= kMatchMaxDepth;
+        uo[u].fast = fast ? 1u : 0u;
+        if (!fast) list_slow(ls, slow_units, u);
+    }
+    // 4096-chunk minima: wave h holds groups [64 h, 64 h + 64).
+    for (int off = 32; off >= 1; off >>= 1) m1 = min(m1, __shfl_xor(m1, off, 64));
+    if (lane == 0 && w < (nch + 4095) / 4096) gmin2[h0 + w] = m1;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kScanSlots; i++) {
+        const uint32_t j = w + kW * i, x = 64 * j + lane;
+        if (j < G && x < nch) {
+            const int32_t gd = s_gd0[j];
+            ck[c0 + x].d0 = gd + ex[i];
+            gmin[c0 + x] = gd + lo[i];
+        }
+    }
+}
+
 __device__ __forceinline__ void chunk_scan_blocks(
     uint32_t bid, uint32_t nblk, UnitOut *__restrict__ uo, ChunkStat *__restrict__ ck, int32_t *__restrict__ gmin,
     int32_t *__restrict__ gmin1, int32_t *__restrict__ gmin2, DepthAgg *__restrict__ gagg,
@@ -184,6 +277,10 @@ __device__ __forceinline__ void chunk_scan_blocks(
     const uint32_t c0 = uo[u].chunk0, nch = uo[u].nchunks;
     const uint32_t g0 = uo[u].grp0, h0 = uo[u].sgrp0;
     const uint32_t G = (nch + 63) / 64, H = (nch + 4095) / 4096;
+    if (G <= (uint32_t)(kW * kScanSlots)) {
+        chunk_scan_resident(u, c0, nch, g0, h0, uo, ck, gmin, gmin1, gmin2, ls, slow_units);
+        continue;
+    }
     // (a) group walks: kScanBatch groups per wave step, their loads issued
     // together.
     for (uint32_t j0 = w; j0 < G; j0 += kScanBatch * kW) {
@@ -872,11 +969,28 @@ namespace {
 uint32_t chunk_grid(uint32_t chunk_bound, uint32_t cap) {
     return chunk_bound < cap ? (chunk_bound ? chunk_bound : 1u) : cap;
 }
+
+// Blocks of k_match_front the device holds at once.  Its chunk walks take
+// chunks bid, bid + g, ...: with a larger grid the last blocks would start
+// only when the first ones end, and the walks would take two turns.
+uint32_t front_capacity() {
+    static const uint32_t cap = [] {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_match_front, kMatchThreads, 0) !=
+                hipSuccess ||
+            cus <= 0 || per_cu <= 0)
+            return 2048u;
+        return (uint32_t)(cus * per_cu);
+    }();
+    return cap;
+}
 }  // namespace
 
 hipError_t launch_match_front(hipStream_t s, const LaunchBufs &b, uint32_t n_units,
                               uint32_t chunk_bound, int32_t *pair_row) {
-    const uint32_t g = chunk_grid(chunk_bound, 2048u);
+    const uint32_t g = chunk_grid(chunk_bound, front_capacity());
     const uint32_t gb = n_units < 256u ? n_units : 256u;
     hipLaunchKernelGGL(k_match_front, dim3(g + (n_units + 3) / 4), dim3(kMatchThreads), 0, s,
                        b.oc_char, b.oc_idx, b.uout, n_units, b.chunk_unit, b.ck, pair_row, b.ls, g,
