@@ -59,18 +59,27 @@ __device__ __forceinline__ DepthAgg shfl_agg_up(const DepthAgg &a, int off) {
     return o;
 }
 
-// Ordered reduction of a DepthAgg over a wave (lane order = walk order).
-__device__ __forceinline__ DepthAgg wave_reduce_agg(DepthAgg a) {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        DepthAgg o;
-        o.d = __shfl_xor(a.d, off, 64);
-        o.mn = __shfl_xor(a.mn, off, 64);
-        o.mx = __shfl_xor(a.mx, off, 64);
-        a = (lane & off) ? agg_compose(o, a) : agg_compose(a, o);
-    }
-    return a;
+// Running minimum / maximum over the lanes up to this one (DPP row shifts
+// and broadcasts; lanes without a source keep their own value).
+template <bool MAX>
+__device__ __forceinline__ int32_t wave_incl_minmax(int32_t x) {
+    auto step = [&](int32_t o) { x = MAX ? max(x, o) : min(x, o); };
+    step((int32_t)dpp<kDppRowShr1>((uint32_t)x, (uint32_t)x));
+    step((int32_t)dpp<kDppRowShr2>((uint32_t)x, (uint32_t)x));
+    step((int32_t)dpp<kDppRowShr4>((uint32_t)x, (uint32_t)x));
+    step((int32_t)dpp<kDppRowShr8>((uint32_t)x, (uint32_t)x));
+    step((int32_t)dpp<kDppRowBcast15, 0xA>((uint32_t)x, (uint32_t)x));
+    step((int32_t)dpp<kDppRowBcast31, 0xC>((uint32_t)x, (uint32_t)x));
+    return x;
+}
+
+// Ordered composition of a DepthAgg over a wave (lane order = walk order),
+// the total in lane 63: the lanes' start depths are a DPP prefix sum, and
+// the minimum / maximum are running ones of start + mn / start + mx.
+__device__ __forceinline__ DepthAgg wave_total_agg(const DepthAgg &a) {
+    const int32_t incl = (int32_t)wave_incl_scan((uint32_t)a.d);
+    const int32_t ex = incl - a.d;
+    return DepthAgg{incl, wave_incl_minmax<false>(ex + a.mn), wave_incl_minmax<true>(ex + a.mx)};
 }
 
 // Inclusive ordered scan of a DepthAgg over a wave.
@@ -148,8 +157,8 @@ __device__ __forceinline__ void chunk_stats_blocks(uint32_t bid, uint32_t nblk,
                 a.mx = max(a.mx, a.d);
             }
         }
-        a = wave_reduce_agg(a);
-        if (lane_id() == 0) part[t >> 6] = a;
+        a = wave_total_agg(a);
+        if (lane_id() == 63) part[t >> 6] = a;
         __syncthreads();
         if (t == 0) {
             DepthAgg r = part[0];
@@ -171,20 +180,6 @@ __device__ __forceinline__ void chunk_stats_blocks(uint32_t bid, uint32_t nblk,
 constexpr int kScanThreads = 1024;
 constexpr int kScanBatch = 2;  // 64-chunk groups per wave step of the scan passes
 constexpr int kScanSlots = 12;  // groups per wave of the resident pass (units of <= 12288 chunks)
-
-// Running minimum / maximum over the lanes up to this one (DPP row shifts
-// and broadcasts; lanes without a source keep their own value).
-template <bool MAX>
-__device__ __forceinline__ int32_t wave_incl_minmax(int32_t x) {
-    auto step = [&](int32_t o) { x = MAX ? max(x, o) : min(x, o); };
-    step((int32_t)dpp<kDppRowShr1>((uint32_t)x, (uint32_t)x));
-    step((int32_t)dpp<kDppRowShr2>((uint32_t)x, (uint32_t)x));
-    step((int32_t)dpp<kDppRowShr4>((uint32_t)x, (uint32_t)x));
-    step((int32_t)dpp<kDppRowShr8>((uint32_t)x, (uint32_t)x));
-    step((int32_t)dpp<kDppRowBcast15, 0xA>((uint32_t)x, (uint32_t)x));
-    step((int32_t)dpp<kDppRowBcast31, 0xC>((uint32_t)x, (uint32_t)x));
-    return x;
-}
 
 // Resident pass of K2 for a unit of at most kScanThreads / 64 * kScanSlots
 // groups.  Wave w holds groups w, w + kW, ... (lane l: chunk 64 j + l), all
@@ -218,12 +213,10 @@ __device__ __forceinline__ void chunk_scan_resident(uint32_t u, uint32_t c0, uin
     for (int i = 0; i < kScanSlots; i++) {
         const uint32_t j = w + kW * i;
         if (j < G) {
-            const int32_t incl = (int32_t)wave_incl_scan((uint32_t)st[i].d);
-            ex[i] = incl - st[i].d;
+            const DepthAgg g = wave_total_agg(DepthAgg{st[i].d, st[i].mn, st[i].mx});
+            ex[i] = (int32_t)g.d - st[i].d;  // lane's running sum less its own
             lo[i] = ex[i] + st[i].mn;
-            const int32_t mn = wave_incl_minmax<false>(lo[i]);
-            const int32_t mx = wave_incl_minmax<true>(ex[i] + st[i].mx);
-            if (lane == 63) s_g[j] = DepthAgg{incl, mn, mx};
+            if (lane == 63) s_g[j] = g;
         }
     }
     __syncthreads();
@@ -297,8 +290,8 @@ __device__ __forceinline__ void chunk_scan_blocks(
 #pragma unroll
         for (int i = 0; i < kScanBatch; i++) {
             const uint32_t j = j0 + i * kW;
-            const DepthAgg r = wave_reduce_agg(v[i]);
-            if (lane == 0 && j < G) gagg[g0 + j] = r;
+            const DepthAgg r = wave_total_agg(v[i]);
+            if (lane == 63 && j < G) gagg[g0 + j] = r;
         }
     }
     __syncthreads();
@@ -587,17 +580,9 @@ __launch_bounds__(kMatchThreads) void k_match_local(
             s_lmin[tid] = ne ? dm : INT32_MAX;
             s_lst[tid] = ds;
             // Lowest depth before this lane within the chunk (chunk start included).
-            int32_t pm = ne ? dm : INT32_MAX;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int32_t o = __shfl_up(pm, off, 64);
-                if (lane >= (uint32_t)off) pm = min(pm, o);
-            }
+            int32_t pm = wave_incl_minmax<false>(ne ? dm : INT32_MAX);
             if (lane == 63) s_wmin[w] = pm;
-            {
-                const int32_t o = __shfl_up(pm, 1, 64);
-                pm = lane ? o : INT32_MAX;
-            }
+            pm = (int32_t)dpp<kDppWaveShr1>((uint32_t)INT32_MAX, (uint32_t)pm);  // lanes before this one
             __syncthreads();
             pm = min(pm, d0);
             for (uint32_t k = 0; k < w; k++) pm = min(pm, s_wmin[k]);
