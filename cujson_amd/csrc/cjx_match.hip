@@ -855,9 +855,11 @@ __device__ __forceinline__ uint32_t wave_next_reaching(const int32_t *__restrict
 
 // K4: openers left open by their chunk, one wave per chunk and one lane per
 // depth.  A lane first looks at the next few chunks itself; openers closed
-// further away are then searched by the whole wave, one at a time.
+// further away are then searched by the whole wave, one at a time.  A block
+// is kCrossWaves waves, each taking its own chunks.
 constexpr uint32_t kNearChunks = 8;
-__global__ __launch_bounds__(64) void k_match_cross(
+constexpr uint32_t kCrossWaves = 4;
+__global__ __launch_bounds__(64 * kCrossWaves) void k_match_cross(
     const uint8_t *__restrict__ oc_char, const uint32_t *__restrict__ oc_idx,
     UnitOut *__restrict__ uo, const uint32_t *__restrict__ chunk_unit,
     const ChunkStat *__restrict__ ck, const int32_t *__restrict__ gmin, const int32_t *__restrict__ gmin1,
@@ -867,7 +869,8 @@ __global__ __launch_bounds__(64) void k_match_cross(
     if (ls->overflow) return;
     const uint32_t n_chunks = ls->n_chunks;
     const uint32_t lane = lane_id();
-    for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (uint32_t c = blockIdx.x * kCrossWaves + w; c < n_chunks; c += gridDim.x * kCrossWaves) {
         const uint32_t u = chunk_unit[c];
         if (!uo[u].fast) continue;
         const uint32_t c0 = uo[u].chunk0, nch = uo[u].nchunks;
@@ -955,20 +958,29 @@ uint32_t chunk_grid(uint32_t chunk_bound, uint32_t cap) {
     return chunk_bound < cap ? (chunk_bound ? chunk_bound : 1u) : cap;
 }
 
-// Blocks of k_match_front the device holds at once.  Its chunk walks take
-// chunks bid, bid + g, ...: with a larger grid the last blocks would start
-// only when the first ones end, and the walks would take two turns.
+// Blocks of `kernel` (kMatchThreads threads each) that the device holds at
+// once, or 2048 if the runtime cannot tell.
+uint32_t resident_blocks(const void *kernel) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kMatchThreads, 0) != hipSuccess ||
+        cus <= 0 || per_cu <= 0)
+        return 2048u;
+    return (uint32_t)(cus * per_cu);
+}
+
+// k_match_front's chunk walks take chunks bid, bid + g, ...: with a grid
+// above what the device holds at once, the last blocks would start only
+// when the first ones end, and the walks would take two turns.
 uint32_t front_capacity() {
-    static const uint32_t cap = [] {
-        int dev = 0, cus = 0, per_cu = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_match_front, kMatchThreads, 0) !=
-                hipSuccess ||
-            cus <= 0 || per_cu <= 0)
-            return 2048u;
-        return (uint32_t)(cus * per_cu);
-    }();
+    static const uint32_t cap = resident_blocks(reinterpret_cast<const void *>(k_match_front));
+    return cap;
+}
+// k_match_local takes chunks from a ticket counter: blocks beyond what the
+// device holds at once would only find the tickets gone.
+uint32_t local_capacity() {
+    static const uint32_t cap = resident_blocks(reinterpret_cast<const void *>(k_match_local));
     return cap;
 }
 }  // namespace
@@ -988,14 +1000,14 @@ hipError_t launch_match_front(hipStream_t s, const LaunchBufs &b, uint32_t n_uni
 
 hipError_t launch_match_local(hipStream_t s, const LaunchBufs &b, uint32_t chunk_bound,
                               int32_t *pair_row) {
-    const uint32_t g = chunk_grid(chunk_bound, 2048u);
-    const uint32_t gc = chunk_grid(chunk_bound, 8192u);
+    const uint32_t g = chunk_grid(chunk_bound, local_capacity());
+    const uint32_t gc = chunk_grid((chunk_bound + kCrossWaves - 1) / kCrossWaves, 2048u);
     // uo twice: the kernel's fields as read-only (scalar loads) and the
     // error flag it sets; it writes no field that it reads.
     hipLaunchKernelGGL(k_match_local, dim3(g), dim3(kMatchThreads), 0, s, b.oc_char, b.oc_idx,
                        b.uout, b.chunk_unit, b.ck, b.lows, b.highs, pair_row, b.ls,
                        (const UnitOut *)b.uout);
-    hipLaunchKernelGGL(k_match_cross, dim3(gc), dim3(64), 0, s, b.oc_char, b.oc_idx, b.uout,
+    hipLaunchKernelGGL(k_match_cross, dim3(gc), dim3(64 * kCrossWaves), 0, s, b.oc_char, b.oc_idx, b.uout,
                        b.chunk_unit, b.ck, b.gmin, b.gmin1, b.gmin2, b.lows, b.highs, pair_row,
                        b.ls);
     return hipGetLastError();
