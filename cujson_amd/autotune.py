@@ -138,13 +138,18 @@ def tune_tokenizer_shape(ctx, run: Callable[[], None], signature: str,
                          tiles: bool = True, small: bool = False,
                          tile_candidates: Sequence[int] = TILE_WAVES,
                          small_candidates: Sequence[int] = SMALL_WAVES,
-                         reps: int = 5, use_cache: bool = True
+                         reps: int = 5, use_cache: bool = True, rounds: int = 3
                          ) -> Tuple[Tuple[int, int], Dict[str, float]]:
     """Fastest tokenizer shape (tile_waves, small_waves) for the parse that
     run() performs, and the measured table {"tile_waves x small_waves": ms}.
     The tile shape is searched first with the small shape at its default,
     then the small shape with the best tile shape (`tiles` / `small` say
-    which kinds of unit the workload has).  The winner is left set on ctx."""
+    which kinds of unit the workload has).  The candidates of a search are
+    timed in `rounds` interleaved passes (each pass: two untimed calls after
+    the switch, then `reps` timed ones), and a candidate's time is the median
+    of its passes, so that a drift of the device over the search, or the
+    first calls after a switch, favour no candidate.  The winner is left set
+    on ctx."""
     key = f"tok-shape:{signature}"
     cache = _load_cache() if use_cache else {}
     if key in cache:
@@ -154,28 +159,24 @@ def tune_tokenizer_shape(ctx, run: Callable[[], None], signature: str,
         return best, dict(e["table"])
     table: Dict[str, float] = {}
 
-    def measure(tw: int, sw: int) -> float:
-        ctx.set_tokenizer_shape(tw, sw)
-        ms = time_device(ctx, run, reps)
-        table[f"{tw}x{sw}"] = round(ms, 4)
-        return ms
+    def search(shapes):
+        runs: Dict[Tuple[int, int], list] = {sh: [] for sh in shapes}
+        for _ in range(max(1, int(rounds))):
+            for sh in shapes:
+                ctx.set_tokenizer_shape(*sh)
+                run()
+                runs[sh].append(time_device(ctx, run, reps))
+        for sh, v in runs.items():
+            table[f"{sh[0]}x{sh[1]}"] = round(float(np.median(v)), 4)
+        return min(shapes, key=lambda sh: float(np.median(runs[sh])))
 
     best_t, best_s = ctx.tokenizer_shape()
-    best_ms = measure(best_t, best_s)
     if tiles:
-        for tw in tile_candidates:
-            if f"{tw}x{best_s}" not in table:
-                ms = measure(tw, best_s)
-                if ms < best_ms:
-                    best_ms, best_t = ms, tw
+        shapes = [(best_t, best_s)] + [(tw, best_s) for tw in tile_candidates if tw != best_t]
+        best_t, best_s = search(shapes)
     if small:
-        s0 = best_s
-        for sw in small_candidates:
-            if f"{best_t}x{sw}" not in table:
-                ms = measure(best_t, sw)
-                if ms < best_ms:
-                    best_ms, s0 = ms, sw
-        best_s = s0
+        shapes = [(best_t, best_s)] + [(best_t, sw) for sw in small_candidates if sw != best_s]
+        best_t, best_s = search(shapes)
     ctx.set_tokenizer_shape(best_t, best_s)
     if use_cache:
         cache[key] = {"best": [best_t, best_s], "table": table}
