@@ -480,6 +480,15 @@ __launch_bounds__(kMatchThreads) void k_match_local(
     // chunk starts and handed over once the chunk's loads are in, so its
     // round trip overlaps theirs.  (Fetch-and-increment: the compiler gives
     // it no wave-wide rewrite that would wait for the result at once.)
+    // With several chunks per block, blocks that start together stay in
+    // step: their tile writes come in bursts that the memory serves one
+    // after the other while the matching leaves it idle.  Blocks start in
+    // four groups about 7 us apart (s_sleep 127 = 8128 cycles, twice per
+    // step), which keeps the writes of some blocks under the matching of
+    // others (k_match_local 223 -> 215 us per 2 GB).  Small launches start
+    // at once.
+    if (n_chunks >= 4u * gridDim.x)
+        for (uint32_t k = 0; k < 2u * (blockIdx.x & 3u); k++) __builtin_amdgcn_s_sleep(127);
     if (tid == 0) s_next = atomicInc(&ls->chunk_ticket, 0xFFFFFFFFu);
     __syncthreads();
     for (uint32_t c = s_next; c < n_chunks;) {

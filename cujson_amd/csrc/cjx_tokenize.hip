@@ -256,6 +256,10 @@ __device__ __forceinline__ void publish_inc(unsigned long long *st, uint32_t til
 // / 0.811 ms (sleep 64); sleep 8 / 16 / 32 / 64 / 127 between polls (64-tile
 // windows) 0.883 (4 waves per SIMD) / 0.844 / 0.827 / 0.811 / 0.798 ms; a
 // first sleep of 127 before the first poll 0.794 -> 0.779 ms.
+#ifndef CJX_TOK_STAGGER
+#define CJX_TOK_STAGGER 2
+#endif
+
 #ifndef CJX_LB_PRESLEEP
 #define CJX_LB_PRESLEEP 127
 #endif
@@ -531,7 +535,8 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
     const uint32_t *__restrict__ tile_unit, const uint32_t *__restrict__ small_units,
     uint32_t n_small, unsigned long long *__restrict__ tile_st, LaunchState *__restrict__ ls,
     UnitOut *__restrict__ uo, uint32_t *__restrict__ out_struct, uint8_t *__restrict__ oc_char,
-    uint32_t *__restrict__ oc_idx, uint8_t *__restrict__ row_nl, uint32_t weak_utf8) {
+    uint32_t *__restrict__ oc_idx, uint8_t *__restrict__ row_nl, uint32_t weak_utf8,
+    uint32_t stagger_blocks) {
     constexpr int ROUNDS = SMALL ? SROUNDS : kTileBytes / (WAVES * kRoundBytes);
     constexpr uint64_t WAVE_BYTES = (uint64_t)ROUNDS * kRoundBytes;
     static_assert(SMALL || WAVES * ROUNDS * kRoundBytes == kTileBytes, "a tile is 64 KiB");
@@ -562,6 +567,13 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
         if (idx >= n_small) return;  // the whole wave; SMALL code has no workgroup barrier
         u = small_units[idx];
     } else {
+        // The first blocks of a long launch start in four groups about 7 us
+        // apart (s_sleep 127 = 8128 cycles, twice per step) before they draw
+        // a tile: blocks that start together stay in step, their reads, look-
+        // backs and writes in bursts.  (Tiles go in ticket order, so a block
+        // that sleeps only draws a later tile.)
+        if (blockIdx.x < stagger_blocks)
+            for (uint32_t k = 0; k < CJX_TOK_STAGGER * (blockIdx.x & 3u); k++) __builtin_amdgcn_s_sleep(127);
         if (tid == 0) {
             const uint32_t t = atomicAdd(&ls->tile_counter, 1u);
             s_tile = t;
@@ -847,12 +859,30 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
 
 namespace {
 
+// Workgroups of a kernel (of `threads` threads) that the device holds at
+// once, or 0 if the runtime cannot tell.
+uint32_t resident_blocks(const void *kernel, int threads) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+        cus <= 0 || per_cu <= 0)
+        return 0u;
+    return (uint32_t)(cus * per_cu);
+}
+
 template <bool JSONL, bool SMALL, int WAVES, int WPE, int SROUNDS = kTokRounds>
 void launch_variant(hipStream_t s, uint32_t grid, const LaunchBufs &b, const uint32_t *small_units,
                     const uint8_t *d_in, uint32_t n_small, uint32_t *out_struct, uint32_t weak) {
+    // Tile launches of at least four turns of the device stagger their
+    // first turn of blocks.
+    static const uint32_t resident = SMALL ? 0u : resident_blocks(
+        reinterpret_cast<const void *>(k_tokenize<JSONL, SMALL, WAVES, WPE, SROUNDS>), 64 * WAVES);
+    const uint32_t stagger = !SMALL && grid >= 4u * resident ? resident : 0u;
     hipLaunchKernelGGL((k_tokenize<JSONL, SMALL, WAVES, WPE, SROUNDS>), dim3(grid),
                        dim3(64 * WAVES), 0, s, d_in, b.units, b.tile_unit, small_units, n_small,
-                       b.tile_st, b.ls, b.uout, out_struct, b.oc_char, b.oc_idx, b.row_nl, weak);
+                       b.tile_st, b.ls, b.uout, out_struct, b.oc_char, b.oc_idx, b.row_nl, weak,
+                       stagger);
 }
 
 // Register budgets: 6 waves per SIMD for 4 and 8 waves of 16 and 8 KiB (79
