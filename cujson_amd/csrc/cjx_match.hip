@@ -116,7 +116,12 @@ __device__ __forceinline__ uint4 chunk_lane_bytes(const uint8_t *__restrict__ oc
 
 // K1: depth walk of every chunk.  Thread t walks brackets [16t, 16t + 16)
 // of a chunk straight from registers; the next chunk's bytes are loaded
-// before this one's walk is reduced.
+// before this one's walk is reduced.  The block's chunks are bid, bid + nblk,
+// ...; lane i of every wave resolves the unit and bracket range of the
+// block's chunk i up front (64 at a time), so the walk over them waits for
+// no chain of dependent loads per chunk.  (Loading the bytes of eight chunks
+// at once instead of one ahead measured slower: 25.9 against 21.4 us per
+// 2 GB.)
 __device__ __forceinline__ void chunk_stats_blocks(uint32_t bid, uint32_t nblk,
                                                    const uint8_t *__restrict__ oc_char,
                                                    const UnitOut *__restrict__ uo,
@@ -127,49 +132,52 @@ __device__ __forceinline__ void chunk_stats_blocks(uint32_t bid, uint32_t nblk,
     __shared__ DepthAgg part[kMatchThreads / 64];
     if (ls->overflow) return;
     const uint32_t n_chunks = ls->n_chunks;
-    const uint32_t t = threadIdx.x;
-    auto chunk_range = [&](uint32_t c, uint32_t &first, uint32_t &cnt) {
-        const uint32_t u = chunk_unit[c];
-        first = uo[u].m_base + (c - uo[u].chunk0) * kMatchChunk;
-        cnt = uo[u].m_end > first ? min((uint32_t)kMatchChunk, uo[u].m_end - first) : 0u;
-    };
-    uint32_t c = bid;
-    if (c >= n_chunks) return;
-    uint32_t first, cnt;
-    chunk_range(c, first, cnt);
-    uint4 cur = chunk_lane_bytes(oc_char, first, cnt, t);
-    for (;;) {
-        const uint32_t cn = c + nblk;
-        uint32_t first_n = 0, cnt_n = 0;
-        uint4 nxt = make_uint4(0u, 0u, 0u, 0u);
-        if (cn < n_chunks) {
-            chunk_range(cn, first_n, cnt_n);
-            nxt = chunk_lane_bytes(oc_char, first_n, cnt_n, t);
+    if (bid >= n_chunks) return;
+    const uint32_t t = threadIdx.x, lane = lane_id();
+    const uint32_t e0 = 16u * t;
+    const uint32_t nper = (n_chunks - bid + nblk - 1) / nblk;  // chunks of this block
+    for (uint32_t g0 = 0; g0 < nper; g0 += 64) {
+        // Lane i: chunk g0 + i of the block.
+        uint32_t fv = 0, cv_n = 0;
+        if (g0 + lane < nper) {
+            const uint32_t c = bid + (g0 + lane) * nblk;
+            const uint32_t u = chunk_unit[c];
+            fv = uo[u].m_base + (c - uo[u].chunk0) * kMatchChunk;
+            cv_n = uo[u].m_end > fv ? min((uint32_t)kMatchChunk, uo[u].m_end - fv) : 0u;
         }
-        DepthAgg a{0, 0, 0};
-        const uint32_t cv[4] = {cur.x, cur.y, cur.z, cur.w};
-        const uint32_t e0 = 16u * t;
-#pragma unroll
-        for (uint32_t k = 0; k < 16; k++) {
-            if (e0 + k < cnt) {
-                a.d += delta_of((uint8_t)(cv[k >> 2] >> (8 * (k & 3))));
-                a.mn = min(a.mn, a.d);
-                a.mx = max(a.mx, a.d);
+        const uint32_t gn = min(64u, nper - g0);
+        uint32_t first = __builtin_amdgcn_readlane(fv, 0), cnt = __builtin_amdgcn_readlane(cv_n, 0);
+        uint4 cur = chunk_lane_bytes(oc_char, first, cnt, t);
+        for (uint32_t i = 0; i < gn; i++) {
+            uint32_t cnt_n = 0;
+            uint4 nxt = make_uint4(0u, 0u, 0u, 0u);
+            if (i + 1 < gn) {
+                const uint32_t first_n = __builtin_amdgcn_readlane(fv, (int)(i + 1));
+                cnt_n = __builtin_amdgcn_readlane(cv_n, (int)(i + 1));
+                nxt = chunk_lane_bytes(oc_char, first_n, cnt_n, t);
             }
+            DepthAgg a{0, 0, 0};
+            const uint32_t cv[4] = {cur.x, cur.y, cur.z, cur.w};
+#pragma unroll
+            for (uint32_t k = 0; k < 16; k++) {
+                if (e0 + k < cnt) {
+                    a.d += delta_of((uint8_t)(cv[k >> 2] >> (8 * (k & 3))));
+                    a.mn = min(a.mn, a.d);
+                    a.mx = max(a.mx, a.d);
+                }
+            }
+            a = wave_total_agg(a);
+            if (lane == 63) part[t >> 6] = a;
+            __syncthreads();
+            if (t == 0) {
+                DepthAgg r = part[0];
+                for (int k = 1; k < kMatchThreads / 64; k++) r = agg_compose(r, part[k]);
+                ck[bid + (g0 + i) * nblk] = ChunkStat{r.d, r.mn, r.mx, 0};
+            }
+            __syncthreads();
+            cur = nxt;
+            cnt = cnt_n;
         }
-        a = wave_total_agg(a);
-        if (lane_id() == 63) part[t >> 6] = a;
-        __syncthreads();
-        if (t == 0) {
-            DepthAgg r = part[0];
-            for (int k = 1; k < kMatchThreads / 64; k++) r = agg_compose(r, part[k]);
-            ck[c] = ChunkStat{r.d, r.mn, r.mx, 0};
-        }
-        __syncthreads();
-        if (cn >= n_chunks) break;
-        c = cn;
-        cur = nxt;
-        cnt = cnt_n;
     }
 }
 
