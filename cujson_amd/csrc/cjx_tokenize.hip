@@ -446,10 +446,13 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
         // ---- phase 1: UTF-8 (reference rule) --------------------------------
         // Skipped for a round without high bytes whose carry has none either.
         // cu: the previous round's masks (lane 63 of them reaches lane 0).
+        // They are formed only in rounds that run the check: a skipped round
+        // has no high bytes (its masks are zero) and follows a round without
+        // any, whose masks were zero too, so cu is already right for it.
         if (weak_utf8) {
-            const Utf8Masks u = utf8_masks(R);
             const bool hi = __ballot(R[7] != 0u) != 0ull;
             if (hi || hi_prev) {
+                const Utf8Masks u = utf8_masks(R);
                 const Utf8Masks pm{wave_shr1_from(u.c, cu.c), wave_shr1_from(u.l, cu.l),
                                    wave_shr1_from(u.e, cu.e), wave_shr1_from(u.f, cu.f)};
                 uint32_t valid = 0x88888888u;
@@ -460,8 +463,8 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
                         if (lane_off + 4 * w4 < n) valid |= 8u << (4 * w4);
                 }
                 if (utf8_plane_error(R[7], u, pm, valid)) utf8_bad = 1;
+                cu = u;
             }
-            cu = u;
             hi_prev = hi;
         }
 
