@@ -477,7 +477,10 @@ __device__ __forceinline__ void scan_rounds(const uint8_t *ubase, uint64_t wave_
         b15[i][lane] = make_uint2(cls.b1, cls.b5);
 
         const uint32_t all_bs = (bs == 0xFFFFFFFFu);
-        const uint32_t run = (uint32_t)__clz(~bs);  // trailing backslash run of the lane
+        // Trailing backslash run of the lane.  Its parity is only read for
+        // lanes that are not all backslashes (~bs != 0), so the count leading
+        // zeros need not define the zero case.
+        const uint32_t run = (uint32_t)__builtin_clz(~bs);
         const uint64_t NB = __ballot(!all_bs);
         const uint64_t TP = __ballot(run & 1u);
         // Parity of the backslash run that ends just before the lane: the
