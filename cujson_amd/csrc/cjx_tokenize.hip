@@ -283,13 +283,18 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
 struct SpanCounts {
     uint32_t n0, n1, nsum, m0, m1, msum;
 };
+// (Selections by masks: written as conditionals, the compiler turned them
+// into indexed loads from a private copy of the span in scratch memory.)
+__device__ __forceinline__ uint32_t sel_mask(uint32_t a, uint32_t b, uint32_t bit) {
+    return a ^ ((a ^ b) & (0u - bit));  // bit ? b : a
+}
 __device__ __forceinline__ uint32_t span_n(const SpanCounts &f, uint32_t st) {
-    const uint32_t t = (st & 1u) ? f.n1 : f.n0;
-    return (st & 2u) ? f.nsum - t : t;
+    const uint32_t t = sel_mask(f.n0, f.n1, st & 1u);
+    return sel_mask(t, f.nsum - t, (st >> 1) & 1u);
 }
 __device__ __forceinline__ uint32_t span_m(const SpanCounts &f, uint32_t st) {
-    const uint32_t t = (st & 1u) ? f.m1 : f.m0;
-    return (st & 2u) ? f.msum - t : t;
+    const uint32_t t = sel_mask(f.m0, f.m1, st & 1u);
+    return sel_mask(t, f.msum - t, (st >> 1) & 1u);
 }
 
 // Decoupled look-back over windows of kLookLanes tiles (lane i reads tile
@@ -704,10 +709,16 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
     // are fewer and start no later), so its rounds need no capacity checks.
     const bool fits = (uint64_t)nb + wave_rows <= cap_end;
     uint32_t overflow = 0;
+    // The first row position of the wave (1-based) and the stage of the
+    // wave: uniform, computed once in scalar registers.
+    const uint32_t pos_w = __builtin_amdgcn_readfirstlane((uint32_t)(U.pos_base + wave_off + 1));
+    const uint32_t lane_pos = lane * kLaneBytes;
+    uint32_t *const stg = s_stage[w];
+    // Bracket characters by (open, curly) code: ']' 0x5D, '[' 0x5B, '}' 0x7D, '{' 0x7B.
+    const uint32_t br_chars = __builtin_amdgcn_readfirstlane(0x7B7D5B5Du);
 #pragma unroll
     for (int i = 0; i < ROUNDS; i++) {
         if (wave_off + (uint64_t)i * kRoundBytes >= U.n) break;  // no bytes, no rows
-        const uint64_t lane_off = wave_off + (uint64_t)i * kRoundBytes + (uint64_t)lane * kLaneBytes;
         const uint32_t sti = op_r[i] & ~(instr_r[i] ^ fix);
         const uint32_t oci = br_r[i] & sti;
         const uint32_t cnt = (uint32_t)__popc(sti) | ((uint32_t)__popc(oci) << 16);
@@ -716,7 +727,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
         const uint32_t rtot = __builtin_amdgcn_readfirstlane(wave_read(incl, 63));
         const uint32_t idx0 = nb + (excl & 0xFFFFu);
         uint32_t midx = mb + (excl >> 16);
-        const uint32_t pos0 = (uint32_t)(U.pos_base + lane_off + 1);
+        const uint32_t pos0 = (pos_w + (uint32_t)i * kRoundBytes) + lane_pos;
         uint32_t m = sti;
         // Structural positions.  The wave's positions of the round are
         // gathered in LDS and stored as one contiguous run; a round with more
@@ -731,12 +742,12 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
             // lane has structurals.  A last single bit is both ends: the same
             // value to the same slot twice.
             uint32_t k = excl & 0xFFFFu, kh = k + (cnt & 0xFFFFu) - 1u;
-            const uint32_t posh = pos0 + 31u;
+            const uint32_t posh = (pos_w + (uint32_t)i * kRoundBytes + 31u) + lane_pos;
             if (want_nl) {  // and the row's newline flag (in s_bchar, free until the brackets)
                 while (m) {
                     const uint32_t jl = (uint32_t)__builtin_ctz(m), jz = (uint32_t)__builtin_clz(m);
-                    s_stage[w][k] = pos0 + jl;
-                    s_stage[w][kh] = posh - jz;
+                    stg[k] = pos0 + jl;
+                    stg[kh] = posh - jz;
                     s_bchar[w][k] = (uint8_t)((nlm >> jl) & 1u);
                     s_bchar[w][kh] = (uint8_t)((nlm >> (31u - jz)) & 1u);
                     m = (m & (m - 1u)) & ~(0x80000000u >> jz);
@@ -746,8 +757,8 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
             } else {
                 while (m) {
                     const uint32_t jl = (uint32_t)__builtin_ctz(m), jz = (uint32_t)__builtin_clz(m);
-                    s_stage[w][k] = pos0 + jl;
-                    s_stage[w][kh] = posh - jz;
+                    stg[k] = pos0 + jl;
+                    stg[kh] = posh - jz;
                     m = (m & (m - 1u)) & ~(0x80000000u >> jz);
                     k++;
                     kh--;
@@ -765,14 +776,14 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
                 for (uint32_t t = 4u * lane; t < rn; t += 256u) {
                     const uint32_t s = min(t, rn - 4u);
                     U32x4A4 v;
-                    v.x = s_stage[w][s];
-                    v.y = s_stage[w][s + 1u];
-                    v.z = s_stage[w][s + 2u];
-                    v.w = s_stage[w][s + 3u];
+                    v.x = stg[s];
+                    v.y = stg[s + 1u];
+                    v.z = stg[s + 2u];
+                    v.w = stg[s + 3u];
                     *reinterpret_cast<U32x4A4 *>(out_struct + nb + s) = v;
                 }
             } else {
-                for (uint32_t t = lane; t < rn; t += 64) out_struct[nb + t] = s_stage[w][t];
+                for (uint32_t t = lane; t < rn; t += 64) out_struct[nb + t] = stg[t];
             }
             if (want_nl)
                 for (uint32_t t = lane; t < rn; t += 64) row_nl[nb + t] = s_bchar[w][t];
@@ -822,8 +833,8 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
                 const uint32_t jh = 31u - jz;
                 const uint32_t el = ((uint32_t)__popc(sti & ((1u << jl) - 1u)) + r0) + (jl << 16);
                 const uint32_t eh = ((uint32_t)__popc(sti & ((1u << jh) - 1u)) + r0) + (jh << 16);
-                s_stage[w][k] = el;
-                s_stage[w][kh] = eh;
+                stg[k] = el;
+                stg[kh] = eh;
                 m = (m & (m - 1u)) & ~(0x80000000u >> jz);
                 k++;
                 kh--;
@@ -832,12 +843,12 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             for (uint32_t t = lane; t < rm; t += 64) {
-                const uint32_t e = s_stage[w][t];
+                const uint32_t e = stg[t];
                 const uint2 bm = s_b15[w][i][e >> 21];
                 const uint32_t j = (e >> 16) & 31u;
                 const uint32_t sel = __builtin_amdgcn_ubfe(bm.x, j, 1u) | (__builtin_amdgcn_ubfe(bm.y, j, 1u) << 1);
                 oc_idx[mb + t] = nb + (e & 0xFFFFu);
-                oc_char[mb + t] = (uint8_t)perm(0u, 0x7B7D5B5Du, sel);
+                oc_char[mb + t] = (uint8_t)perm(0u, br_chars, sel);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
