@@ -709,16 +709,31 @@ __global__ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __launch_bounds__(64 *
     // are fewer and start no later), so its rounds need no capacity checks.
     const bool fits = (uint64_t)nb + wave_rows <= cap_end;
     uint32_t overflow = 0;
-    // The first row position of the wave (1-based) and the stage of the
-    // wave: uniform, computed once in scalar registers.
+    // The rounds that hold bytes (the others hold no rows), the first row
+    // position of the wave (1-based) and the stage of the wave: uniform,
+    // computed once in scalar registers.  (With 16 rounds the loop below is
+    // not unrolled when it ends at a round count, so it keeps the per-round
+    // test of the round's offset.)
+    const uint64_t wave_rem = U.n > wave_off ? U.n - wave_off : 0ull;
+    const uint32_t nrounds = __builtin_amdgcn_readfirstlane(
+        (uint32_t)min((uint64_t)ROUNDS, (wave_rem + kRoundBytes - 1) / kRoundBytes));
     const uint32_t pos_w = __builtin_amdgcn_readfirstlane((uint32_t)(U.pos_base + wave_off + 1));
     const uint32_t lane_pos = lane * kLaneBytes;
-    uint32_t *const stg = s_stage[w];
+    // (The stage's LDS address as one scalar, so an entry's address is one
+    // operation from its index.)
+    typedef __attribute__((address_space(3))) uint32_t LdsU32;
+    const uint32_t stg_addr =
+        __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(LdsU32 *)&s_stage[w][0]);
+    LdsU32 *const stg = (LdsU32 *)(uintptr_t)stg_addr;
     // Bracket characters by (open, curly) code: ']' 0x5D, '[' 0x5B, '}' 0x7D, '{' 0x7B.
     const uint32_t br_chars = __builtin_amdgcn_readfirstlane(0x7B7D5B5Du);
 #pragma unroll
     for (int i = 0; i < ROUNDS; i++) {
-        if (wave_off + (uint64_t)i * kRoundBytes >= U.n) break;  // no bytes, no rows
+        if constexpr (ROUNDS <= 8) {
+            if ((uint32_t)i >= nrounds) break;  // no bytes, no rows
+        } else {
+            if (wave_off + (uint64_t)i * kRoundBytes >= U.n) break;
+        }
         const uint32_t sti = op_r[i] & ~(instr_r[i] ^ fix);
         const uint32_t oci = br_r[i] & sti;
         const uint32_t cnt = (uint32_t)__popc(sti) | ((uint32_t)__popc(oci) << 16);
