@@ -105,6 +105,8 @@ __device__ __forceinline__ DepthAgg wave_scan_agg(DepthAgg a) {
 __device__ __forceinline__ uint4 chunk_lane_bytes(const uint8_t *__restrict__ oc_char,
                                                   uint32_t first, uint32_t cnt, uint32_t t) {
     const uint32_t e0 = 16u * t;
+    if ((first & 15u) == 0 && cnt == (uint32_t)kMatchChunk)  // wave-uniform: a full aligned chunk
+        return *reinterpret_cast<const uint4 *>(oc_char + first + e0);
     if ((first & 15u) == 0 && e0 + 16u <= cnt)
         return *reinterpret_cast<const uint4 *>(oc_char + first + e0);
     uint32_t w[4] = {0u, 0u, 0u, 0u};
@@ -112,6 +114,29 @@ __device__ __forceinline__ uint4 chunk_lane_bytes(const uint8_t *__restrict__ oc
     for (uint32_t k = 0; k < 16; k++)
         if (e0 + k < cnt) w[k >> 2] |= (uint32_t)oc_char[first + e0 + k] << (8 * (k & 3));
     return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Depth walk of a lane's 16 brackets when all 16 are there: one byte permute
+// per word turns its brackets into +1 / -1 bytes (bit 1 tells openers from
+// closers: selector byte 2 picks 0x01, selector byte 0 picks 0xFF), and the
+// running minimum and maximum take two depths per step.
+__device__ __forceinline__ DepthAgg lane_walk_full(uint4 v) {
+    const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+    int32_t d = 0, mn = 0, mx = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint32_t y = __builtin_amdgcn_perm(0u, 0x000100FFu, wd[q] & 0x02020202u);
+        const int32_t d1 = d + (int32_t)(int8_t)y;
+        const int32_t d2 = d1 + (int32_t)(int8_t)(y >> 8);
+        const int32_t d3 = d2 + (int32_t)(int8_t)(y >> 16);
+        const int32_t d4 = d3 + ((int32_t)y >> 24);
+        mn = min(mn, min(d1, d2));
+        mn = min(mn, min(d3, d4));
+        mx = max(mx, max(d1, d2));
+        mx = max(mx, max(d3, d4));
+        d = d4;
+    }
+    return DepthAgg{d, mn, mx};
 }
 
 // K1: depth walk of every chunk.  Thread t walks brackets [16t, 16t + 16)
@@ -130,6 +155,7 @@ __device__ __forceinline__ void chunk_stats_blocks(uint32_t bid, uint32_t nblk,
                                                    const LaunchState *__restrict__ ls) {
     static_assert(kMatchChunk == 16 * kMatchThreads, "16 brackets per thread");
     __shared__ DepthAgg part[kMatchThreads / 64];
+    __shared__ DepthAgg res[64];  // walks of the block's current 64 chunks
     if (ls->overflow) return;
     const uint32_t n_chunks = ls->n_chunks;
     if (bid >= n_chunks) return;
@@ -146,6 +172,33 @@ __device__ __forceinline__ void chunk_stats_blocks(uint32_t bid, uint32_t nblk,
             cv_n = uo[u].m_end > fv ? min((uint32_t)kMatchChunk, uo[u].m_end - fv) : 0u;
         }
         const uint32_t gn = min(64u, nper - g0);
+        // Wave composition, then the block's: chunk i's walk into res[i].
+        auto reduce = [&](DepthAgg a, uint32_t i) {
+            a = wave_total_agg(a);
+            if (lane == 63) part[t >> 6] = a;
+            __syncthreads();
+            if (t == 0) {
+                DepthAgg r = part[0];
+                for (int k = 1; k < kMatchThreads / 64; k++) r = agg_compose(r, part[k]);
+                res[i] = r;
+            }
+            __syncthreads();
+        };
+        const bool all_full = __builtin_amdgcn_ballot_w64(
+                                  lane < gn && ((fv & 15u) != 0 || cv_n != (uint32_t)kMatchChunk)) == 0;
+        if (all_full) {
+            // Every chunk of the group full and 16-byte aligned: one 16-byte
+            // load per lane and chunk with no branches around it, so the walk
+            // waits only for its own chunk (the last step loads its chunk again).
+            uint4 cur = *reinterpret_cast<const uint4 *>(oc_char + __builtin_amdgcn_readlane(fv, 0) + e0);
+            for (uint32_t i = 0; i < gn; i++) {
+                const uint32_t in = min(i + 1, gn - 1);
+                const uint4 nxt =
+                    *reinterpret_cast<const uint4 *>(oc_char + __builtin_amdgcn_readlane(fv, (int)in) + e0);
+                reduce(lane_walk_full(cur), i);
+                cur = nxt;
+            }
+        } else {
         uint32_t first = __builtin_amdgcn_readlane(fv, 0), cnt = __builtin_amdgcn_readlane(cv_n, 0);
         uint4 cur = chunk_lane_bytes(oc_char, first, cnt, t);
         for (uint32_t i = 0; i < gn; i++) {
@@ -157,26 +210,31 @@ __device__ __forceinline__ void chunk_stats_blocks(uint32_t bid, uint32_t nblk,
                 nxt = chunk_lane_bytes(oc_char, first_n, cnt_n, t);
             }
             DepthAgg a{0, 0, 0};
-            const uint32_t cv[4] = {cur.x, cur.y, cur.z, cur.w};
+            if (cnt == (uint32_t)kMatchChunk) {
+                a = lane_walk_full(cur);
+            } else {
+                const uint32_t cv[4] = {cur.x, cur.y, cur.z, cur.w};
 #pragma unroll
-            for (uint32_t k = 0; k < 16; k++) {
-                if (e0 + k < cnt) {
-                    a.d += delta_of((uint8_t)(cv[k >> 2] >> (8 * (k & 3))));
-                    a.mn = min(a.mn, a.d);
-                    a.mx = max(a.mx, a.d);
+                for (uint32_t k = 0; k < 16; k++) {
+                    if (e0 + k < cnt) {
+                        a.d += delta_of((uint8_t)(cv[k >> 2] >> (8 * (k & 3))));
+                        a.mn = min(a.mn, a.d);
+                        a.mx = max(a.mx, a.d);
+                    }
                 }
             }
-            a = wave_total_agg(a);
-            if (lane == 63) part[t >> 6] = a;
-            __syncthreads();
-            if (t == 0) {
-                DepthAgg r = part[0];
-                for (int k = 1; k < kMatchThreads / 64; k++) r = agg_compose(r, part[k]);
-                ck[bid + (g0 + i) * nblk] = ChunkStat{r.d, r.mn, r.mx, 0};
-            }
-            __syncthreads();
+            reduce(a, i);
             cur = nxt;
             cnt = cnt_n;
+        }
+        }
+        // The group's results are stored after its walks: a store among the
+        // walks' loads would make the next chunk's walk wait for every load
+        // in flight (stores and loads share one counter and may complete out
+        // of order), the prefetched chunk included.
+        if (t < gn) {
+            const DepthAgg r = res[t];
+            ck[bid + (g0 + t) * nblk] = ChunkStat{r.d, r.mn, r.mx, 0};
         }
     }
 }
