@@ -141,12 +141,10 @@ __device__ __forceinline__ DepthAgg lane_walk_full(uint4 v) {
 
 // K1: depth walk of every chunk.  Thread t walks brackets [16t, 16t + 16)
 // of a chunk straight from registers; the next chunk's bytes are loaded
-// before this one's walk is reduced.  The block's chunks are bid, bid + nblk,
-// ...; lane i of every wave resolves the unit and bracket range of the
-// block's chunk i up front (64 at a time), so the walk over them waits for
-// no chain of dependent loads per chunk.  (Loading the bytes of eight chunks
-// at once instead of one ahead measured slower: 25.9 against 21.4 us per
-// 2 GB.)
+// before this one is walked.  The block's chunks are bid, bid + nblk, ...;
+// lane i of every wave resolves the unit and bracket range of the block's
+// chunk i up front (64 at a time), so the walk over them waits for no chain
+// of dependent loads per chunk.
 __device__ __forceinline__ void chunk_stats_blocks(uint32_t bid, uint32_t nblk,
                                                    const uint8_t *__restrict__ oc_char,
                                                    const UnitOut *__restrict__ uo,
