@@ -321,16 +321,23 @@ __device__ __forceinline__ void chunk_scan_resident(uint32_t u, uint32_t c0, uin
 }
 
 __device__ __forceinline__ void chunk_scan_blocks(
-    uint32_t bid, uint32_t nblk, UnitOut *__restrict__ uo, ChunkStat *__restrict__ ck, int32_t *__restrict__ gmin,
+    uint32_t bid, uint32_t nblk, uint32_t n_units, UnitOut *__restrict__ uo, ChunkStat *__restrict__ ck,
+    int32_t *__restrict__ gmin,
     int32_t *__restrict__ gmin1, int32_t *__restrict__ gmin2, DepthAgg *__restrict__ gagg,
     int32_t *__restrict__ gd0, const uint32_t *__restrict__ big_units,
     LaunchState *__restrict__ ls, uint32_t *__restrict__ slow_units) {
     constexpr int kW = kScanThreads / 64;
     __shared__ DepthAgg part[kW];
-    if (ls->overflow) return;
-    for (uint32_t bi = bid; bi < ls->n_big; bi += nblk) {
+    // The flag, the count of big units and this block's first entry of the
+    // list are read at once (the list has room for n_units entries and
+    // bid < n_units; the entry is used only when bid < n_big).
+    const uint32_t overflow = ls->overflow, n_big = ls->n_big;
+    const uint32_t u_first = min(big_units[bid], n_units - 1);
+    asm volatile("" ::"s"(u_first));  // keeps that read here, beside the other two
+    if (overflow) return;
+    for (uint32_t bi = bid; bi < n_big; bi += nblk) {
     __syncthreads();  // part[] of the previous unit no longer read
-    const uint32_t u = big_units[bi], tid = threadIdx.x, w = tid >> 6, lane = lane_id();
+    const uint32_t u = bi == bid ? u_first : big_units[bi], tid = threadIdx.x, w = tid >> 6, lane = lane_id();
     const uint32_t c0 = uo[u].chunk0, nch = uo[u].nchunks;
     const uint32_t g0 = uo[u].grp0, h0 = uo[u].sgrp0;
     const uint32_t G = (nch + 63) / 64, H = (nch + 4095) / 4096;
@@ -1017,7 +1024,7 @@ __global__ __launch_bounds__(kScanThreads) void k_chunk_scans(
     DepthAgg *__restrict__ gagg, int32_t *__restrict__ gd0, const uint32_t *__restrict__ big_units,
     LaunchState *__restrict__ ls, uint32_t g_big, uint32_t *__restrict__ slow_units) {
     if (blockIdx.x < g_big)
-        chunk_scan_blocks(blockIdx.x, g_big, uo, ck, gmin, gmin1, gmin2, gagg, gd0, big_units, ls,
+        chunk_scan_blocks(blockIdx.x, g_big, n_units, uo, ck, gmin, gmin1, gmin2, gagg, gd0, big_units, ls,
                           slow_units);
     else
         chunk_scan_small_block(blockIdx.x - g_big, uo, n_units, ck, gmin, gmin1, gmin2, ls,
