@@ -388,7 +388,7 @@ __global__ __launch_bounds__(kP3Threads) void k_depth_keys(
     uint32_t kmask, const ChunkStat *__restrict__ ck, uint16_t *__restrict__ keys,
     uint32_t *__restrict__ hist, LaunchState *__restrict__ ls) {
     __shared__ uint32_t s_last;
-    if (ls->overflow || uo[u].fast) return;
+    if ((ls->overflow | uo[u].fast) != 0u) return;  // both read at once (u < n_units)
     __shared__ int32_t wsum[kP3Threads / 64];
     __shared__ uint32_t h[256];
     const uint32_t tid = threadIdx.x, w = tid >> 6;
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(kP3Threads) void k_hist(const uint16_t *__restrict_
                                                      LaunchState *__restrict__ ls) {
     __shared__ uint32_t h[256];
     __shared__ uint32_t s_last, wsum[kP3Threads / 64];
-    if (ls->overflow || uo[u].fast) return;
+    if ((ls->overflow | uo[u].fast) != 0u) return;  // both read at once (u < n_units)
     const uint32_t tid = threadIdx.x;
     uint32_t b0, b1;
     block_range(uo[u].m_base, uo[u].m_end, blockIdx.x, gridDim.x, b0, b1);
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(kP3Threads) void k_scatter(
     uint32_t *__restrict__ perm_out, uint32_t *__restrict__ spos,
     const LaunchState *__restrict__ ls) {
     __shared__ uint32_t run[256];
-    if (ls->overflow || uo[u].fast) return;
+    if ((ls->overflow | uo[u].fast) != 0u) return;  // both read at once (u < n_units)
     __shared__ uint32_t wcnt[kP3Threads / 64][256];
     const uint32_t tid = threadIdx.x, w = tid >> 6;
     const uint64_t lt = lanemask_lt();
@@ -515,7 +515,7 @@ __global__ __launch_bounds__(kP3Threads) void k_pairs(
     UnitOut *__restrict__ uo, uint32_t u, int32_t *__restrict__ pair_row,
     const LaunchState *__restrict__ ls) {
     __shared__ int32_t row[kPairChunk];
-    if (ls->overflow || uo[u].fast) return;
+    if ((ls->overflow | uo[u].fast) != 0u) return;  // both read at once (u < n_units)
     __shared__ uint32_t s_m0, s_m1;
     __shared__ int32_t red[kP3Threads / 64];
     const uint32_t tid = threadIdx.x;
